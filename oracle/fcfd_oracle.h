@@ -1,0 +1,123 @@
+/* fcfd_oracle.h -- CPU restatement of the reference's per-timestep cell and interface update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product (fortrancfd_amd/) never does.
+ *
+ * Every function states the reference routine it restates (paths relative to the reference root;
+ * CKLIB/TRANLIB are WORK/KineticModels/Chem_bin/GRI3.0_CH4/{cklib,tranlib}.f).  Units follow the
+ * reference: the thermo.f90-level functions are SI, the CK* / MC* level functions are cgs.
+ * Layout of per-cell vectors: q[0]=rho, q[1]=u (or rho*u), q[2]=T (or rho*E), q[3+k]=Y_k (rho*Y_k),
+ * i.e. the reference's (nequ) column, equation index fastest (SRC/variables.f90:6-16).
+ */
+#ifndef FCFD_ORACLE_H
+#define FCFD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fo_mech fo_mech;
+
+/* Mechanism from CHEMKIN-II link files (records as read by CKINIT cklib.f:3972-4170 and MCINIT
+ * tranlib.f:93-326).  tran_bin may be NULL.  Returns NULL on failure (message on stderr). */
+fo_mech *fo_mech_load(const char *chem_bin, const char *tran_bin);
+void fo_mech_free(fo_mech *m);
+int fo_mech_kk(const fo_mech *m);
+int fo_mech_ii(const fo_mech *m);
+/* Export of the parsed tables, in CKINIT's own units and orders, for pinning against CKINIT. */
+void fo_mech_weights(const fo_mech *m, double *wt);               /* [KK] g/mol          */
+void fo_mech_nasa(const fo_mech *m, double *tt, double *aa);      /* [KK*3], [KK*2*7]    */
+void fo_mech_arrhenius(const fo_mech *m, double *par, int *nu, int *nk, int *ns, int *nr);
+
+/* ---- CKLIB species / mixture thermo (cgs, molar) ---------------------------------------- */
+void fo_ckhml(const fo_mech *m, double T, double *hml);   /* CKHML  cklib.f:3500 */
+void fo_ckuml(const fo_mech *m, double T, double *uml);   /* CKUML  cklib.f:8409 */
+void fo_ckcpml(const fo_mech *m, double T, double *cpml); /* CKCPML cklib.f:1811 */
+void fo_cksmh(const fo_mech *m, double T, double *smh);   /* CKSMH  cklib.f:7501 */
+void fo_ckhms(const fo_mech *m, double T, double *hms);   /* CKHMS  cklib.f:3562 */
+void fo_ckytx(const fo_mech *m, const double *Y, double *X); /* CKYTX cklib.f:9587 */
+void fo_ckxty(const fo_mech *m, const double *X, double *Y); /* CKXTY cklib.f:9421 */
+void fo_adjust_sum(int n, double *v);                     /* SRC/thermo.f90:136-149 */
+
+/* ---- thermo.f90 wrappers (SI) -------------------------------------------------------------- */
+double fo_pressure(const fo_mech *m, double rho, double T, const double *X);     /* :166 */
+double fo_density(const fo_mech *m, double p, double T, const double *X);        /* :151 */
+double fo_specific_energy(const fo_mech *m, double T, const double *X);          /* :356 */
+double fo_cp(const fo_mech *m, double T, const double *X);                       /* :324 */
+double fo_cv(const fo_mech *m, double T, const double *X);                       /* :304 */
+double fo_sonic_speed(const fo_mech *m, double rho, double p, double T, const double *X); /* :421 */
+double fo_temperature_con(const fo_mech *m, double e, const double *X);          /* :211 */
+double fo_heat_release_rate(const fo_mech *m, const double *Y0, const double *Y, double rho,
+                            double dt);                                          /* :83  */
+double fo_heat_release_rate_omegadot(const fo_mech *m, const double *Y, double rho, double T);
+
+/* ---- eos.f90 --------------------------------------------------------------------------------- */
+void fo_qf_to_qc(const fo_mech *m, const double *qf, double *Qc);   /* SRC/eos.f90:11-46 */
+void fo_qc_to_qf(const fo_mech *m, const double *Qc, double *qf);   /* SRC/eos.f90:49-83 */
+
+/* ---- reconstruction (SRC/weno.f90, SRC/muscl.f90) ------------------------------------------- */
+void fo_weno3z(const double *s, double *sl, double *sr);
+void fo_weno5(const double *s, double *sl, double *sr, int flag_weno);
+void fo_weno7z(const double *s, double *sl, double *sr);
+void fo_muscl3(const double *s, double *sl, double *sr, int flag_muscl);
+
+typedef struct {
+    int flag_acu, flag_weno, flag_muscl, select_riemann, select_time;
+    int is_spherical, do_hydro, do_tran;
+    double dx_comp;
+} fo_opts;
+
+/* Number of ghost layers for a scheme (SRC/params.f90:161-173). */
+int fo_lvir(const fo_opts *o);
+
+/* calc_flux_advection (SRC/flux.f90:130-373) over one block of n interior cells.
+ * qf: (n + 2*lv) cells x nequ, first cell = ista-lv.  Fplus/godunov: (n+1) interfaces
+ * (ista-1 .. iend) x nequ / x 3.  Fplus is ADDED to (caller zeroes it, as TVDRK3 does). */
+void fo_flux_advection(const fo_mech *m, const fo_opts *o, int n, int lv, double dx_block,
+                       const double *qf, double *Fplus, double *godunov);
+
+/* calc_flux_tran (SRC/flux.f90:18-127) over one block, with the Wilke viscosity MCAVIS in place of
+ * the reference's undefined MCAVIS_EA (SRC/trans.f90:148).  Qc: (n+2*lv) x nequ; Fplus added to. */
+void fo_flux_tran(const fo_mech *m, int n, int lv, double dx_block, const double *Qc,
+                  double *Fplus);
+
+/* TRANLIB mixture-averaged properties (cgs). */
+double fo_mcavis(const fo_mech *m, double T, const double *X);                  /* :681 */
+double fo_mcacon(const fo_mech *m, double T, const double *X);                  /* :827 */
+void fo_mcadif(const fo_mech *m, double p, double T, const double *X, double *D); /* :973 */
+
+/* ---- multi-block wall-bounded domain (reference storage: per block (ncell+2lv) x nequ) ------ */
+typedef struct {
+    const fo_mech *m;
+    fo_opts o;
+    int nblk, ncell, lv, nequ;
+    double dx;
+    double *Qc, *qf;     /* nblk * (ncell+2lv) * nequ each */
+    double *Fplus;       /* nblk * (ncell+1) * nequ        */
+    double *godunov;     /* nblk * (ncell+1) * 3           */
+    double *Qc0, *k1, *k2, *k3, *k4;  /* interior work   */
+} fo_domain;
+
+/* Allocates and initialises from (T, p, u, X) per interior cell the way initialize_variables /
+ * set_block_params / main_sim do (SRC/params.f90:305-580, SRC/block.f90:394-431,
+ * SRC/main.f90:94-114). cells: nblk*ncell rows of (3+KK). */
+fo_domain *fo_domain_create(const fo_mech *m, const fo_opts *o, int nblk, int ncell, double dx,
+                            const double *cells);
+void fo_domain_free(fo_domain *d);
+double fo_calc_dt(fo_domain *d, double cfl);   /* SRC/update.f90:13-59, single rank */
+void fo_solve_fluid(fo_domain *d, double dt);  /* SRC/update.f90:62-97 -> timeintegration.f90 */
+void fo_refresh(fo_domain *d);                 /* post-step BC/halo/QcToqf, SRC/main.f90:149-163 */
+
+/* ---- CKLIB kinetics -------------------------------------------------------------------------- */
+/* CKWYR (cklib.f:9107-9204): rho g/cm^3, T K, Y -> wdot mol/(cm^3 s).  qf_/qr_ (may be NULL):
+ * per-reaction forward / reverse rates of progress as left in RCKWRK(NcI1/NcI2). */
+void fo_ckwyr(const fo_mech *m, double rho, double T, const double *Y, double *wdot,
+              double *qf_, double *qr_);
+/* CKCDYR (cklib.f:1422-1541). */
+void fo_ckcdyr(const fo_mech *m, double rho, double T, const double *Y, double *cdot,
+               double *ddot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

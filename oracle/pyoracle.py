@@ -1,0 +1,274 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libfcfd_oracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfcfd_oracle.so")
+REPO = os.path.dirname(HERE)
+MECH_DIR = os.path.join(REPO, "fortrancfd_amd", "data", "GRI3.0_CH4")
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+
+
+def _lib():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    lib.fo_mech_load.restype = C.c_void_p
+    lib.fo_mech_load.argtypes = [C.c_char_p, C.c_char_p]
+    for name in ["fo_pressure", "fo_density", "fo_specific_energy", "fo_cp", "fo_cv",
+                 "fo_sonic_speed", "fo_temperature_con", "fo_heat_release_rate",
+                 "fo_heat_release_rate_omegadot", "fo_mcavis", "fo_mcacon", "fo_calc_dt"]:
+        getattr(lib, name).restype = C.c_double
+    lib.fo_domain_create.restype = C.c_void_p
+    return lib
+
+
+class FoOpts(C.Structure):
+    _fields_ = [("flag_acu", C.c_int), ("flag_weno", C.c_int), ("flag_muscl", C.c_int),
+                ("select_riemann", C.c_int), ("select_time", C.c_int), ("is_spherical", C.c_int),
+                ("do_hydro", C.c_int), ("do_tran", C.c_int), ("dx_comp", C.c_double)]
+
+
+class FoDomain(C.Structure):
+    _fields_ = [("m", C.c_void_p), ("o", FoOpts), ("nblk", C.c_int), ("ncell", C.c_int),
+                ("lv", C.c_int), ("nequ", C.c_int), ("dx", C.c_double),
+                ("Qc", _D), ("qf", _D), ("Fplus", _D), ("godunov", _D)]
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Oracle:
+    """CPU restatement of the reference path on one mechanism."""
+
+    def __init__(self, mech_dir: str = MECH_DIR, transport: bool = True):
+        self.lib = _lib()
+        chem = os.path.join(mech_dir, "chem.bin").encode()
+        tran = os.path.join(mech_dir, "tran.bin").encode() if transport else None
+        self.m = C.c_void_p(self.lib.fo_mech_load(chem, tran))
+        if not self.m:
+            raise RuntimeError("oracle: mechanism load failed")
+        self.kk = self.lib.fo_mech_kk(self.m)
+        self.ii = self.lib.fo_mech_ii(self.m)
+        self.nequ = 3 + self.kk
+
+    def __del__(self):
+        try:
+            self.lib.fo_mech_free(self.m)
+        except Exception:
+            pass
+
+    # ---- tables ---------------------------------------------------------------------------
+    def weights(self):
+        w = np.zeros(self.kk)
+        self.lib.fo_mech_weights(self.m, _p(w))
+        return w
+
+    def nasa(self):
+        tt = np.zeros(self.kk * 3)
+        aa = np.zeros(self.kk * 2 * 7)
+        self.lib.fo_mech_nasa(self.m, _p(tt), _p(aa))
+        return tt.reshape(self.kk, 3), aa.reshape(self.kk, 2, 7)
+
+    def arrhenius(self):
+        par = np.zeros(self.ii * 4)
+        nu = np.zeros(self.ii * 6, np.int32)
+        nk = np.zeros(self.ii * 6, np.int32)
+        ns = np.zeros(self.ii, np.int32)
+        nr = np.zeros(self.ii, np.int32)
+        self.lib.fo_mech_arrhenius(self.m, _p(par), nu.ctypes.data_as(_I), nk.ctypes.data_as(_I),
+                                   ns.ctypes.data_as(_I), nr.ctypes.data_as(_I))
+        return par.reshape(self.ii, 4), nu.reshape(self.ii, 6), nk.reshape(self.ii, 6), ns, nr
+
+    # ---- species / mixture thermo -----------------------------------------------------------
+    def _species(self, fn, T):
+        out = np.zeros(self.kk)
+        getattr(self.lib, fn)(self.m, C.c_double(T), _p(out))
+        return out
+
+    def hml(self, T): return self._species("fo_ckhml", T)
+    def uml(self, T): return self._species("fo_ckuml", T)
+    def cpml(self, T): return self._species("fo_ckcpml", T)
+    def smh(self, T): return self._species("fo_cksmh", T)
+    def hms(self, T): return self._species("fo_ckhms", T)
+
+    def ytx(self, Y):
+        Y = _f64(Y)
+        X = np.zeros(self.kk)
+        self.lib.fo_ckytx(self.m, _p(Y), _p(X))
+        return X
+
+    def xty(self, X):
+        X = _f64(X)
+        Y = np.zeros(self.kk)
+        self.lib.fo_ckxty(self.m, _p(X), _p(Y))
+        return Y
+
+    def adjust_sum(self, v):
+        v = _f64(v).copy()
+        self.lib.fo_adjust_sum(C.c_int(v.size), _p(v))
+        return v
+
+    def pressure(self, rho, T, X):
+        return self.lib.fo_pressure(self.m, C.c_double(rho), C.c_double(T), _p(_f64(X)))
+
+    def density(self, p, T, X):
+        return self.lib.fo_density(self.m, C.c_double(p), C.c_double(T), _p(_f64(X)))
+
+    def specific_energy(self, T, X):
+        return self.lib.fo_specific_energy(self.m, C.c_double(T), _p(_f64(X)))
+
+    def cp(self, T, X): return self.lib.fo_cp(self.m, C.c_double(T), _p(_f64(X)))
+    def cv(self, T, X): return self.lib.fo_cv(self.m, C.c_double(T), _p(_f64(X)))
+
+    def sonic_speed(self, rho, p, T, X):
+        return self.lib.fo_sonic_speed(self.m, C.c_double(rho), C.c_double(p), C.c_double(T),
+                                       _p(_f64(X)))
+
+    def temperature_con(self, e, X):
+        return self.lib.fo_temperature_con(self.m, C.c_double(e), _p(_f64(X)))
+
+    def heat_release_rate(self, Y0, Y, rho, dt):
+        return self.lib.fo_heat_release_rate(self.m, _p(_f64(Y0)), _p(_f64(Y)), C.c_double(rho),
+                                             C.c_double(dt))
+
+    def heat_release_rate_omegadot(self, Y, rho, T):
+        return self.lib.fo_heat_release_rate_omegadot(self.m, _p(_f64(Y)), C.c_double(rho),
+                                                      C.c_double(T))
+
+    # ---- eos ------------------------------------------------------------------------------
+    def qf_to_qc(self, qf):
+        qf = _f64(qf)
+        out = np.zeros_like(qf)
+        for r in range(qf.shape[0]):
+            self.lib.fo_qf_to_qc(self.m, _p(qf[r]), out[r].ctypes.data_as(_D))
+        return out
+
+    def qc_to_qf(self, Qc):
+        Qc = _f64(Qc)
+        out = np.zeros_like(Qc)
+        for r in range(Qc.shape[0]):
+            self.lib.fo_qc_to_qf(self.m, _p(Qc[r]), out[r].ctypes.data_as(_D))
+        return out
+
+    # ---- reconstruction -------------------------------------------------------------------
+    def recon(self, s):
+        """s: (n, 8) stencils -> (n, 18) in the harness order (weno3z, weno5 js, weno5 z, weno7z,
+        muscl3 flags 1..4; left then right)."""
+        s = _f64(s)
+        out = np.zeros((s.shape[0], 18))
+        l, r = C.c_double(), C.c_double()
+        for i in range(s.shape[0]):
+            row = s[i]
+            self.lib.fo_weno3z(_p(row), C.byref(l), C.byref(r)); out[i, 0:2] = l.value, r.value
+            self.lib.fo_weno5(_p(row), C.byref(l), C.byref(r), 2); out[i, 2:4] = l.value, r.value
+            self.lib.fo_weno5(_p(row), C.byref(l), C.byref(r), 3); out[i, 4:6] = l.value, r.value
+            self.lib.fo_weno7z(_p(row), C.byref(l), C.byref(r)); out[i, 6:8] = l.value, r.value
+            for f in range(1, 5):
+                self.lib.fo_muscl3(_p(row), C.byref(l), C.byref(r), f)
+                out[i, 6 + 2 * f:8 + 2 * f] = l.value, r.value
+        return out
+
+    # ---- flux -----------------------------------------------------------------------------
+    @staticmethod
+    def opts(flag_acu=2, flag_weno=2, flag_muscl=0, select_riemann=2, select_time=1, dx_comp=1e-3,
+             do_tran=0):
+        return FoOpts(flag_acu, flag_weno, flag_muscl, select_riemann, select_time, 0, 1, do_tran,
+                      dx_comp)
+
+    def flux_advection(self, qf, n, lv, dx, **kw):
+        o = self.opts(dx_comp=dx, **kw)
+        qf = _f64(qf)
+        F = np.zeros((n + 1, self.nequ))
+        G = np.zeros((n + 1, 3))
+        self.lib.fo_flux_advection(self.m, C.byref(o), C.c_int(n), C.c_int(lv), C.c_double(dx),
+                                   _p(qf), _p(F), _p(G))
+        return F, G
+
+    def flux_tran(self, Qc, n, lv, dx):
+        Qc = _f64(Qc)
+        F = np.zeros((n + 1, self.nequ))
+        self.lib.fo_flux_tran(self.m, C.c_int(n), C.c_int(lv), C.c_double(dx), _p(Qc), _p(F))
+        return F
+
+    def mcavis(self, T, X): return self.lib.fo_mcavis(self.m, C.c_double(T), _p(_f64(X)))
+    def mcacon(self, T, X): return self.lib.fo_mcacon(self.m, C.c_double(T), _p(_f64(X)))
+
+    def mcadif(self, p, T, X):
+        D = np.zeros(self.kk)
+        self.lib.fo_mcadif(self.m, C.c_double(p), C.c_double(T), _p(_f64(X)), _p(D))
+        return D
+
+    # ---- kinetics -------------------------------------------------------------------------
+    def ckwyr(self, rho_cgs, T, Y):
+        w = np.zeros(self.kk)
+        qf = np.zeros(self.ii)
+        qr = np.zeros(self.ii)
+        self.lib.fo_ckwyr(self.m, C.c_double(rho_cgs), C.c_double(T), _p(_f64(Y)), _p(w), _p(qf),
+                          _p(qr))
+        return w, qf, qr
+
+    def ckcdyr(self, rho_cgs, T, Y):
+        c = np.zeros(self.kk)
+        d = np.zeros(self.kk)
+        self.lib.fo_ckcdyr(self.m, C.c_double(rho_cgs), C.c_double(T), _p(_f64(Y)), _p(c), _p(d))
+        return c, d
+
+
+class Domain:
+    """Multi-block wall-bounded domain stepped by the oracle (reference storage layout)."""
+
+    def __init__(self, orc: Oracle, cells, nblk, ncell, dx, **opts):
+        self.orc = orc
+        self.o = orc.opts(dx_comp=dx, **opts)
+        cells = _f64(cells)
+        self.ptr = C.c_void_p(orc.lib.fo_domain_create(orc.m, C.byref(self.o), C.c_int(nblk),
+                                                        C.c_int(ncell), C.c_double(dx), _p(cells)))
+        self.d = C.cast(self.ptr, C.POINTER(FoDomain)).contents
+        self.nblk, self.ncell, self.lv, self.nequ = nblk, ncell, self.d.lv, orc.nequ
+
+    def __del__(self):
+        try:
+            self.orc.lib.fo_domain_free(self.ptr)
+        except Exception:
+            pass
+
+    def _arr(self, ptr):
+        n = self.nblk * (self.ncell + 2 * self.lv) * self.nequ
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.nblk, self.ncell + 2 * self.lv,
+                                                              self.nequ)
+
+    @property
+    def Qc(self): return self._arr(self.d.Qc).copy()
+
+    @property
+    def qf(self): return self._arr(self.d.qf).copy()
+
+    def calc_dt(self, cfl):
+        return self.orc.lib.fo_calc_dt(self.ptr, C.c_double(cfl))
+
+    def solve_fluid(self, dt):
+        self.orc.lib.fo_solve_fluid(self.ptr, C.c_double(dt))
+
+    def refresh(self):
+        self.orc.lib.fo_refresh(self.ptr)
