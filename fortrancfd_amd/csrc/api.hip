@@ -1,0 +1,489 @@
+// api.hip -- C ABI of libfcfd_hip.so (declared in include/fcfd.h).
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/fcfd.h"
+#include "fcfd_common.hpp"
+#include "fluid.hpp"
+
+namespace fcfd {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+}  // namespace fcfd
+
+using namespace fcfd;
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+            return FCFD_E_HIP;                                                                 \
+        }                                                                                      \
+    } while (0)
+
+namespace {
+
+constexpr int NTIMERS = 3;   // 0 flux, 1 stage end, 2 CFL
+
+int lvir_of(const fcfd_opts &o) {                                  // SRC/params.f90:161-173
+    if (o.flag_acu == 0) return 1;
+    if (o.flag_acu == 1) return 2;
+    if (o.flag_weno == 1) return 2;
+    if (o.flag_weno == 4) return 4;
+    return 3;
+}
+
+int check_opts(const fcfd_opts &o, int kk) {
+    if (o.is_spherical != 0) { set_error("is_spherical = 1 is not supported (Cartesian only)"); return FCFD_E_UNSUPPORTED; }
+    if (o.flag_lbound != 0 || o.flag_rbound != 0) { set_error("only reflective walls (flag_lbound = flag_rbound = 0) are supported"); return FCFD_E_UNSUPPORTED; }
+    if (o.select_time != 0 && o.select_time != 1) { set_error("select_time must be 0 (Euler) or 1 (TVD-RK3)"); return FCFD_E_UNSUPPORTED; }
+    if (o.select_riemann < 0 || o.select_riemann > 2) { set_error("select_riemann must be 0, 1 or 2"); return FCFD_E_ARG; }
+    if (o.flag_acu < 0 || o.flag_acu > 2) { set_error("flag_acu must be 0, 1 or 2"); return FCFD_E_ARG; }
+    if (o.flag_acu == 1 && (o.flag_muscl < 1 || o.flag_muscl > 4)) { set_error("flag_muscl must be 1..4"); return FCFD_E_ARG; }
+    if (o.flag_acu == 2 && (o.flag_weno < 1 || o.flag_weno > 4)) { set_error("flag_weno must be 1..4"); return FCFD_E_ARG; }
+    if (o.do_tran) { set_error("do_tran = 1 is not available in this build"); return FCFD_E_UNSUPPORTED; }
+    bool ok = false;
+#define FCFD_CASE(N) ok = ok || (kk == N);
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    if (!ok) { set_error("species count " + std::to_string(kk) + " has no kernel instantiation"); return FCFD_E_UNSUPPORTED; }
+    return FCFD_OK;
+}
+
+long round_up(long n, long m) { return (n + m - 1) / m * m; }
+
+}  // namespace
+
+struct fcfd_ctx {
+    const fcfd_mech *mech = nullptr;
+    fcfd_opts o{};
+    int nblk = 0, ncell = 0, lv = 0, S = 0, kk = 0, nequ = 0;
+    long P = 0, PF = 0;
+    double dx = 0.0;
+    hipStream_t stream = nullptr;
+    DevThermo *thermo = nullptr;
+    double *Q[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;
+    double *qf = nullptr, *F = nullptr, *god = nullptr, *cells = nullptr;
+    double *d_dt = nullptr, *d_time = nullptr;
+    unsigned long long *d_cvel = nullptr;
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    double graph_cfl = -1.0;
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_pending;   // (timer id, event index of the start event)
+    size_t ev_used = 0;
+    double ms[NTIMERS] = {0, 0, 0};
+    long long cnt[NTIMERS] = {0, 0, 0};
+};
+
+namespace {
+
+hipEvent_t next_event(fcfd_ctx *c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// Runs `launch` bracketed by timing events when timing is on.
+template <class Fn>
+hipError_t timed(fcfd_ctx *c, int id, Fn launch) {
+    if (!c->timing) return launch();
+    const size_t i0 = c->ev_used;
+    hipEvent_t e0 = next_event(c), e1 = next_event(c);
+    if (!e0 || !e1) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(e0, c->stream);
+    if (e != hipSuccess) return e;
+    e = launch();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(e1, c->stream);
+    c->ev_pending.push_back({id, int(i0)});
+    return e;
+}
+
+hipError_t collect_timing(fcfd_ctx *c) {
+    if (c->ev_pending.empty()) { c->ev_used = 0; return hipSuccess; }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    for (auto &p : c->ev_pending) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, c->ev_pool[p.second], c->ev_pool[p.second + 1]);
+        if (e != hipSuccess) return e;
+        c->ms[p.first] += ms;
+        c->cnt[p.first] += 1;
+    }
+    c->ev_pending.clear();
+    c->ev_used = 0;
+    return hipSuccess;
+}
+
+FluxArgs flux_args(fcfd_ctx *c, bool god) {
+    FluxArgs a{};
+    a.thermo = c->thermo; a.qf = c->qf; a.F = c->F; a.godunov = god ? c->god : nullptr;
+    a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P; a.PF = c->PF;
+    a.muscl = c->o.flag_muscl; a.riemann = c->o.select_riemann;
+    return a;
+}
+
+StageArgs stage_args(fcfd_ctx *c, int mode, const double *Q0, const double *Qin, double *Qout,
+                     double dt, const double *dt_dev, double h1, double h2) {
+    StageArgs a{};
+    a.thermo = c->thermo; a.Q0 = Q0; a.Qin = Qin; a.F = c->F; a.Qout = Qout; a.qf = c->qf;
+    a.dt_dev = dt_dev; a.dt = dt; a.dx = c->dx; a.h1 = h1; a.h2 = h2; a.mode = mode; a.ghosts_only = 0;
+    a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P; a.PF = c->PF;
+    return a;
+}
+
+// One solve_fluid on the stream; dt from the host (dt_dev == nullptr) or from device memory.
+hipError_t enqueue_fluid(fcfd_ctx *c, double dt, const double *dt_dev) {
+    const int nsp = c->kk;
+    const FluxArgs fa = flux_args(c, false);
+    auto flux = [&]() { return timed(c, 0, [&]() { return launch_flux(fa, nsp, c->o.flag_acu, c->o.flag_weno, c->stream); }); };
+    auto stage = [&](const StageArgs &s) { return timed(c, 1, [&]() { return launch_stage_end(s, nsp, c->stream); }); };
+    hipError_t e;
+    const int a = c->cur, b = (c->cur + 1) % 3, d = (c->cur + 2) % 3;
+    if (c->o.select_time == 0) {                                   // explicit_euler
+        if ((e = flux()) != hipSuccess) return e;
+        if ((e = stage(stage_args(c, 0, nullptr, c->Q[a], c->Q[b], dt, dt_dev, 1.0, 1.0))) != hipSuccess) return e;
+        c->cur = b;
+        return hipSuccess;
+    }
+    // TVDRK3 (SRC/timeintegration.f90:132-217)
+    if ((e = flux()) != hipSuccess) return e;
+    if ((e = stage(stage_args(c, 0, nullptr, c->Q[a], c->Q[b], dt, dt_dev, 1.0, 1.0))) != hipSuccess) return e;
+    if ((e = flux()) != hipSuccess) return e;
+    if ((e = stage(stage_args(c, 1, c->Q[a], c->Q[b], c->Q[d], dt, dt_dev, 3.0 / 4.0, 1.0 / 4.0))) != hipSuccess) return e;
+    if ((e = flux()) != hipSuccess) return e;
+    if ((e = stage(stage_args(c, 1, c->Q[a], c->Q[d], c->Q[b], dt, dt_dev, 1.0 / 3.0, 2.0 / 3.0))) != hipSuccess) return e;
+    c->cur = b;
+    return hipSuccess;
+}
+
+hipError_t enqueue_dt(fcfd_ctx *c, double cfl, double *time_dev) {
+    DtArgs a{};
+    a.thermo = c->thermo; a.qf = c->qf; a.cvel_bits = c->d_cvel; a.dt_out = c->d_dt; a.time = time_dev;
+    a.cfl = cfl; a.dx = c->dx; a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P;
+    return timed(c, 2, [&]() { return launch_dt(a, c->kk, c->stream); });
+}
+
+hipError_t enqueue_refresh(fcfd_ctx *c) {
+    StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
+    return timed(c, 1, [&]() { return launch_stage_end(s, c->kk, c->stream); });
+}
+
+void drop_graphs(fcfd_ctx *c) {
+    for (auto &g : c->graph)
+        if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fcfd_last_error(void) { return g_err.c_str(); }
+
+int fcfd_mech_load(const char *chem_bin, const char *tran_bin, fcfd_mech **out) {
+    if (!out) { set_error("out is NULL"); return FCFD_E_ARG; }
+    *out = nullptr;
+    auto m = std::make_unique<fcfd_mech>();
+    std::string err;
+    int rc = load_mech(chem_bin, tran_bin, m->host, err);
+    if (rc != 0) { set_error(err); return rc; }
+    *out = m.release();
+    return FCFD_OK;
+}
+
+void fcfd_mech_free(fcfd_mech *m) { delete m; }
+
+int fcfd_mech_dims(const fcfd_mech *m, int *kk, int *ii) {
+    if (!m) { set_error("mech is NULL"); return FCFD_E_ARG; }
+    if (kk) *kk = m->host.kk;
+    if (ii) *ii = m->host.ii;
+    return FCFD_OK;
+}
+
+int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, double dx, fcfd_ctx **out) {
+    if (!m || !o || !out || nblk < 1 || ncell < 1 || !(dx > 0.0)) { set_error("fcfd_create: invalid argument"); return FCFD_E_ARG; }
+    *out = nullptr;
+    int rc = check_opts(*o, m->host.kk);
+    if (rc) return rc;
+    auto c = std::make_unique<fcfd_ctx>();
+    c->mech = m; c->o = *o; c->nblk = nblk; c->ncell = ncell; c->dx = dx;
+    c->lv = lvir_of(*o);
+    if (ncell < c->lv) { set_error("ncell must be at least lvir"); return FCFD_E_ARG; }
+    c->S = ncell + 2 * c->lv; c->kk = m->host.kk; c->nequ = 3 + c->kk;
+    c->P = round_up(long(nblk) * c->S, 64);
+    c->PF = round_up(long(nblk) * (ncell + 1), 64);
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipMalloc(&c->thermo, sizeof(DevThermo)));
+    HIP_TRY(hipMemcpy(c->thermo, &m->host.thermo, sizeof(DevThermo), hipMemcpyHostToDevice));
+    const size_t sb = sizeof(double) * size_t(c->nequ) * c->P;
+    for (auto &q : c->Q) { HIP_TRY(hipMalloc(&q, sb)); HIP_TRY(hipMemset(q, 0, sb)); }
+    HIP_TRY(hipMalloc(&c->qf, sb));
+    HIP_TRY(hipMemset(c->qf, 0, sb));
+    HIP_TRY(hipMalloc(&c->F, sizeof(double) * size_t(c->nequ) * c->PF));
+    HIP_TRY(hipMalloc(&c->god, sizeof(double) * 3 * c->PF));
+    HIP_TRY(hipMalloc(&c->d_dt, sizeof(double)));
+    HIP_TRY(hipMalloc(&c->d_time, sizeof(double)));
+    HIP_TRY(hipMalloc(&c->d_cvel, sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(c->d_cvel, 0, sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(c->d_dt, 0, sizeof(double)));
+    *out = c.release();
+    return FCFD_OK;
+}
+
+void fcfd_destroy(fcfd_ctx *c) {
+    if (!c) return;
+    drop_graphs(c);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto q : c->Q) (void)hipFree(q);
+    (void)hipFree(c->qf); (void)hipFree(c->F); (void)hipFree(c->god); (void)hipFree(c->cells);
+    (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int fcfd_lvir(const fcfd_ctx *c) { return c ? c->lv : FCFD_E_ARG; }
+
+int fcfd_upload(fcfd_ctx *c, int ib, const double *Qc, const double *qf) {
+    if (!c || !Qc || ib < 0 || ib >= c->nblk) { set_error("fcfd_upload: invalid argument"); return FCFD_E_ARG; }
+    const int S = c->S, ne = c->nequ;
+    std::vector<double> h(size_t(ne) * S);
+    auto put = [&](const double *src, double *dst) -> int {
+        for (int v = 0; v < ne; ++v)
+            for (int j = 0; j < S; ++j) h[size_t(v) * S + j] = src[size_t(j) * ne + v];
+        HIP_TRY(hipMemcpy2DAsync(dst + long(ib) * S, sizeof(double) * c->P, h.data(), sizeof(double) * S,
+                                 sizeof(double) * S, ne, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return 0;
+    };
+    if (int rc = put(Qc, c->Q[c->cur])) return rc;
+    if (qf) {
+        if (int rc = put(qf, c->qf)) return rc;
+    } else {
+        HIP_TRY(launch_qc_to_qf(c->thermo, c->kk, S, c->Q[c->cur] + long(ib) * S, c->qf + long(ib) * S, c->P, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return FCFD_OK;
+}
+
+int fcfd_download(fcfd_ctx *c, int ib, double *Qc, double *qf) {
+    if (!c || ib < 0 || ib >= c->nblk) { set_error("fcfd_download: invalid argument"); return FCFD_E_ARG; }
+    const int S = c->S, ne = c->nequ;
+    std::vector<double> h(size_t(ne) * S);
+    auto get = [&](const double *src, double *dst) -> int {
+        HIP_TRY(hipMemcpy2DAsync(h.data(), sizeof(double) * S, src + long(ib) * S, sizeof(double) * c->P,
+                                 sizeof(double) * S, ne, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int v = 0; v < ne; ++v)
+            for (int j = 0; j < S; ++j) dst[size_t(j) * ne + v] = h[size_t(v) * S + j];
+        return 0;
+    };
+    if (Qc)
+        if (int rc = get(c->Q[c->cur], Qc)) return rc;
+    if (qf)
+        if (int rc = get(c->qf, qf)) return rc;
+    return FCFD_OK;
+}
+
+int fcfd_init_cells(fcfd_ctx *c, const double *cells) {
+    if (!c || !cells) { set_error("fcfd_init_cells: invalid argument"); return FCFD_E_ARG; }
+    const size_t n = size_t(c->nblk) * c->ncell * (3 + c->kk);
+    if (!c->cells) HIP_TRY(hipMalloc(&c->cells, sizeof(double) * n));
+    HIP_TRY(hipMemcpyAsync(c->cells, cells, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    InitArgs a{};
+    a.thermo = c->thermo; a.cells = c->cells; a.Qc = c->Q[c->cur]; a.qf = c->qf;
+    a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P;
+    HIP_TRY(launch_init(a, c->kk, c->stream));
+    // ghost fill + QcToqf on ghosts only (SRC/main.f90:96-114)
+    StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
+    s.ghosts_only = 1;
+    HIP_TRY(launch_stage_end(s, c->kk, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+int fcfd_calc_dt(fcfd_ctx *c, double cfl, double *dt) {
+    if (!c || !dt) { set_error("fcfd_calc_dt: invalid argument"); return FCFD_E_ARG; }
+    HIP_TRY(enqueue_dt(c, cfl, nullptr));
+    HIP_TRY(hipMemcpyAsync(dt, c->d_dt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(collect_timing(c));
+    return FCFD_OK;
+}
+
+int fcfd_solve_fluid(fcfd_ctx *c, double dt) {
+    if (!c) { set_error("fcfd_solve_fluid: invalid argument"); return FCFD_E_ARG; }
+    if (c->o.do_hydro != 1) return FCFD_OK;
+    HIP_TRY(enqueue_fluid(c, dt, nullptr));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(collect_timing(c));
+    return FCFD_OK;
+}
+
+int fcfd_refresh(fcfd_ctx *c) {
+    if (!c) { set_error("fcfd_refresh: invalid argument"); return FCFD_E_ARG; }
+    HIP_TRY(enqueue_refresh(c));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(collect_timing(c));
+    return FCFD_OK;
+}
+
+int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
+    if (!c || nsteps < 0 || !(cfl > 0.0)) { set_error("fcfd_advance: invalid argument"); return FCFD_E_ARG; }
+    const double t0 = time ? *time : 0.0;
+    HIP_TRY(hipMemcpyAsync(c->d_time, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // Without a source term the post-step refresh recomputes identical primitives (QcToqf of
+    // an unchanged Qc), so it is only enqueued when the chemistry has changed the state.
+    const bool refresh = c->o.do_reaction == 1;
+    if (c->timing) {
+        for (int s = 0; s < nsteps; ++s) {
+            HIP_TRY(enqueue_dt(c, cfl, c->d_time));
+            if (c->o.do_hydro == 1) HIP_TRY(enqueue_fluid(c, 0.0, c->d_dt));
+            if (refresh) HIP_TRY(enqueue_refresh(c));
+        }
+        HIP_TRY(collect_timing(c));
+    } else {
+        if (cfl != c->graph_cfl) { drop_graphs(c); c->graph_cfl = cfl; }
+        for (int s = 0; s < nsteps; ++s) {
+            const int cur = c->cur;
+            if (!c->graph[cur]) {
+                hipGraph_t g;
+                HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                hipError_t e = enqueue_dt(c, cfl, c->d_time);
+                if (e == hipSuccess && c->o.do_hydro == 1) e = enqueue_fluid(c, 0.0, c->d_dt);
+                if (e == hipSuccess && refresh) e = enqueue_refresh(c);
+                hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+                HIP_TRY(e);
+                HIP_TRY(e2);
+                c->cur = cur;                       // capture only recorded the step
+                HIP_TRY(hipGraphInstantiate(&c->graph[cur], g, nullptr, nullptr, 0));
+                HIP_TRY(hipGraphDestroy(g));
+            }
+            HIP_TRY(hipGraphLaunch(c->graph[cur], c->stream));
+            c->cur = c->o.do_hydro == 1 ? (cur + 1) % 3 : cur;
+        }
+    }
+    if (time) HIP_TRY(hipMemcpyAsync(time, c->d_time, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+int fcfd_set_timing(fcfd_ctx *c, int enable) {
+    if (!c) { set_error("fcfd_set_timing: invalid argument"); return FCFD_E_ARG; }
+    c->timing = enable != 0;
+    for (int i = 0; i < NTIMERS; ++i) { c->ms[i] = 0.0; c->cnt[i] = 0; }
+    return FCFD_OK;
+}
+
+int fcfd_get_timing(fcfd_ctx *c, int id, double *ms_total, long long *launches) {
+    if (!c || id < 0 || id >= NTIMERS) { set_error("fcfd_get_timing: invalid argument"); return FCFD_E_ARG; }
+    if (ms_total) *ms_total = c->ms[id];
+    if (launches) *launches = c->cnt[id];
+    return FCFD_OK;
+}
+
+}  // extern "C"
+
+// ---- stateless host-array entry points ----------------------------------------------------
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+// rows x cols (row-major, AoS) -> cols x rows (SoA)
+std::vector<double> transpose(const double *in, long rows, int cols) {
+    std::vector<double> out(size_t(rows) * cols);
+    for (long r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) out[size_t(c) * rows + r] = in[size_t(r) * cols + c];
+    return out;
+}
+
+void untranspose(const std::vector<double> &in, long rows, int cols, double *out) {
+    for (long r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) out[size_t(r) * cols + c] = in[size_t(c) * rows + r];
+}
+
+}  // namespace
+
+extern "C" {
+
+int fcfd_flux_advection_host(const fcfd_mech *m, const fcfd_opts *o, int n, int lv, double dx,
+                             const double *qf, double *Fplus, double *godunov) {
+    if (!m || !o || !qf || !Fplus || n < 1 || lv < 1) { set_error("fcfd_flux_advection_host: invalid argument"); return FCFD_E_ARG; }
+    fcfd_opts oo = *o;
+    oo.do_tran = 0;
+    if (int rc = check_opts(oo, m->host.kk)) return rc;
+    if (lv < lvir_of(oo)) { set_error("lv smaller than the scheme's stencil half-width"); return FCFD_E_ARG; }
+    (void)dx;
+    const int ne = 3 + m->host.kk;
+    const long cols = n + 2L * lv, nif = n + 1;
+    DevBuf th, dq, dF, dG;
+    HIP_TRY(hipMalloc(&th.p, sizeof(DevThermo)));
+    HIP_TRY(hipMemcpy(th.p, &m->host.thermo, sizeof(DevThermo), hipMemcpyHostToDevice));
+    auto hq = transpose(qf, cols, ne);
+    HIP_TRY(hipMalloc(&dq.p, sizeof(double) * hq.size()));
+    HIP_TRY(hipMemcpy(dq.p, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&dF.p, sizeof(double) * ne * nif));
+    HIP_TRY(hipMalloc(&dG.p, sizeof(double) * 3 * nif));
+    FluxArgs a{};
+    a.thermo = static_cast<DevThermo *>(th.p); a.qf = static_cast<double *>(dq.p);
+    a.F = static_cast<double *>(dF.p); a.godunov = static_cast<double *>(dG.p);
+    a.nblk = 1; a.ncell = n; a.lv = lv; a.S = int(cols); a.P = cols; a.PF = nif;
+    a.muscl = oo.flag_muscl; a.riemann = oo.select_riemann;
+    HIP_TRY(launch_flux(a, m->host.kk, oo.flag_acu, oo.flag_weno, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<double> hF(size_t(ne) * nif), hG(size_t(3) * nif);
+    HIP_TRY(hipMemcpy(hF.data(), dF.p, sizeof(double) * hF.size(), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hG.data(), dG.p, sizeof(double) * hG.size(), hipMemcpyDeviceToHost));
+    untranspose(hF, nif, ne, Fplus);
+    if (godunov) untranspose(hG, nif, 3, godunov);
+    return FCFD_OK;
+}
+
+static int conv_host(const fcfd_mech *m, int n, const double *in, double *out, bool to_qc) {
+    if (!m || !in || !out || n < 1) { set_error("conversion: invalid argument"); return FCFD_E_ARG; }
+    const int ne = 3 + m->host.kk;
+    DevBuf th, di, dout;
+    HIP_TRY(hipMalloc(&th.p, sizeof(DevThermo)));
+    HIP_TRY(hipMemcpy(th.p, &m->host.thermo, sizeof(DevThermo), hipMemcpyHostToDevice));
+    auto h = transpose(in, n, ne);
+    HIP_TRY(hipMalloc(&di.p, sizeof(double) * h.size()));
+    HIP_TRY(hipMalloc(&dout.p, sizeof(double) * h.size()));
+    HIP_TRY(hipMemcpy(di.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    const DevThermo *t = static_cast<DevThermo *>(th.p);
+    if (to_qc) HIP_TRY(launch_qf_to_qc(t, m->host.kk, n, static_cast<double *>(di.p), static_cast<double *>(dout.p), n, nullptr));
+    else HIP_TRY(launch_qc_to_qf(t, m->host.kk, n, static_cast<double *>(di.p), static_cast<double *>(dout.p), n, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(h.data(), dout.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    untranspose(h, n, ne, out);
+    return FCFD_OK;
+}
+
+int fcfd_qf_to_qc_host(const fcfd_mech *m, int n, const double *qf, double *Qc) { return conv_host(m, n, qf, Qc, true); }
+int fcfd_qc_to_qf_host(const fcfd_mech *m, int n, const double *Qc, double *qf) { return conv_host(m, n, Qc, qf, false); }
+
+int fcfd_recon_host(int n, const double *s, double *out) {
+    if (!s || !out || n < 1) { set_error("fcfd_recon_host: invalid argument"); return FCFD_E_ARG; }
+    DevBuf ds, dout;
+    HIP_TRY(hipMalloc(&ds.p, sizeof(double) * 8 * n));
+    HIP_TRY(hipMalloc(&dout.p, sizeof(double) * 18 * n));
+    HIP_TRY(hipMemcpy(ds.p, s, sizeof(double) * 8 * n, hipMemcpyHostToDevice));
+    HIP_TRY(launch_recon(n, static_cast<double *>(ds.p), static_cast<double *>(dout.p), nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, dout.p, sizeof(double) * 18 * n, hipMemcpyDeviceToHost));
+    return FCFD_OK;
+}
+
+}  // extern "C"

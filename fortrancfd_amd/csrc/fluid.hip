@@ -1,0 +1,451 @@
+// fluid.hip -- kernels of the hydrodynamic part of one time step.
+//
+// Device layout ("SoA, ghost-padded"): variable v of padded column c lives at q[v*P + c].  Block b
+// owns columns [b*S, (b+1)*S), S = ncell + 2*lv; its interior cell i (0-based) is column
+// b*S + lv + i, exactly the reference's b_obj(ib)%qf(:, ista-lv : iend+lv) column range.
+// Interface j (0..ncell) of block b -- the reference's Fplus(:, ista-1+j) -- is F[v*PF + b*(ncell+1)+j].
+#include "fcfd_device.hpp"
+#include "fluid.hpp"
+
+namespace fcfd {
+
+// ---------------------------------------------------------------------------------------------
+// Per-side interface state: what qfToQc + pressure + sonic_speed give for one reconstructed
+// primitive vector (SRC/flux.f90:247-268, SRC/eos.f90:11-46).
+// ---------------------------------------------------------------------------------------------
+struct Side {
+    double rho, u, T, sy, p, a, E;
+};
+
+template <int NSP>
+__device__ __forceinline__ Side side_state(const DevThermo *__restrict__ m, double rho, double u,
+                                           double T, const double (&y)[NSP]) {
+    Side s;
+    s.rho = rho; s.u = u; s.T = T;
+    // adjust_sum(Y) denominator (Y itself is renormalised lazily where QcL/R needs it).
+    double sy = 0.0, syw = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+        sy = sy + fmax(y[k], 0.0);
+        syw = syw + y[k] / m->wt[k];          // CKYTX on the raw reconstructed Y
+    }
+    s.sy = sy;
+    double x[NSP];
+    double sx = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+        x[k] = fmax(y[k] / (syw * m->wt[k]), 0.0);
+        sx = sx + x[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) x[k] = x[k] / sx;
+    const double wtm = mix_w<NSP>(m, x);
+    const MixSums ms = mix_sums<NSP>(m, x, T, true, true, true);
+    s.p = (rho * 1.e-3) * m->ru * T / wtm * 0.1;                 // pressure(), CKPX
+    const double e = ms.ubml * 1.e-4 / wtm;                      // specific_energy()
+    s.E = rho * (e + 0.5 * (u * u));
+    const double cp = ms.cpbml * 1.e-4 / wtm, cv = ms.cvbml * 1.e-4 / wtm;
+    s.a = sqrt(cp / cv * s.p / rho);                             // sonic_speed()
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// calc_flux_advection: one thread per interface.
+// ---------------------------------------------------------------------------------------------
+template <int NSP, int RK>
+__global__ void __launch_bounds__(64) k_flux(FluxArgs a) {
+    const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int nif = a.ncell + 1;
+    if (g >= long(a.nblk) * nif) return;
+    const int b = int(g / nif), j = int(g % nif);
+    const long c0 = long(b) * a.S + a.lv + j - 1;      // left cell of the interface
+    const DevThermo *__restrict__ m = a.thermo;
+    const double *__restrict__ q = a.qf;
+
+    double rl, rr, ul, ur, tl, tr;
+    reconstruct<RK>(q + 0 * a.P, c0, a.muscl, rl, rr);
+    reconstruct<RK>(q + 1 * a.P, c0, a.muscl, ul, ur);
+    reconstruct<RK>(q + 2 * a.P, c0, a.muscl, tl, tr);
+    double yl[NSP], yr[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) reconstruct<RK>(q + (3 + k) * a.P, c0, a.muscl, yl[k], yr[k]);
+
+    const Side L = side_state<NSP>(m, rl, ul, tl, yl);
+    const Side R = side_state<NSP>(m, rr, ur, tr, yr);
+
+    // physical fluxes of mass, momentum, energy (SRC/flux.f90:270-280)
+    const double fl0 = L.rho * L.u, fr0 = R.rho * R.u;
+    const double fl1 = L.rho * L.u * L.u + L.p, fr1 = R.rho * R.u * R.u + R.p;
+    const double fl2 = (L.E + L.p) * L.u, fr2 = (R.E + R.p) * R.u;
+    double *__restrict__ F = a.F + g;
+    const long PF = a.PF;
+
+    if (a.riemann == 0) {                                         // LxF (SRC/flux.f90:291-294)
+        const double lL = fmax(fmax(fabs(L.u - L.a), fabs(L.u)), fabs(L.u + L.a));
+        const double lR = fmax(fmax(fabs(R.u - R.a), fabs(R.u)), fabs(R.u + R.a));
+        const double al = fmax(lL, lR);
+        F[0] = 0.5 * (fl0 + fr0 - al * (R.rho - L.rho));
+        F[PF] = 0.5 * (fl1 + fr1 - al * (R.rho * R.u - L.rho * L.u));
+        F[2 * PF] = 0.5 * (fl2 + fr2 - al * (R.E - L.E));
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) {
+            const double QL = L.rho * (fmax(yl[k], 0.0) / L.sy), QR = R.rho * (fmax(yr[k], 0.0) / R.sy);
+            const double fL = L.rho * yl[k] * L.u, fR = R.rho * yr[k] * R.u;
+            F[(3 + k) * PF] = 0.5 * (fL + fR - al * (QR - QL));
+        }
+    } else {
+        const double rt = sqrt(R.rho / L.rho);
+        const double uro = (L.u + rt * R.u) / (1.0 + rt);
+        const double aro = (L.a + rt * R.a) / (1.0 + rt);
+        const double sL = fmin(L.u - L.a, uro - aro), sR = fmax(R.u + R.a, uro + aro);
+        if (a.riemann == 1) {                                     // HLL (SRC/flux.f90:300-315)
+            int br = (0.0 <= sL) ? 0 : ((sL <= 0.0 && 0.0 <= sR) ? 1 : 2);
+            auto hll = [&](double fL, double fR, double QL, double QR) {
+                return br == 0 ? fL : (br == 1 ? (sR * fL - sL * fR + sL * sR * (QR - QL)) / (sR - sL) : fR);
+            };
+            F[0] = hll(fl0, fr0, L.rho, R.rho);
+            F[PF] = hll(fl1, fr1, L.rho * L.u, R.rho * R.u);
+            F[2 * PF] = hll(fl2, fr2, L.E, R.E);
+#pragma unroll
+            for (int k = 0; k < NSP; ++k) {
+                const double QL = L.rho * (fmax(yl[k], 0.0) / L.sy), QR = R.rho * (fmax(yr[k], 0.0) / R.sy);
+                F[(3 + k) * PF] = hll(L.rho * yl[k] * L.u, R.rho * yr[k] * R.u, QL, QR);
+            }
+        } else {                                                  // HLLC (SRC/flux.f90:321-354)
+            const double sM = (L.p - R.p + R.rho * R.u * (sR - R.u) - L.rho * L.u * (sL - L.u)) /
+                              (R.rho * (sR - R.u) - L.rho * (sL - L.u));
+            // branch chain of SRC/flux.f90:332-354
+            const int br = (0.0 <= sL) ? 0 : ((sL <= 0.0 && 0.0 <= sM) ? 1 : ((sM <= 0.0 && 0.0 <= sR) ? 2 : 3));
+            const bool left = br <= 1;
+            const Side S = left ? L : R;
+            const double f0 = left ? fl0 : fr0, f1 = left ? fl1 : fr1, f2 = left ? fl2 : fr2;
+            if (br == 0 || br == 3) {                             // supersonic: upwind flux
+                F[0] = f0;
+                F[PF] = f1;
+                F[2 * PF] = f2;
+#pragma unroll
+                for (int k = 0; k < NSP; ++k) F[(3 + k) * PF] = S.rho * (left ? yl[k] : yr[k]) * S.u;
+            } else {                                              // star state of the upwind side
+                const double sK = left ? sL : sR;
+                const double c = S.rho * (sK - S.u) / (sK - sM);
+                const double v2 = S.E / S.rho + (sM - S.u) * (sM + S.p / (S.rho * (sK - S.u)));
+                F[0] = f0 + sK * (c * 1.0 - S.rho);
+                F[PF] = f1 + sK * (c * sM - S.rho * S.u);
+                F[2 * PF] = f2 + sK * (c * v2 - S.E);
+#pragma unroll
+                for (int k = 0; k < NSP; ++k) {
+                    const double yk = left ? yl[k] : yr[k];
+                    const double Qk = S.rho * (fmax(yk, 0.0) / S.sy);
+                    const double fk = S.rho * yk * S.u;
+                    F[(3 + k) * PF] = fk + sK * (c * (Qk / S.rho) - Qk);
+                }
+            }
+        }
+    }
+    if (a.godunov) {                                              // SRC/flux.f90:360-367
+        const double rt = sqrt(R.rho / L.rho);
+        double *G = a.godunov + g;
+        G[0] = sqrt(R.rho * L.rho);
+        G[PF] = (L.u + rt * R.u) / (1.0 + rt);
+        G[2 * PF] = (L.p + rt * R.p) / (1.0 + rt);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// QcToqf on one column, writing qf (SRC/eos.f90:49-83).
+// ---------------------------------------------------------------------------------------------
+template <int NSP>
+__device__ __forceinline__ void qc_to_qf_store(const DevThermo *__restrict__ m, const double (&Qv)[NSP + 3],
+                                               double *__restrict__ qf, long P, long c) {
+    const double rho = Qv[0];
+    const double u = Qv[1] / rho;
+    double y[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) y[k] = Qv[3 + k] / rho;
+    adjust_sum<NSP>(y);
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) qf[(3 + k) * P + c] = y[k];
+    ytx<NSP>(m, y);
+    adjust_sum<NSP>(y);
+    const double e = Qv[2] / rho - 0.5 * (u * u);
+    qf[c] = rho;
+    qf[P + c] = u;
+    qf[2 * P + c] = temperature_con<NSP>(m, e, y);
+}
+
+// Source column of padded column (b, i): interior -> itself; wall ghost -> mirrored interior cell
+// with momentum negated (SRC/boundary.f90:152-199); block ghost -> the neighbour block's interior
+// cell (SRC/block.f90:490-497).
+__device__ __forceinline__ void ghost_source(const StageArgs &a, int b, int i, int &sb, int &si, double &sgn) {
+    sb = b; si = i; sgn = 1.0;
+    if (i < 0) {
+        if (b == 0) { si = -i - 1; sgn = -1.0; }
+        else { sb = b - 1; si = a.ncell + i; }
+    } else if (i >= a.ncell) {
+        if (b == a.nblk - 1) { si = 2 * a.ncell - 1 - i; sgn = -1.0; }
+        else { sb = b + 1; si = i - a.ncell; }
+    }
+}
+
+// Stage end: RK combine of the source cell, ghost fill, primitive recovery -- the interior update
+// of SRC/timeintegration.f90:150-175 (or :48-56 for Euler), then update_boundary /
+// interpolation_block / QcToqf on every column (:201-215).  mode 0: Q = Qin - dt/dx*dF;
+// mode 1: Q = h1*Q0 + h2*(Qin - dt/dx*dF); mode 2: no update (ghost fill + QcToqf only).
+template <int NSP>
+__global__ void __launch_bounds__(64) k_stage_end(StageArgs a) {
+    const long c = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= long(a.nblk) * a.S) return;
+    const int b = int(c / a.S), i = int(c % a.S) - a.lv;
+    const bool ghost = (i < 0 || i >= a.ncell);
+    if (a.ghosts_only && !ghost) return;
+    int sb, si;
+    double sgn;
+    ghost_source(a, b, i, sb, si, sgn);
+    const long sc = long(sb) * a.S + a.lv + si;
+    const long fj = long(sb) * (a.ncell + 1) + si;
+    const double dtdx = (a.dt_dev ? *a.dt_dev : a.dt) / a.dx;
+    double Qv[NSP + 3];
+#pragma unroll
+    for (int v = 0; v < NSP + 3; ++v) {
+        double Q;
+        if (a.mode == 2) {
+            Q = a.Qin[v * a.P + sc];
+        } else {
+            const double d = a.F[v * a.PF + fj + 1] - a.F[v * a.PF + fj];
+            const double qi = a.Qin[v * a.P + sc];
+            if (a.mode == 0) Q = qi - dtdx * d;
+            else Q = a.h1 * a.Q0[v * a.P + sc] + a.h2 * (qi - dtdx * d);
+        }
+        if (v == 1) Q = sgn < 0.0 ? -Q : Q;
+        Qv[v] = Q;
+        a.Qout[v * a.P + c] = Q;
+    }
+    qc_to_qf_store<NSP>(a.thermo, Qv, a.qf, a.P, c);
+}
+
+// ---------------------------------------------------------------------------------------------
+// calc_dt (SRC/update.f90:13-59): max wave speed over interior cells -> dt on the device.
+// ---------------------------------------------------------------------------------------------
+template <int NSP>
+__global__ void __launch_bounds__(256) k_wave_speed(DtArgs a) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    double aupc = 0.0;
+    if (t < long(a.nblk) * a.ncell) {
+        const int b = int(t / a.ncell), i = int(t % a.ncell);
+        const long c = long(b) * a.S + a.lv + i;
+        const DevThermo *__restrict__ m = a.thermo;
+        double x[NSP];
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) x[k] = a.qf[(3 + k) * a.P + c];
+        ytx<NSP>(m, x);                                   // no adjust_sum here (update.f90:396)
+        const double rho = a.qf[c], u = a.qf[a.P + c], T = a.qf[2 * a.P + c];
+        const double wtm = mix_w<NSP>(m, x);
+        const double p = (rho * 1.e-3) * m->ru * T / wtm * 0.1;
+        const MixSums ms = mix_sums<NSP>(m, x, T, false, true, true);
+        const double cp = ms.cpbml * 1.e-4 / wtm, cv = ms.cvbml * 1.e-4 / wtm;
+        const double as = sqrt(cp / cv * p / rho);
+        aupc = fmax(fmax(fabs(u - as), fabs(u)), fabs(u + as));
+    }
+    // wave64 max via cross-lane shuffles, then one LDS slot per wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) aupc = fmax(aupc, __shfl_xor(aupc, off, 64));
+    __shared__ double wmax[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wmax[w] = aupc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = wmax[0];
+        for (int k = 1; k < int(blockDim.x >> 6); ++k) mx = fmax(mx, wmax[k]);
+        // non-negative doubles order like their bit patterns
+        atomicMax(reinterpret_cast<unsigned long long *>(a.cvel_bits), (unsigned long long)__double_as_longlong(mx));
+    }
+}
+
+__global__ void k_dt_finalize(DtArgs a) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double cvel = __longlong_as_double((long long)*a.cvel_bits);
+        const double dt = (a.cfl * a.dx) / cvel;
+        *a.dt_out = dt;
+        if (a.time) *a.time = *a.time + dt;
+        *a.cvel_bits = 0ull;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Initial state (SRC/params.f90:305-580 + SRC/block.f90:413-429): interior columns.
+// ---------------------------------------------------------------------------------------------
+template <int NSP>
+__global__ void __launch_bounds__(64) k_init_cells(InitArgs a) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= long(a.nblk) * a.ncell) return;
+    const int b = int(t / a.ncell), i = int(t % a.ncell);
+    const long c = long(b) * a.S + a.lv + i;
+    const DevThermo *__restrict__ m = a.thermo;
+    const double *cell = a.cells + t * (3 + NSP);
+    const double T = cell[0], p = cell[1], u = cell[2];
+    double x[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) x[k] = cell[3 + k];
+    adjust_sum<NSP>(x);
+    const double wx = mix_w<NSP>(m, x);
+    const double rho = wx * (p * 10.0) / (m->ru * T) * 1.e3;      // density(): CKRHOX
+    const MixSums ms = mix_sums<NSP>(m, x, T, true, false, false);
+    const double e = ms.ubml * 1.e-4 / wx;                        // specific_energy()
+    a.qf[c] = rho; a.qf[a.P + c] = u; a.qf[2 * a.P + c] = T;
+    a.Qc[c] = rho; a.Qc[a.P + c] = rho * u; a.Qc[2 * a.P + c] = rho * (e + 0.5 * (u * u));
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+        const double y = x[k] * m->wt[k] / wx;                    // CKXTY
+        a.qf[(3 + k) * a.P + c] = y;
+        a.Qc[(3 + k) * a.P + c] = rho * y;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Conversions and reconstruction on host-array batches (stateless entry points).
+// ---------------------------------------------------------------------------------------------
+template <int NSP>
+__global__ void k_qf_to_qc(const DevThermo *__restrict__ m, int n, const double *__restrict__ qf,
+                           double *__restrict__ Qc, long P) {
+    const long c = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double y[NSP];
+    const double rho = qf[c], u = qf[P + c], T = qf[2 * P + c];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) y[k] = qf[(3 + k) * P + c];
+    double ya[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) ya[k] = y[k];
+    adjust_sum<NSP>(ya);
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) Qc[(3 + k) * P + c] = rho * ya[k];
+    ytx<NSP>(m, y);
+    adjust_sum<NSP>(y);
+    const double wtm = mix_w<NSP>(m, y);
+    const MixSums ms = mix_sums<NSP>(m, y, T, true, false, false);
+    const double e = ms.ubml * 1.e-4 / wtm;
+    Qc[c] = rho;
+    Qc[P + c] = rho * u;
+    Qc[2 * P + c] = rho * (e + 0.5 * (u * u));
+}
+
+template <int NSP>
+__global__ void k_qc_to_qf(const DevThermo *__restrict__ m, int n, const double *__restrict__ Qc,
+                           double *__restrict__ qf, long P) {
+    const long c = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double Qv[NSP + 3];
+#pragma unroll
+    for (int v = 0; v < NSP + 3; ++v) Qv[v] = Qc[v * P + c];
+    qc_to_qf_store<NSP>(m, Qv, qf, P, c);
+}
+
+__global__ void k_recon(int n, const double *__restrict__ s, double *__restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double st[8];
+    for (int j = 0; j < 8; ++j) st[j] = s[r * 8 + j];
+    double *o = out + r * 18;
+    weno3z(st, o[0], o[1]);
+    weno5<2>(st, o[2], o[3]);
+    weno5<3>(st, o[4], o[5]);
+    weno7z(st, o[6], o[7]);
+    for (int f = 1; f <= 4; ++f) muscl3(st, o[6 + 2 * f], o[7 + 2 * f], f);
+    o[16] = 0.0;
+    o[17] = 0.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------------------------
+static int recon_kind(int flag_acu, int flag_weno) {
+    if (flag_acu == 0) return R_FIRST;
+    if (flag_acu == 1) return R_MUSCL;
+    switch (flag_weno) {
+        case 1: return R_WENO3Z;
+        case 2: return R_WENO5JS;
+        case 3: return R_WENO5Z;
+        case 4: return R_WENO7Z;
+        default: return -1;
+    }
+}
+
+template <int NSP>
+static hipError_t launch_flux_nsp(const FluxArgs &a, int rk, hipStream_t st) {
+    const long n = long(a.nblk) * (a.ncell + 1);
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+    switch (rk) {
+        case R_FIRST: hipLaunchKernelGGL((k_flux<NSP, R_FIRST>), grid, blk, 0, st, a); break;
+        case R_MUSCL: hipLaunchKernelGGL((k_flux<NSP, R_MUSCL>), grid, blk, 0, st, a); break;
+        case R_WENO3Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO3Z>), grid, blk, 0, st, a); break;
+        case R_WENO5JS: hipLaunchKernelGGL((k_flux<NSP, R_WENO5JS>), grid, blk, 0, st, a); break;
+        case R_WENO5Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO5Z>), grid, blk, 0, st, a); break;
+        case R_WENO7Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO7Z>), grid, blk, 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_flux(const FluxArgs &a, int nsp, int flag_acu, int flag_weno, hipStream_t st) {
+    const int rk = recon_kind(flag_acu, flag_weno);
+    if (rk < 0) return hipErrorInvalidValue;
+#define FCFD_CASE(N) if (nsp == N) return launch_flux_nsp<N>(a, rk, st);
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_stage_end(const StageArgs &a, int nsp, hipStream_t st) {
+    const long n = long(a.nblk) * a.S;
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_stage_end<N>), grid, blk, 0, st, a); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_dt(const DtArgs &a, int nsp, hipStream_t st) {
+    const long n = long(a.nblk) * a.ncell;
+    const dim3 grid(unsigned((n + 255) / 256)), blk(256);
+    bool launched = false;
+#define FCFD_CASE(N) if (!launched && nsp == N) { hipLaunchKernelGGL((k_wave_speed<N>), grid, blk, 0, st, a); launched = true; }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    if (!launched) return hipErrorInvalidValue;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(64), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_init(const InitArgs &a, int nsp, hipStream_t st) {
+    const long n = long(a.nblk) * a.ncell;
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_init_cells<N>), grid, blk, 0, st, a); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_qf_to_qc(const DevThermo *m, int nsp, int n, const double *qf, double *Qc, long P, hipStream_t st) {
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_qf_to_qc<N>), grid, blk, 0, st, m, n, qf, Qc, P); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_qc_to_qf(const DevThermo *m, int nsp, int n, const double *Qc, double *qf, long P, hipStream_t st) {
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_qc_to_qf<N>), grid, blk, 0, st, m, n, Qc, qf, P); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_recon(int n, const double *s, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_recon, dim3(unsigned((n + 63) / 64)), dim3(64), 0, st, n, s, out);
+    return hipGetLastError();
+}
+
+}  // namespace fcfd

@@ -1,0 +1,57 @@
+// fluid.hpp -- argument blocks and launchers of the hydrodynamic kernels (fluid.hip).
+#pragma once
+
+#include "fcfd_common.hpp"
+
+namespace fcfd {
+
+struct FluxArgs {
+    const DevThermo *thermo;
+    const double *qf;     // [nequ][P] primitive, ghost-padded
+    double *F;            // [nequ][PF] interface fluxes
+    double *godunov;      // [3][PF] or nullptr
+    int nblk, ncell, lv, S;
+    long P, PF;
+    int muscl, riemann;
+};
+
+struct StageArgs {
+    const DevThermo *thermo;
+    const double *Q0, *Qin, *F;
+    double *Qout, *qf;
+    const double *dt_dev;  // device dt (nullptr: use dt)
+    double dt, dx, h1, h2;
+    int mode;              // 0 Qin - dt/dx dF, 1 h1*Q0 + h2*(Qin - dt/dx dF), 2 copy
+    int ghosts_only;
+    int nblk, ncell, lv, S;
+    long P, PF;
+};
+
+struct DtArgs {
+    const DevThermo *thermo;
+    const double *qf;
+    unsigned long long *cvel_bits;
+    double *dt_out;
+    double *time;          // optional running time, advanced by dt
+    double cfl, dx;
+    int nblk, ncell, lv, S;
+    long P;
+};
+
+struct InitArgs {
+    const DevThermo *thermo;
+    const double *cells;   // [ntot][3+KK]: T, p, u, X
+    double *Qc, *qf;
+    int nblk, ncell, lv, S;
+    long P;
+};
+
+hipError_t launch_flux(const FluxArgs &a, int nsp, int flag_acu, int flag_weno, hipStream_t st);
+hipError_t launch_stage_end(const StageArgs &a, int nsp, hipStream_t st);
+hipError_t launch_dt(const DtArgs &a, int nsp, hipStream_t st);
+hipError_t launch_init(const InitArgs &a, int nsp, hipStream_t st);
+hipError_t launch_qf_to_qc(const DevThermo *m, int nsp, int n, const double *qf, double *Qc, long P, hipStream_t st);
+hipError_t launch_qc_to_qf(const DevThermo *m, int nsp, int n, const double *Qc, double *qf, long P, hipStream_t st);
+hipError_t launch_recon(int n, const double *s, double *out, hipStream_t st);
+
+}  // namespace fcfd

@@ -1,0 +1,180 @@
+"""ctypes binding of libfcfd_hip.so (include/fcfd.h).
+
+The library is the product: every compute call below runs HIP kernels on the current device.  If
+the shared object is missing, loading fails loudly -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "_lib", "libfcfd_hip.so")
+DATA_DIR = os.path.join(PKG, "data")
+GRI30 = os.path.join(DATA_DIR, "GRI3.0_CH4")
+
+_D = C.POINTER(C.c_double)
+_lib = None
+
+
+class FcfdError(RuntimeError):
+    pass
+
+
+class Opts(C.Structure):
+    """fcfd_opts: the reference's &params switches (SRC/params.f90:106-111)."""
+    _fields_ = [("do_hydro", C.c_int), ("do_tran", C.c_int), ("do_reaction", C.c_int),
+                ("select_time", C.c_int), ("select_riemann", C.c_int), ("flag_acu", C.c_int),
+                ("flag_muscl", C.c_int), ("flag_weno", C.c_int), ("is_spherical", C.c_int),
+                ("flag_lbound", C.c_int), ("flag_rbound", C.c_int), ("atol", C.c_double),
+                ("rtol", C.c_double)]
+
+
+def opts(do_hydro=1, do_tran=0, do_reaction=0, select_time=1, select_riemann=2, flag_acu=2,
+         flag_muscl=0, flag_weno=2, atol=1e-13, rtol=1e-5) -> Opts:
+    return Opts(do_hydro, do_tran, do_reaction, select_time, select_riemann, flag_acu, flag_muscl,
+                flag_weno, 0, 0, 0, atol, rtol)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FcfdError(f"{LIB_PATH} is missing: build it with `python -m fortrancfd_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.fcfd_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise FcfdError(f"fcfd error {rc}: {lib().fcfd_last_error().decode()}")
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Mechanism:
+    """CHEMKIN-II link files (chem.bin, tran.bin) as read by CKINIT / MCINIT."""
+
+    def __init__(self, mech_dir: str = GRI30, transport: bool = True):
+        L = lib()
+        self.handle = C.c_void_p()
+        chem = os.path.join(mech_dir, "chem.bin").encode()
+        tran = os.path.join(mech_dir, "tran.bin").encode() if transport else None
+        _check(L.fcfd_mech_load(chem, tran, C.byref(self.handle)))
+        kk, ii = C.c_int(), C.c_int()
+        _check(L.fcfd_mech_dims(self.handle, C.byref(kk), C.byref(ii)))
+        self.kk, self.ii, self.nequ = kk.value, ii.value, 3 + kk.value
+
+    def __del__(self):
+        try:
+            lib().fcfd_mech_free(self.handle)
+        except Exception:
+            pass
+
+    # ---- stateless kernels on host arrays ----------------------------------------------------
+    def calc_flux_advection(self, qf, n, lv, dx, **kw):
+        """calc_flux_advection (SRC/flux.f90:130-373) on one block: qf rows (n+2lv, nequ) ->
+        (Fplus (n+1, nequ), godunov_state (n+1, 3))."""
+        o = opts(**kw)
+        qf = _f64(qf)
+        F = np.zeros((n + 1, self.nequ))
+        G = np.zeros((n + 1, 3))
+        _check(lib().fcfd_flux_advection_host(self.handle, C.byref(o), C.c_int(n), C.c_int(lv),
+                                              C.c_double(dx), _p(qf), _p(F), _p(G)))
+        return F, G
+
+    def qfToQc(self, qf):
+        qf = _f64(np.atleast_2d(qf))
+        out = np.zeros_like(qf)
+        _check(lib().fcfd_qf_to_qc_host(self.handle, C.c_int(qf.shape[0]), _p(qf), _p(out)))
+        return out
+
+    def QcToqf(self, Qc):
+        Qc = _f64(np.atleast_2d(Qc))
+        out = np.zeros_like(Qc)
+        _check(lib().fcfd_qc_to_qf_host(self.handle, C.c_int(Qc.shape[0]), _p(Qc), _p(out)))
+        return out
+
+
+def recon(stencils):
+    """weno3z / weno5 (JS, Z) / weno7z / muscl3 (flags 1-4) on (n, 8) stencils -> (n, 18)."""
+    s = _f64(stencils)
+    out = np.zeros((s.shape[0], 18))
+    _check(lib().fcfd_recon_host(C.c_int(s.shape[0]), _p(s), _p(out)))
+    return out
+
+
+class Solver:
+    """A multi-block wall-bounded domain on the GPU: the state b_obj(:)%Qc/%qf of the reference
+    (SRC/block.f90:10-23) held on the device, advanced by calc_dt / solve_fluid / refresh."""
+
+    def __init__(self, mech: Mechanism, nblk: int, ncell: int, dx: float, **kw):
+        self.mech = mech
+        self.o = opts(**kw)
+        self.handle = C.c_void_p()
+        _check(lib().fcfd_create(mech.handle, C.byref(self.o), C.c_int(nblk), C.c_int(ncell),
+                                 C.c_double(dx), C.byref(self.handle)))
+        self.nblk, self.ncell, self.dx = nblk, ncell, dx
+        self.lv = lib().fcfd_lvir(self.handle)
+        self.nequ = mech.nequ
+
+    def __del__(self):
+        try:
+            lib().fcfd_destroy(self.handle)
+        except Exception:
+            pass
+
+    def init_cells(self, cells):
+        """cells: (nblk*ncell, 3+KK) rows of T [K], p [Pa], u [m/s], X."""
+        cells = _f64(cells)
+        assert cells.shape == (self.nblk * self.ncell, self.nequ)
+        _check(lib().fcfd_init_cells(self.handle, _p(cells)))
+
+    def upload(self, Qc, qf=None):
+        """Qc, qf: (nblk, ncell+2lv, nequ) in the reference's per-block layout."""
+        Qc = _f64(Qc)
+        qf = None if qf is None else _f64(qf)
+        for b in range(self.nblk):
+            _check(lib().fcfd_upload(self.handle, C.c_int(b), _p(Qc[b]),
+                                     None if qf is None else _p(qf[b])))
+
+    def download(self):
+        shape = (self.nblk, self.ncell + 2 * self.lv, self.nequ)
+        Qc, qf = np.zeros(shape), np.zeros(shape)
+        for b in range(self.nblk):
+            _check(lib().fcfd_download(self.handle, C.c_int(b), _p(Qc[b]), _p(qf[b])))
+        return Qc, qf
+
+    def calc_dt(self, cfl):
+        dt = C.c_double()
+        _check(lib().fcfd_calc_dt(self.handle, C.c_double(cfl), C.byref(dt)))
+        return dt.value
+
+    def solve_fluid(self, dt):
+        _check(lib().fcfd_solve_fluid(self.handle, C.c_double(dt)))
+
+    def refresh(self):
+        _check(lib().fcfd_refresh(self.handle))
+
+    def advance(self, cfl, nsteps, time=0.0):
+        t = C.c_double(time)
+        _check(lib().fcfd_advance(self.handle, C.c_double(cfl), C.c_int(nsteps), C.byref(t)))
+        return t.value
+
+    def set_timing(self, on: bool):
+        _check(lib().fcfd_set_timing(self.handle, C.c_int(1 if on else 0)))
+
+    def timing(self, kernel_id):
+        ms, n = C.c_double(), C.c_longlong()
+        _check(lib().fcfd_get_timing(self.handle, C.c_int(kernel_id), C.byref(ms), C.byref(n)))
+        return ms.value, n.value

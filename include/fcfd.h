@@ -1,0 +1,118 @@
+/* fcfd.h -- C ABI of libfcfd_hip.so, the MI355X (gfx950) implementation of fortranCFD's
+ * per-timestep cell and interface update.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success or a negative FCFD_E*
+ * code; fcfd_last_error() then holds a message (per calling thread).  Nothing throws across the
+ * ABI.  Host arrays use the reference's Fortran layout: per cell the (nequ = 3 + KK) column
+ * rho, rho*u, rho*E, rho*Y_k (conserved, "Qc") or rho, u, T, Y_k (primitive, "qf"), equation index
+ * fastest, cells consecutive (SRC/variables.f90:6-16, SRC/block.f90:409-413).
+ *
+ * Threading: one host thread per context; each context owns one HIP stream.  Entry points that
+ * take a context are synchronous at return unless stated otherwise.
+ *
+ * Reference call sites each entry point replaces are cited as file:line (paths relative to the
+ * reference root).
+ */
+#ifndef FCFD_H
+#define FCFD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FCFD_OK = 0,
+    FCFD_E_ARG = -1,      /* invalid argument / unsupported option          */
+    FCFD_E_IO = -2,       /* link file unreadable or malformed              */
+    FCFD_E_HIP = -3,      /* HIP runtime error (no device, launch failure)  */
+    FCFD_E_UNSUPPORTED = -4
+};
+
+typedef struct fcfd_mech fcfd_mech;
+typedef struct fcfd_ctx fcfd_ctx;
+
+/* Solver switches, same names and codes as the reference's &params namelist
+ * (SRC/params.f90:106-111) and &problem/&boundary groups. */
+typedef struct fcfd_opts {
+    int do_hydro;        /* 1: advection fluxes                                   */
+    int do_tran;         /* 1: viscous/heat/diffusive fluxes                      */
+    int do_reaction;     /* 1: chemistry source                                   */
+    int select_time;     /* 0 explicit Euler, 1 TVD-RK3                           */
+    int select_riemann;  /* 0 LxF, 1 HLL, 2 HLLC                                  */
+    int flag_acu;        /* 0 first order, 1 MUSCL, 2 WENO                        */
+    int flag_muscl;      /* 1 unlimited, 2 minmod, 3 superbee, 4 MC               */
+    int flag_weno;       /* 1 WENO3-Z, 2 WENO5-JS, 3 WENO5-Z, 4 WENO7-Z           */
+    int is_spherical;    /* must be 0 (Cartesian)                                 */
+    int flag_lbound;     /* 0 reflective wall (only value supported)              */
+    int flag_rbound;     /* 0 reflective wall (only value supported)              */
+    double atol, rtol;   /* chemistry tolerances (solver.inp atol/rtol)          */
+} fcfd_opts;
+
+/* ---- mechanism ---------------------------------------------------------------------------
+ * Reads the CHEMKIN-II link files the reference opens in set_thermo / set_trans
+ * (SRC/thermo.f90:43-52 -> CKINIT cklib.f:3972; SRC/trans.f90:35-42 -> MCINIT tranlib.f:93).
+ * tran_bin may be NULL when transport is not used. */
+int fcfd_mech_load(const char *chem_bin, const char *tran_bin, fcfd_mech **out);
+void fcfd_mech_free(fcfd_mech *m);
+int fcfd_mech_dims(const fcfd_mech *m, int *kk, int *ii);
+
+/* ---- domain context -----------------------------------------------------------------------
+ * nblk equal blocks of ncell interior cells and spacing dx, all on the current HIP device
+ * (replaces set_one_block_size / set_block_size + set_block_params, SRC/block.f90:38,91,394). */
+int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, double dx,
+                fcfd_ctx **out);
+void fcfd_destroy(fcfd_ctx *c);
+/* Ghost layers per block end for the configured scheme (SRC/params.f90:161-173). */
+int fcfd_lvir(const fcfd_ctx *c);
+
+/* Host <-> device state of block ib (0-based): (ncell + 2*lvir) cells x nequ, ghosts included,
+ * i.e. b_obj(ib)%Qc / %qf (SRC/block.f90:409-413).  qf may be NULL on upload, in which case it
+ * is rebuilt from Qc on every cell. */
+int fcfd_upload(fcfd_ctx *c, int ib, const double *Qc, const double *qf);
+int fcfd_download(fcfd_ctx *c, int ib, double *Qc, double *qf);
+
+/* Initial state from (T [K], p [Pa], u [m/s], X_1..X_KK) per interior cell, all blocks in order,
+ * as initialize_variables + set_block_params + the ghost fill of main_sim do
+ * (SRC/params.f90:305-580, SRC/block.f90:394-431, SRC/main.f90:94-114). */
+int fcfd_init_cells(fcfd_ctx *c, const double *cells);
+
+/* calc_dt (SRC/main.f90:134 -> SRC/update.f90:13-59): CFL time step over all interior cells. */
+int fcfd_calc_dt(fcfd_ctx *c, double cfl, double *dt);
+
+/* solve_fluid (SRC/main.f90:142 -> SRC/update.f90:62-97 -> SRC/timeintegration.f90:12,99). */
+int fcfd_solve_fluid(fcfd_ctx *c, double dt);
+
+/* Post-step boundary / halo / primitive refresh (SRC/main.f90:149-163). */
+int fcfd_refresh(fcfd_ctx *c);
+
+/* Device-resident stepping for production loops: nsteps x {calc_dt, solve_fluid, refresh}
+ * with dt kept on the device and the step replayed from a captured HIP graph.  *time is
+ * advanced by the sum of the steps' dt.  Equivalent to the driver loop SRC/main.f90:130-163
+ * without output. */
+int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time);
+
+/* Kernel timing with HIP events on the context's stream (0 = off).  Kernel ids: 0 flux,
+ * 1 stage update + primitive recovery, 2 CFL reduction. */
+int fcfd_set_timing(fcfd_ctx *c, int enable);
+int fcfd_get_timing(fcfd_ctx *c, int kernel_id, double *ms_total, long long *launches);
+
+/* ---- stateless entry points (host arrays in, host arrays out; run on the current device) ----
+ * calc_flux_advection on one block (SRC/flux.f90:130-373; called at
+ * SRC/timeintegration.f90:146): qf (n + 2*lv) cells x nequ -> Fplus (n+1) x nequ and
+ * godunov_state (n+1) x 3 (overwritten, not accumulated). */
+int fcfd_flux_advection_host(const fcfd_mech *m, const fcfd_opts *o, int n, int lv, double dx,
+                             const double *qf, double *Fplus, double *godunov);
+/* qfToQc / QcToqf (SRC/eos.f90:11,49) on n cells. */
+int fcfd_qf_to_qc_host(const fcfd_mech *m, int n, const double *qf, double *Qc);
+int fcfd_qc_to_qf_host(const fcfd_mech *m, int n, const double *Qc, double *qf);
+/* Reconstruction kernels (SRC/weno.f90, SRC/muscl.f90) on n 8-value stencils: out (n x 18) =
+ * weno3z(s1..4), weno5 JS(s1..6), weno5 Z, weno7z(s1..8), muscl3(s1..4) flags 1..4, (l, r) pairs,
+ * last two columns zero. */
+int fcfd_recon_host(int n, const double *s, double *out);
+
+const char *fcfd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
