@@ -8,6 +8,7 @@
 #include "../../include/fcfd.h"
 #include "fcfd_common.hpp"
 #include "fluid.hpp"
+#include "chem.hpp"
 
 namespace fcfd {
 
@@ -29,7 +30,7 @@ using namespace fcfd;
 
 namespace {
 
-constexpr int NTIMERS = 3;   // 0 flux, 1 stage end, 2 CFL
+constexpr int NTIMERS = 5;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry, 4 source Qc refresh
 
 int lvir_of(const fcfd_opts &o) {                                  // SRC/params.f90:161-173
     if (o.flag_acu == 0) return 1;
@@ -48,6 +49,7 @@ int check_opts(const fcfd_opts &o, int kk) {
     if (o.flag_acu == 1 && (o.flag_muscl < 1 || o.flag_muscl > 4)) { set_error("flag_muscl must be 1..4"); return FCFD_E_ARG; }
     if (o.flag_acu == 2 && (o.flag_weno < 1 || o.flag_weno > 4)) { set_error("flag_weno must be 1..4"); return FCFD_E_ARG; }
     if (o.do_tran) { set_error("do_tran = 1 is not available in this build"); return FCFD_E_UNSUPPORTED; }
+    if (o.do_reaction && !(o.rtol > 0.0 && o.atol > 0.0)) { set_error("do_reaction needs atol > 0 and rtol > 0"); return FCFD_E_ARG; }
     bool ok = false;
 #define FCFD_CASE(N) ok = ok || (kk == N);
     FCFD_FOR_EACH_NSP(FCFD_CASE)
@@ -79,9 +81,39 @@ struct fcfd_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_pending;   // (timer id, event index of the start event)
     size_t ev_used = 0;
-    double ms[NTIMERS] = {0, 0, 0};
-    long long cnt[NTIMERS] = {0, 0, 0};
+    double ms[NTIMERS] = {0, 0, 0, 0, 0};
+    long long cnt[NTIMERS] = {0, 0, 0, 0, 0};
+    // chemistry
+    unsigned int *d_counter = nullptr;
+    double *d_hrr = nullptr, *d_hrr_dot = nullptr;
+    ChemCellStats *d_stats = nullptr;
 };
+
+namespace {
+
+// Device copies of a mechanism's tables on the current device (created once).
+int ensure_dev_tables(const fcfd_mech *cm) {
+    fcfd_mech *m = const_cast<fcfd_mech *>(cm);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (m->d_thermo && m->device == dev) return FCFD_OK;
+    if (m->d_thermo) { set_error("mechanism tables already live on another device"); return FCFD_E_ARG; }
+    HIP_TRY(hipDeviceGetAttribute(&m->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipMalloc(&m->d_thermo, sizeof(DevThermo)));
+    HIP_TRY(hipMemcpy(m->d_thermo, &m->host.thermo, sizeof(DevThermo), hipMemcpyHostToDevice));
+    auto k = std::make_unique<DevKin>();
+    build_dev_kin(m->host, *k);
+    HIP_TRY(hipMalloc(&m->d_kin, sizeof(DevKin)));
+    HIP_TRY(hipMemcpy(m->d_kin, k.get(), sizeof(DevKin), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&m->d_counter, sizeof(unsigned int)));
+    m->device = dev;
+    return FCFD_OK;
+}
+
+// Persistent grid size: four one-wave workgroups per CU (LDS-limited).
+int chem_workgroups(const fcfd_mech *m) { return 4 * (m->ncu > 0 ? m->ncu : 256); }
+
+}  // namespace
 
 namespace {
 
@@ -180,6 +212,22 @@ hipError_t enqueue_refresh(fcfd_ctx *c) {
     return timed(c, 1, [&]() { return launch_stage_end(s, c->kk, c->stream); });
 }
 
+// reaction_macks (SRC/source.f90:17-84): chemistry on every interior cell, then Qc = qfToQc(qf).
+hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev) {
+    ChemArgs a{};
+    a.thermo = c->thermo; a.kin = static_cast<const DevKin *>(c->mech->d_kin); a.counter = c->d_counter;
+    a.ncells = c->nblk * c->ncell; a.rtol = c->o.rtol; a.atol = c->o.atol;
+    a.qf = c->qf; a.P = c->P; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S;
+    a.dt = dt; a.dt_dev = dt_dev; a.hrr = c->d_hrr; a.hrr_dot = c->d_hrr_dot; a.stats = c->d_stats;
+    a.layout = 0;
+    hipError_t e = timed(c, 3, [&]() { return launch_chem(a, c->kk, chem_workgroups(c->mech), c->stream); });
+    if (e != hipSuccess) return e;
+    return timed(c, 4, [&]() {
+        return launch_qf_to_qc_interior(c->thermo, c->kk, c->nblk, c->ncell, c->lv, c->S, c->qf,
+                                        c->Q[c->cur], c->P, c->stream);
+    });
+}
+
 void drop_graphs(fcfd_ctx *c) {
     for (auto &g : c->graph)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
@@ -237,6 +285,17 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
     HIP_TRY(hipMalloc(&c->d_cvel, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->d_cvel, 0, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->d_dt, 0, sizeof(double)));
+    if (o->do_reaction) {
+        if (int rc2 = ensure_dev_tables(m)) return rc2;
+        const size_t nint = size_t(nblk) * ncell;
+        HIP_TRY(hipMalloc(&c->d_counter, sizeof(unsigned int)));
+        HIP_TRY(hipMalloc(&c->d_hrr, sizeof(double) * nint));
+        HIP_TRY(hipMalloc(&c->d_hrr_dot, sizeof(double) * nint));
+        HIP_TRY(hipMalloc(&c->d_stats, sizeof(ChemCellStats) * nint));
+        HIP_TRY(hipMemset(c->d_hrr, 0, sizeof(double) * nint));
+        HIP_TRY(hipMemset(c->d_hrr_dot, 0, sizeof(double) * nint));
+        HIP_TRY(hipMemset(c->d_stats, 0, sizeof(ChemCellStats) * nint));
+    }
     *out = c.release();
     return FCFD_OK;
 }
@@ -248,6 +307,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     for (auto q : c->Q) (void)hipFree(q);
     (void)hipFree(c->qf); (void)hipFree(c->F); (void)hipFree(c->god); (void)hipFree(c->cells);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
+    (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -344,11 +404,14 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
     HIP_TRY(hipMemcpyAsync(c->d_time, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
     // Without a source term the post-step refresh recomputes identical primitives (QcToqf of
     // an unchanged Qc), so it is only enqueued when the chemistry has changed the state.
+    // With do_reaction the step is the driver's Lie splitting (SRC/main.f90:134-163):
+    // calc_dt, solve_fluid, solve_source, refresh.
     const bool refresh = c->o.do_reaction == 1;
     if (c->timing) {
         for (int s = 0; s < nsteps; ++s) {
             HIP_TRY(enqueue_dt(c, cfl, c->d_time));
             if (c->o.do_hydro == 1) HIP_TRY(enqueue_fluid(c, 0.0, c->d_dt));
+            if (refresh) HIP_TRY(enqueue_source(c, 0.0, c->d_dt));
             if (refresh) HIP_TRY(enqueue_refresh(c));
         }
         HIP_TRY(collect_timing(c));
@@ -361,6 +424,7 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
                 HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 hipError_t e = enqueue_dt(c, cfl, c->d_time);
                 if (e == hipSuccess && c->o.do_hydro == 1) e = enqueue_fluid(c, 0.0, c->d_dt);
+                if (e == hipSuccess && refresh) e = enqueue_source(c, 0.0, c->d_dt);
                 if (e == hipSuccess && refresh) e = enqueue_refresh(c);
                 hipError_t e2 = hipStreamEndCapture(c->stream, &g);
                 HIP_TRY(e);
@@ -375,6 +439,45 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
     }
     if (time) HIP_TRY(hipMemcpyAsync(time, c->d_time, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+int fcfd_solve_source(fcfd_ctx *c, double dt, double time, double *hrr, double *hrr_dot) {
+    (void)time;
+    if (!c) { set_error("fcfd_solve_source: invalid argument"); return FCFD_E_ARG; }
+    if (c->o.do_reaction != 1) return FCFD_OK;
+    HIP_TRY(enqueue_source(c, dt, nullptr));
+    const size_t nint = size_t(c->nblk) * c->ncell;
+    if (hrr) HIP_TRY(hipMemcpyAsync(hrr, c->d_hrr, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
+    if (hrr_dot) HIP_TRY(hipMemcpyAsync(hrr_dot, c->d_hrr_dot, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(collect_timing(c));
+    return FCFD_OK;
+}
+
+int fcfd_chem_stats(fcfd_ctx *c, int *per_cell) {
+    if (!c || !per_cell || !c->d_stats) { set_error("fcfd_chem_stats: no chemistry on this context"); return FCFD_E_ARG; }
+    HIP_TRY(hipMemcpyAsync(per_cell, c->d_stats, sizeof(ChemCellStats) * size_t(c->nblk) * c->ncell,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double *rho, const double *dt,
+                    double *T, double *Y, int *stats, void *stream) {
+    if (!m || !o || n < 0 || (n > 0 && (!rho || !dt || !T || !Y))) { set_error("fcfd_chem_batch: invalid argument"); return FCFD_E_ARG; }
+    if (!(o->rtol > 0.0 && o->atol > 0.0)) { set_error("fcfd_chem_batch: atol and rtol must be positive"); return FCFD_E_ARG; }
+    if (n == 0) return FCFD_OK;
+    if (int rc = check_opts(fcfd_opts{1, 0, 1, 1, 2, 2, 0, 2, 0, 0, 0, o->atol, o->rtol}, m->host.kk)) return rc;
+    if (int rc = ensure_dev_tables(m)) return rc;
+    ChemArgs a{};
+    a.thermo = m->d_thermo; a.kin = static_cast<const DevKin *>(m->d_kin); a.counter = m->d_counter;
+    a.ncells = n; a.rtol = o->rtol; a.atol = o->atol;
+    a.b_rho = rho; a.b_dt = dt; a.b_T = T; a.b_y = Y;
+    a.stats = reinterpret_cast<ChemCellStats *>(stats);
+    a.layout = 1;
+    HIP_TRY(launch_chem(a, m->host.kk, chem_workgroups(m), static_cast<hipStream_t>(stream)));
+    HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     return FCFD_OK;
 }
 
@@ -473,6 +576,39 @@ static int conv_host(const fcfd_mech *m, int n, const double *in, double *out, b
 
 int fcfd_qf_to_qc_host(const fcfd_mech *m, int n, const double *qf, double *Qc) { return conv_host(m, n, qf, Qc, true); }
 int fcfd_qc_to_qf_host(const fcfd_mech *m, int n, const double *Qc, double *qf) { return conv_host(m, n, Qc, qf, false); }
+
+int fcfd_chem_step_host(const fcfd_mech *m, const fcfd_opts *o, int n, const double *dt,
+                        const double *rho, double *yT, int *stats) {
+    if (!m || !o || n < 1 || !dt || !rho || !yT) { set_error("fcfd_chem_step_host: invalid argument"); return FCFD_E_ARG; }
+    const int kk = m->host.kk;
+    DevBuf dr, dd, dT, dY, ds;
+    HIP_TRY(hipMalloc(&dr.p, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&dd.p, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&dT.p, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&dY.p, sizeof(double) * n * kk));
+    HIP_TRY(hipMalloc(&ds.p, sizeof(ChemCellStats) * n));
+    std::vector<double> hT(n), hY(size_t(n) * kk);
+    for (int c = 0; c < n; ++c) {
+        hT[c] = yT[size_t(c) * (kk + 1) + kk];
+        for (int k = 0; k < kk; ++k) hY[size_t(k) * n + c] = yT[size_t(c) * (kk + 1) + k];
+    }
+    HIP_TRY(hipMemcpy(dr.p, rho, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dd.p, dt, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dT.p, hT.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dY.p, hY.data(), sizeof(double) * hY.size(), hipMemcpyHostToDevice));
+    int rc = fcfd_chem_batch(m, o, n, static_cast<double *>(dr.p), static_cast<double *>(dd.p),
+                             static_cast<double *>(dT.p), static_cast<double *>(dY.p),
+                             static_cast<int *>(ds.p), nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(hT.data(), dT.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hY.data(), dY.p, sizeof(double) * hY.size(), hipMemcpyDeviceToHost));
+    if (stats) HIP_TRY(hipMemcpy(stats, ds.p, sizeof(ChemCellStats) * n, hipMemcpyDeviceToHost));
+    for (int c = 0; c < n; ++c) {
+        yT[size_t(c) * (kk + 1) + kk] = hT[c];
+        for (int k = 0; k < kk; ++k) yT[size_t(c) * (kk + 1) + k] = hY[size_t(k) * n + c];
+    }
+    return FCFD_OK;
+}
 
 int fcfd_recon_host(int n, const double *s, double *out) {
     if (!s || !out || n < 1) { set_error("fcfd_recon_host: invalid argument"); return FCFD_E_ARG; }

@@ -1,0 +1,57 @@
+// chem.hpp -- kinetics tables and launchers of the chemistry source kernels (chem.hip).
+#pragma once
+
+#include "fcfd_common.hpp"
+
+namespace fcfd {
+
+// Reaction data in the form CKRATT / CKRATX / CKWYR consume it (cklib.f:6630-6993, 9107-9204),
+// re-indexed 0-based with one row per reaction, plus per-species accumulation lists that keep
+// CKWYR's summation order (slot-major, then reaction).
+struct DevKin {
+    int ii, kk, nthb, nfal, ncsr;
+    double small, exparg;
+    double A[MAXR], B[MAXR], E[MAXR];   // PAR(1..3): A, temperature exponent, E/R
+    int rev[MAXR];                      // NSPEC > 0: reverse rate from equilibrium
+    int nus[MAXR];                      // sum of the six stoichiometric coefficients
+    int nu[MAXR][6], nk[MAXR][6];       // coefficients and 0-based species (-1: empty slot)
+    int tb[MAXR], fo[MAXR];             // third-body / fall-off table row, -1 if none
+    int ntbs[MAXR], tbk[MAXR][MAXTB];   // enhanced third bodies (species, 0-based)
+    double tba[MAXR][MAXTB];            // their efficiencies
+    int ifop[MAXR], kfal[MAXR];         // fall-off form (1 Lindemann, 2 SRI, 3/4 Troe), [M] species
+    double fpar[MAXR][8];
+    int csr_start[MAXK + 1];
+    int csr_rxn[MAXR * 6];
+    double csr_nu[MAXR * 6];
+};
+
+void build_dev_kin(const Mech &m, DevKin &k);
+
+// Per-cell integrator counters (optional output).
+struct ChemCellStats {
+    int steps, rejects, newton, failed;
+};
+
+struct ChemArgs {
+    const DevThermo *thermo;
+    const DevKin *kin;
+    unsigned int *counter;       // work-queue head, zeroed before launch
+    int ncells;
+    double rtol, atol;
+    // layout A (solve_source): primitive state of a padded SoA domain, updated in place
+    double *qf;
+    long P;
+    int ncell, lv, S;
+    double dt;
+    const double *dt_dev;        // device dt (overrides dt when non-null)
+    double *hrr, *hrr_dot;       // per interior cell (may be null)
+    // layout B (batched cells): arrays of length ncells; y is [KK][ncells]
+    const double *b_rho, *b_dt;
+    double *b_T, *b_y;
+    ChemCellStats *stats;        // may be null
+    int layout;                  // 0: domain, 1: batch
+};
+
+hipError_t launch_chem(const ChemArgs &a, int nsp, int n_workgroups, hipStream_t st);
+
+}  // namespace fcfd

@@ -53,6 +53,7 @@ struct Mech {
 };
 
 int load_mech(const char *chem_bin, const char *tran_bin, Mech &m, std::string &err);
+void thermo_h298(DevThermo &t);
 
 void set_error(const std::string &msg);
 
@@ -60,4 +61,10 @@ void set_error(const std::string &msg);
 
 struct fcfd_mech {
     fcfd::Mech host;
+    // device copies of the tables, created on first use on the current device
+    int device = -1;
+    int ncu = 0;
+    fcfd::DevThermo *d_thermo = nullptr;
+    void *d_kin = nullptr;               // fcfd::DevKin
+    unsigned int *d_counter = nullptr;   // work queue of batched chemistry calls
 };

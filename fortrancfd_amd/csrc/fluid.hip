@@ -340,6 +340,34 @@ __global__ void k_qc_to_qf(const DevThermo *__restrict__ m, int n, const double 
     qc_to_qf_store<NSP>(m, Qv, qf, P, c);
 }
 
+// qfToQc on the interior columns of a padded domain (reaction_macks' final loop,
+// SRC/source.f90:78-82).
+template <int NSP>
+__global__ void k_qf_to_qc_interior(const DevThermo *__restrict__ m, int nblk, int ncell, int lv, int S,
+                                    const double *__restrict__ qf, double *__restrict__ Qc, long P) {
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= long(nblk) * ncell) return;
+    const long c = long(t / ncell) * S + lv + (t % ncell);
+    double y[NSP];
+    const double rho = qf[c], u = qf[P + c], T = qf[2 * P + c];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) y[k] = qf[(3 + k) * P + c];
+    double ya[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) ya[k] = y[k];
+    adjust_sum<NSP>(ya);
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) Qc[(3 + k) * P + c] = rho * ya[k];
+    ytx<NSP>(m, y);
+    adjust_sum<NSP>(y);
+    const double wtm = mix_w<NSP>(m, y);
+    const MixSums ms = mix_sums<NSP>(m, y, T, true, false, false);
+    const double e = ms.ubml * 1.e-4 / wtm;
+    Qc[c] = rho;
+    Qc[P + c] = rho * u;
+    Qc[2 * P + c] = rho * (e + 0.5 * (u * u));
+}
+
 __global__ void k_recon(int n, const double *__restrict__ s, double *__restrict__ out) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
@@ -438,6 +466,16 @@ hipError_t launch_qf_to_qc(const DevThermo *m, int nsp, int n, const double *qf,
 hipError_t launch_qc_to_qf(const DevThermo *m, int nsp, int n, const double *Qc, double *qf, long P, hipStream_t st) {
     const dim3 grid(unsigned((n + 63) / 64)), blk(64);
 #define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_qc_to_qf<N>), grid, blk, 0, st, m, n, Qc, qf, P); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_qf_to_qc_interior(const DevThermo *m, int nsp, int nblk, int ncell, int lv, int S,
+                                    const double *qf, double *Qc, long P, hipStream_t st) {
+    const long n = long(nblk) * ncell;
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_qf_to_qc_interior<N>), grid, blk, 0, st, m, nblk, ncell, lv, S, qf, Qc, P); return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;

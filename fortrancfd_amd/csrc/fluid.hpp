@@ -52,6 +52,8 @@ hipError_t launch_dt(const DtArgs &a, int nsp, hipStream_t st);
 hipError_t launch_init(const InitArgs &a, int nsp, hipStream_t st);
 hipError_t launch_qf_to_qc(const DevThermo *m, int nsp, int n, const double *qf, double *Qc, long P, hipStream_t st);
 hipError_t launch_qc_to_qf(const DevThermo *m, int nsp, int n, const double *Qc, double *qf, long P, hipStream_t st);
+hipError_t launch_qf_to_qc_interior(const DevThermo *m, int nsp, int nblk, int ncell, int lv, int S,
+                                    const double *qf, double *Qc, long P, hipStream_t st);
 hipError_t launch_recon(int n, const double *s, double *out, hipStream_t st);
 
 }  // namespace fcfd

@@ -1,0 +1,81 @@
+// kin_build.cpp -- device kinetics tables from the CHEMKIN-II link-file data (host side).
+#include <cmath>
+#include <cstring>
+
+#include "chem.hpp"
+
+namespace fcfd {
+
+void build_dev_kin(const Mech &m, DevKin &k) {
+    std::memset(&k, 0, sizeof(DevKin));
+    k.ii = m.ii;
+    k.kk = m.kk;
+    k.nthb = int(m.itb.size());
+    k.nfal = int(m.ifl.size());
+    k.small = std::pow(10.0, -300);        // CKINIT SMALL / EXPARG (cklib.f:4054-4057)
+    k.exparg = std::log(std::pow(10.0, 300));
+    const int np1 = m.npar + 1;
+    for (int i = 0; i < m.ii; ++i) {
+        k.A[i] = m.par[size_t(i) * np1 + 0];
+        k.B[i] = m.par[size_t(i) * np1 + 1];
+        k.E[i] = m.par[size_t(i) * np1 + 2];
+        k.rev[i] = m.ns[i] > 0 ? 1 : 0;
+        int s = 0;
+        for (int n = 0; n < 6; ++n) {
+            k.nu[i][n] = m.nu[size_t(i) * 6 + n];
+            k.nk[i][n] = m.nk[size_t(i) * 6 + n] - 1;
+            s += k.nu[i][n];
+        }
+        k.nus[i] = s;
+        k.tb[i] = -1;
+        k.fo[i] = -1;
+    }
+    for (int t = 0; t < k.nthb; ++t) {
+        const int i = m.itb[t] - 1;
+        k.tb[i] = t;
+        k.ntbs[t] = m.ntbs[t];
+        for (int l = 0; l < m.mxtb && l < MAXTB; ++l) {
+            k.tbk[t][l] = m.nktb[size_t(t) * m.mxtb + l] - 1;
+            k.tba[t][l] = m.aik[size_t(t) * m.mxtb + l];
+        }
+    }
+    for (int f = 0; f < k.nfal; ++f) {
+        const int i = m.ifl[f] - 1;
+        k.fo[i] = f;
+        k.ifop[f] = m.ifo[f];
+        k.kfal[f] = m.kfl[f] - 1;
+        for (int c = 0; c < 8 && c < m.nfar; ++c) k.fpar[f][c] = m.fpar[size_t(f) * m.nfar + c];
+    }
+    // production-rate lists per species in CKWYR's order: slot-major, reaction-minor
+    int e = 0;
+    for (int sp = 0; sp < m.kk; ++sp) {
+        k.csr_start[sp] = e;
+        for (int n = 0; n < 6; ++n)
+            for (int i = 0; i < m.ii; ++i)
+                if (k.nk[i][n] == sp) {
+                    k.csr_rxn[e] = i;
+                    k.csr_nu[e] = double(k.nu[i][n]);
+                    ++e;
+                }
+    }
+    k.csr_start[m.kk] = e;
+    k.ncsr = e;
+}
+
+// CKHMS(298.15) (cklib.f:3598-3618): formation enthalpies for the heat-release diagnostics.
+void thermo_h298(DevThermo &t) {
+    const double T = 298.15, rut = T * t.ru;
+    const double t2 = T * T, t3 = T * t2, t4 = t2 * t2;
+    const double tn1 = T / 2.0, tn2 = t2 / 3.0, tn3 = t3 / 4.0, tn4 = t4 / 5.0;
+    for (int k = 0; k < t.kk; ++k) {
+        const double *a = T > t.tmid[k] ? t.ahi[k] : t.alo[k];
+        double s = a[0];
+        s = s + tn1 * a[1];
+        s = s + tn2 * a[2];
+        s = s + tn3 * a[3];
+        s = s + tn4 * a[4];
+        t.h298[k] = rut * (s + a[5] / T) / t.wt[k];
+    }
+}
+
+}  // namespace fcfd

@@ -136,6 +136,8 @@ int load_mech(const char *chem_bin, const char *tran_bin, Mech &m, std::string &
         }
     }
 
+    thermo_h298(d);
+
     if (tran_bin) {
         RecordReader t;
         if (!t.open(tran_bin)) { err = std::string("cannot read ") + tran_bin; return -2; }

@@ -105,6 +105,29 @@ class Mechanism:
         _check(lib().fcfd_qc_to_qf_host(self.handle, C.c_int(Qc.shape[0]), _p(Qc), _p(out)))
         return out
 
+    def macks(self, dt, rho, yT, rtol=1e-5, atol=1e-13):
+        """macks(dt, nspe+1, yiT, rho) (SRC/source.f90:58) on n cells: yT rows (Y..., T), dt and
+        rho [kg/m^3] per cell.  Returns (yT_new, stats[n, 4] = steps, rejects, newton, failed)."""
+        yT = _f64(np.atleast_2d(yT)).copy()
+        n = yT.shape[0]
+        dt = _f64(np.broadcast_to(dt, (n,)))
+        rho = _f64(np.broadcast_to(rho, (n,)))
+        st = np.zeros((n, 4), np.int32)
+        o = opts(atol=atol, rtol=rtol, do_reaction=1)
+        _check(lib().fcfd_chem_step_host(self.handle, C.byref(o), C.c_int(n), _p(dt), _p(rho),
+                                         _p(yT), st.ctypes.data_as(C.POINTER(C.c_int))))
+        return yT, st
+
+    def chem_batch_device(self, rho, dt, T, Y, stats=None, rtol=1e-5, atol=1e-13, stream=None):
+        """Batched chemistry on device tensors (torch): rho, dt, T of shape [n], Y of shape [KK, n]
+        (float64, contiguous, same device); T and Y are updated in place."""
+        n = int(T.numel())
+        o = opts(atol=atol, rtol=rtol, do_reaction=1)
+        ptr = lambda t: C.c_void_p(t.data_ptr())
+        _check(lib().fcfd_chem_batch(self.handle, C.byref(o), C.c_int(n), ptr(rho), ptr(dt), ptr(T),
+                                     ptr(Y), ptr(stats) if stats is not None else None,
+                                     C.c_void_p(stream) if stream else None))
+
 
 def recon(stencils):
     """weno3z / weno5 (JS, Z) / weno7z / muscl3 (flags 1-4) on (n, 8) stencils -> (n, 18)."""
@@ -165,6 +188,19 @@ class Solver:
 
     def refresh(self):
         _check(lib().fcfd_refresh(self.handle))
+
+    def solve_source(self, dt, time=0.0):
+        """reaction_macks on every interior cell; returns (HRR, HRR_dot) per interior cell."""
+        n = self.nblk * self.ncell
+        hrr, hrd = np.zeros(n), np.zeros(n)
+        _check(lib().fcfd_solve_source(self.handle, C.c_double(dt), C.c_double(time), _p(hrr),
+                                       _p(hrd)))
+        return hrr, hrd
+
+    def chem_stats(self):
+        st = np.zeros((self.nblk * self.ncell, 4), np.int32)
+        _check(lib().fcfd_chem_stats(self.handle, st.ctypes.data_as(C.POINTER(C.c_int))))
+        return st
 
     def advance(self, cfl, nsteps, time=0.0):
         t = C.c_double(time)

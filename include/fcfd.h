@@ -82,6 +82,17 @@ int fcfd_calc_dt(fcfd_ctx *c, double cfl, double *dt);
 /* solve_fluid (SRC/main.f90:142 -> SRC/update.f90:62-97 -> SRC/timeintegration.f90:12,99). */
 int fcfd_solve_fluid(fcfd_ctx *c, double dt);
 
+/* solve_source (SRC/main.f90:145 -> SRC/update.f90:100-150 -> reaction_macks,
+ * SRC/source.f90:17-84): integrates every interior cell's (Y, T) over dt at fixed density,
+ * renormalises Y, writes T and Y into the primitive state and rebuilds the interior conserved
+ * state (qfToQc).  hrr / hrr_dot (may be NULL) receive eos_obj(:)%HRR and %HRR_dot, one value per
+ * interior cell, blocks in order.  Requires do_reaction = 1.  The time argument is unused (the
+ * reference passes it for the ignition source, which is not part of this path). */
+int fcfd_solve_source(fcfd_ctx *c, double dt, double time, double *hrr, double *hrr_dot);
+/* Per-cell integrator counters of the last chemistry call: 4 ints per interior cell
+ * (accepted steps, rejected attempts, Newton iterations, failed flag). */
+int fcfd_chem_stats(fcfd_ctx *c, int *per_cell);
+
 /* Post-step boundary / halo / primitive refresh (SRC/main.f90:149-163). */
 int fcfd_refresh(fcfd_ctx *c);
 
@@ -109,6 +120,17 @@ int fcfd_qc_to_qf_host(const fcfd_mech *m, int n, const double *Qc, double *qf);
  * weno3z(s1..4), weno5 JS(s1..6), weno5 Z, weno7z(s1..8), muscl3(s1..4) flags 1..4, (l, r) pairs,
  * last two columns zero. */
 int fcfd_recon_host(int n, const double *s, double *out);
+
+/* The macks call (SRC/source.f90:58, macks(dt, nspe+1, yiT, rho)) on n independent cells:
+ * yT is n rows of (Y_1..Y_KK, T), updated in place; dt and rho (kg/m^3) per cell; stats
+ * (may be NULL) receives 4 ints per cell as fcfd_chem_stats.  Uses o->atol / o->rtol. */
+int fcfd_chem_step_host(const fcfd_mech *m, const fcfd_opts *o, int n, const double *dt,
+                        const double *rho, double *yT, int *stats);
+/* Same on DEVICE arrays of the current device (no copies): rho[n], dt[n], T[n] in/out,
+ * Y[KK][n] in/out (species-major), stats[4n] or NULL; stream is a hipStream_t or NULL.
+ * Calls that share a mechanism must not overlap (one work-queue counter per mechanism). */
+int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double *rho,
+                    const double *dt, double *T, double *Y, int *stats, void *stream);
 
 const char *fcfd_last_error(void);
 

@@ -33,6 +33,7 @@ struct fo_mech {
     int *itb, *ntbs, *nktb;/* [nthb], [nthb], [nthb][mxtb]           */
     double *aik;           /* [nthb][mxtb]                           */
     double *h298;          /* [kk] CKHMS(298.15) erg/g               */
+    int *ncf;              /* [kk][mm] element counts                */
     /* transport */
     int has_tran, no;
     double patmos;
@@ -103,9 +104,10 @@ fo_mech *fo_mech_load(const char *chem_bin, const char *tran_bin) {
     m->wt = calloc(kk, sizeof(double)); m->nt = calloc(kk, sizeof(int));
     m->tt = calloc((size_t)kk * mxtp, sizeof(double));
     m->aa = calloc((size_t)kk * (mxtp - 1) * ncp2, sizeof(double));
+    m->ncf = calloc((size_t)kk * mm, sizeof(int));
     for (int k = 0; k < kk; ++k) {
         p += 16;
-        for (int e = 0; e < mm; ++e) (void)geti(&p);
+        for (int e = 0; e < mm; ++e) m->ncf[k * mm + e] = geti(&p);
         (void)geti(&p); (void)geti(&p);
         m->wt[k] = getd(&p);
         m->nt[k] = geti(&p);
@@ -184,12 +186,14 @@ void fo_mech_free(fo_mech *m) {
     if (!m) return;
     void *ptrs[] = {m->wt, m->nt, m->tt, m->aa, m->ns, m->nr, m->par, m->nu, m->nk, m->ifl, m->ifo,
                     m->kfl, m->fpar, m->itb, m->ntbs, m->nktb, m->aik, m->h298, m->twt, m->cofeta,
-                    m->coflam, m->cofd};
+                    m->coflam, m->cofd, m->ncf};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
     free(m);
 }
 
 int fo_mech_kk(const fo_mech *m) { return m->kk; }
+int fo_mech_mm(const fo_mech *m) { return m->mm; }
+void fo_mech_ncf(const fo_mech *m, int *ncf) { memcpy(ncf, m->ncf, sizeof(int) * m->kk * m->mm); }
 int fo_mech_ii(const fo_mech *m) { return m->ii; }
 void fo_mech_weights(const fo_mech *m, double *wt) { memcpy(wt, m->wt, sizeof(double) * m->kk); }
 void fo_mech_nasa(const fo_mech *m, double *tt, double *aa) {
@@ -951,7 +955,8 @@ void fo_solve_fluid(fo_domain *d, double dt) {
 /* CKLIB kinetics                                                                                */
 /* ============================================================================================ */
 /* CKRATT + CKYTCR + CKRATX (cklib.f:6630-6993, 9533-9583): forward/reverse rates of progress. */
-static void rop(const fo_mech *m, double rho, double T, const double *Y, double *qf, double *qr) {
+static void rop_k(const fo_mech *m, double rho, double T, const double *Y, double *qf, double *qr,
+                  double *kf_eff, double *kr_eff) {
     const int kk = m->kk, ii = m->ii, mxsp = m->mxsp, np1 = m->npar + 1;
     double rkft[512], rkrt[512], eqk[512], smh[64], c[64], ctb[512];
     double alogt = log(T);
@@ -1039,7 +1044,12 @@ static void rop(const fo_mech *m, double rho, double T, const double *Y, double 
         }
         qf[i] = f;
         qr[i] = r;
+        if (kf_eff) { kf_eff[i] = rkft[i]; kr_eff[i] = rkrt[i]; }
     }
+}
+
+static void rop(const fo_mech *m, double rho, double T, const double *Y, double *qf, double *qr) {
+    rop_k(m, rho, T, Y, qf, qr, NULL, NULL);
 }
 
 void fo_ckwyr(const fo_mech *m, double rho, double T, const double *Y, double *wdot, double *qf_,
@@ -1074,4 +1084,269 @@ void fo_ckcdyr(const fo_mech *m, double rho, double T, const double *Y, double *
                 ddot[kp - 1] = ddot[kp - 1] + qr[i] * nup;
             }
         }
+}
+
+/* ============================================================================================ */
+/* Chemistry source: constant-volume adiabatic stiff integrator (this project's own design; the  */
+/* reference calls macks(dt, nspe+1, yiT, rho) from module ODEs, SRC/source.f90:58, whose source */
+/* is not part of the reference).  Same algorithm as the HIP kernel k_chem:                      */
+/*   state y = (Y_1..Y_KK, T) at fixed rho; f from CKWYR rates;                                  */
+/*   3-stage L-stable SDIRK of order 3 (Alexander 1977, gamma = 0.43586652150845900) with an     */
+/*   embedded order-2 solution (b^ from the first two stages), simplified Newton on every stage    */
+/*   with M = I - h*gamma*J and an LU with partial pivoting, error = M^{-1} h*sum (b-b^)_i k_i;    */
+/*   analytic mass-action Jacobian                                                               */
+/*   with frozen third-body / fall-off factors and a finite-difference temperature column;       */
+/*   T finally corrected so that sum Y_k u_k(T)/W_k keeps its initial value (energy exact).      */
+/* ============================================================================================ */
+#define CH_GAMMA 0.435866521508459
+#define CH_A21 0.2820667392457705
+#define CH_B1 1.20849664917601
+#define CH_B2 -0.6443631706844692
+#define CH_E1 0.43586652150845895
+#define CH_E2 -0.8717330430169181
+#define CH_E3 0.435866521508459
+#define CH_KAPPA 0.03
+#define CH_MAXIT 7
+#define CH_MAXSTEPS 5000
+
+void fo_chem_rhs(const fo_mech *m, double rho, const double *y, double *f) {
+    const int kk = m->kk;
+    const double T = y[kk];
+    double w[64], u[64], cp[64];
+    fo_ckwyr(m, rho, T, y, w, NULL, NULL);
+    fo_ckuml(m, T, u);
+    fo_ckcpml(m, T, cp);
+    double cvm = 0.0, su = 0.0;
+    for (int k = 0; k < kk; ++k) {
+        cvm = cvm + y[k] * (cp[k] - m->ru) / m->wt[k];
+        su = su + u[k] * w[k];
+        f[k] = w[k] * m->wt[k] / rho;
+    }
+    f[kk] = -su / (rho * cvm);
+}
+
+void fo_chem_jac(const fo_mech *m, double rho, const double *y, double *J) {
+    const int kk = m->kk, n = kk + 1, mxsp = m->mxsp;
+    const double T = y[kk];
+    double qf[512], qr[512], kf[512], kr[512], c[64], u[64], cp[64], f[65], fp[65], yp[65];
+    static double JC[64 * 64];
+    rop_k(m, rho, T, y, qf, qr, kf, kr);
+    for (int k = 0; k < kk; ++k) c[k] = rho * y[k] / m->wt[k];
+    memset(JC, 0, sizeof(JC));
+    for (int i = 0; i < m->ii; ++i) {
+        const int *nu = m->nu + i * mxsp, *nk = m->nk + i * mxsp;
+        if (nu[0] == 0) continue;
+        double dq[6] = {0, 0, 0, 0, 0, 0};
+        for (int s = 0; s < 6; ++s) {
+            if (nk[s] == 0) continue;
+            const int lo = s < 3 ? 0 : 3;
+            const int a = abs(nu[s]);
+            double d = (double)a * ipow(c[nk[s] - 1], a - 1);
+            for (int t = lo; t < lo + 3; ++t)
+                if (t != s && nk[t] != 0) d = d * ipow(c[nk[t] - 1], abs(nu[t]));
+            dq[s] = s < 3 ? kf[i] * d : -(kr[i] * d);
+        }
+        for (int s = 0; s < 6; ++s) {
+            if (nk[s] == 0) continue;
+            const int col = nk[s] - 1;
+            for (int t = 0; t < 6; ++t)
+                if (nk[t] != 0) JC[(nk[t] - 1) * 64 + col] += (double)nu[t] * dq[s];
+        }
+    }
+    fo_ckuml(m, T, u);
+    fo_ckcpml(m, T, cp);
+    fo_chem_rhs(m, rho, y, f);
+    double cvm = 0.0;
+    for (int k = 0; k < kk; ++k) cvm = cvm + y[k] * (cp[k] - m->ru) / m->wt[k];
+    for (int r = 0; r < kk; ++r)
+        for (int k = 0; k < kk; ++k) J[r * n + k] = JC[r * 64 + k] * m->wt[r] / m->wt[k];
+    for (int k = 0; k < kk; ++k) {
+        double s = 0.0;
+        for (int j = 0; j < kk; ++j) s = s + u[j] * JC[j * 64 + k];
+        s = s * rho / m->wt[k];
+        J[kk * n + k] = -s / (rho * cvm) - f[kk] * ((cp[k] - m->ru) / m->wt[k]) / cvm;
+    }
+    const double dT = T * 1.0e-6;
+    memcpy(yp, y, sizeof(double) * n);
+    yp[kk] = T + dT;
+    fo_chem_rhs(m, rho, yp, fp);
+    for (int r = 0; r < n; ++r) J[r * n + kk] = (fp[r] - f[r]) / dT;
+}
+
+/* LU of A (n x n, row-major) with partial pivoting, rows not swapped: rank[r] is the elimination
+ * step at which row r was the pivot, perm[k] the pivot row of step k; ties go to the lower row. */
+static int ch_lu(double *A, int n, int *perm, int *rank) {
+    for (int r = 0; r < n; ++r) rank[r] = n;
+    for (int k = 0; k < n; ++k) {
+        int p = -1;
+        double best = -1.0;
+        for (int r = 0; r < n; ++r)
+            if (rank[r] == n && fabs(A[r * n + k]) > best) { best = fabs(A[r * n + k]); p = r; }
+        if (!(best > 0.0)) return -1;
+        rank[p] = k;
+        perm[k] = p;
+        const double piv = A[p * n + k];
+        for (int r = 0; r < n; ++r) {
+            if (rank[r] != n) continue;
+            const double l = A[r * n + k] / piv;
+            A[r * n + k] = l;
+            for (int j = k + 1; j < n; ++j) A[r * n + j] = A[r * n + j] - l * A[p * n + j];
+        }
+    }
+    return 0;
+}
+
+/* Solves A x = b in place (b -> x, x in natural order). */
+static void ch_solve(const double *A, int n, const int *perm, const int *rank, double *b) {
+    double x[65];
+    for (int k = 0; k < n; ++k) {
+        const double bp = b[perm[k]];
+        for (int r = 0; r < n; ++r)
+            if (rank[r] > k) b[r] = b[r] - A[r * n + k] * bp;
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        const int p = perm[k];
+        x[k] = b[p] / A[p * n + k];
+        for (int r = 0; r < n; ++r)
+            if (rank[r] < k) b[r] = b[r] - A[r * n + k] * x[k];
+    }
+    memcpy(b, x, sizeof(double) * n);
+}
+
+static double ch_norm(const double *v, const double *w, int n) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = s + (v[i] * w[i]) * (v[i] * w[i]);
+    return sqrt(s / n);
+}
+
+/* Mixture internal energy per unit mass (erg/g) and cv (erg/(g K)) at T for composition y. */
+static void ch_energy(const fo_mech *m, const double *y, double T, double *e, double *cv) {
+    double u[64], cp[64];
+    fo_ckuml(m, T, u);
+    fo_ckcpml(m, T, cp);
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < m->kk; ++k) {
+        s = s + y[k] * u[k] / m->wt[k];
+        c = c + y[k] * (cp[k] - m->ru) / m->wt[k];
+    }
+    *e = s;
+    *cv = c;
+}
+
+int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double rtol, double atol,
+                 fo_chem_stats *st) {
+    const int kk = m->kk, n = kk + 1;
+    const double rho = rho_si * 1.e-3, g = CH_GAMMA;
+    double y[65], w[65], f[65], z[65], base[65], k[3][65], r[65], e[65], ya[64];
+    static double J[65 * 65], M[65 * 65];
+    int perm[65], rank[65];
+    fo_chem_stats s = {0, 0, 0, 0, 0};
+    memcpy(y, yT, sizeof(double) * n);
+    double e0, cv0;
+    memcpy(ya, y, sizeof(double) * kk);
+    fo_adjust_sum(kk, ya);
+    ch_energy(m, ya, y[kk], &e0, &cv0);
+    double t = 0.0, h = dt;
+    while (t < dt && s.steps + s.rejects < CH_MAXSTEPS) {
+        if (h > dt - t) h = dt - t;
+        for (int i = 0; i < kk; ++i) w[i] = 1.0 / (atol + rtol * fabs(y[i]));
+        w[kk] = 1.0 / (rtol * fabs(y[kk]));
+        fo_chem_jac(m, rho, y, J);
+        s.jacobians++;
+        for (;;) {
+            const double hg = h * g;
+            for (int i = 0; i < n * n; ++i) M[i] = -hg * J[i];
+            for (int i = 0; i < n; ++i) M[i * n + i] = M[i * n + i] + 1.0;
+            int ok = ch_lu(M, n, perm, rank) == 0;
+            /* stage s: z = y + h sum_{j<s} a_sj k_j + h g f(z); predictor z = base + h g k_{s-1} */
+            for (int st = 0; st < 3 && ok; ++st) {
+                for (int i = 0; i < n; ++i) {
+                    double b = y[i];
+                    if (st == 1) b = b + h * CH_A21 * k[0][i];
+                    if (st == 2) b = b + h * CH_B1 * k[0][i] + h * CH_B2 * k[1][i];
+                    base[i] = b;
+                    z[i] = st == 0 ? b : b + hg * k[st - 1][i];
+                }
+                double prev = 0.0;
+                int conv = 0;
+                for (int it = 0; it < CH_MAXIT; ++it) {
+                    fo_chem_rhs(m, rho, z, f);
+                    for (int i = 0; i < n; ++i) r[i] = -(z[i] - base[i] - hg * f[i]);
+                    ch_solve(M, n, perm, rank, r);
+                    for (int i = 0; i < n; ++i) z[i] = z[i] + r[i];
+                    s.newton++;
+                    const double nrm = ch_norm(r, w, n);
+                    if (!(nrm == nrm) || (it > 0 && nrm > 0.9 * prev)) break;
+                    if (nrm <= CH_KAPPA) { conv = 1; break; }
+                    prev = nrm;
+                }
+                if (!conv) { ok = 0; break; }
+                for (int i = 0; i < n; ++i) k[st][i] = (z[i] - base[i]) / hg;
+            }
+            if (!ok) {
+                s.rejects++;
+                h = h * 0.25;
+                if (s.steps + s.rejects >= CH_MAXSTEPS) break;
+                continue;
+            }
+            for (int i = 0; i < n; ++i) e[i] = h * (CH_E1 * k[0][i] + CH_E2 * k[1][i] + CH_E3 * k[2][i]);
+            ch_solve(M, n, perm, rank, e);
+            const double err = ch_norm(e, w, n);
+            const double fac = 0.9 * pow(fmax(err, 1.0e-10), -1.0 / 3.0);
+            if (err <= 1.0) {
+                memcpy(y, z, sizeof(double) * n);
+                t = t + h;
+                s.steps++;
+                h = h * fmin(4.0, fmax(0.2, fac));
+                break;
+            }
+            s.rejects++;
+            h = h * fmax(0.2, fac);
+            if (s.steps + s.rejects >= CH_MAXSTEPS) break;
+        }
+    }
+    if (t < dt) s.failed = 1;
+    /* energy correction: sum Y u(T)/W with the renormalised composition equals its initial value */
+    memcpy(ya, y, sizeof(double) * kk);
+    fo_adjust_sum(kk, ya);
+    double T = y[kk];
+    for (int it = 0; it < 20; ++it) {
+        double ek, cvk;
+        ch_energy(m, ya, T, &ek, &cvk);
+        const double d = (e0 - ek) / cvk;
+        T = T + d;
+        if (fabs(d) <= 1.0e-12 * T) break;
+    }
+    y[kk] = T;
+    memcpy(yT, y, sizeof(double) * n);
+    if (st) *st = s;
+    return s.failed ? -1 : 0;
+}
+
+/* reaction_macks (SRC/source.f90:17-84) on every interior cell of a domain; hrr / hrr_dot hold
+ * one value per interior cell (block-major), as eos_obj(i)%HRR / %HRR_dot. */
+void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *hrr,
+                     double *hrr_dot, int *nfail) {
+    const fo_mech *m = d->m;
+    const int kk = m->kk;
+    int fails = 0;
+    for (int ib = 0; ib < d->nblk; ++ib)
+        for (int i = 0; i < d->ncell; ++i) {
+            double *q = cellp(d, d->qf, ib, i);
+            double y0[64], yT[65];
+            memcpy(y0, q + 3, sizeof(double) * kk);
+            memcpy(yT, q + 3, sizeof(double) * kk);
+            fo_adjust_sum(kk, yT);
+            yT[kk] = q[2];
+            if (fo_chem_step(m, dt, q[0], yT, rtol, atol, NULL) != 0) ++fails;
+            fo_adjust_sum(kk, yT);
+            q[2] = yT[kk];
+            memcpy(q + 3, yT, sizeof(double) * kk);
+            const size_t c = (size_t)ib * d->ncell + i;
+            if (hrr) hrr[c] = fo_heat_release_rate(m, y0, q + 3, q[0], dt);
+            if (hrr_dot) hrr_dot[c] = fo_heat_release_rate_omegadot(m, q + 3, q[0], q[2]);
+        }
+    for (int ib = 0; ib < d->nblk; ++ib)
+        for (int i = 0; i < d->ncell; ++i) fo_qf_to_qc(m, cellp(d, d->qf, ib, i), cellp(d, d->Qc, ib, i));
+    if (nfail) *nfail = fails;
 }

@@ -24,6 +24,8 @@ fo_mech *fo_mech_load(const char *chem_bin, const char *tran_bin);
 void fo_mech_free(fo_mech *m);
 int fo_mech_kk(const fo_mech *m);
 int fo_mech_ii(const fo_mech *m);
+int fo_mech_mm(const fo_mech *m);
+void fo_mech_ncf(const fo_mech *m, int *ncf);               /* [KK*MM] element counts */
 /* Export of the parsed tables, in CKINIT's own units and orders, for pinning against CKINIT. */
 void fo_mech_weights(const fo_mech *m, double *wt);               /* [KK] g/mol          */
 void fo_mech_nasa(const fo_mech *m, double *tt, double *aa);      /* [KK*3], [KK*2*7]    */
@@ -116,6 +118,20 @@ void fo_ckwyr(const fo_mech *m, double rho, double T, const double *Y, double *w
 /* CKCDYR (cklib.f:1422-1541). */
 void fo_ckcdyr(const fo_mech *m, double rho, double T, const double *Y, double *cdot,
                double *ddot);
+
+/* ---- chemistry source (own stiff integrator; see fcfd_oracle.c) -------------------------------- */
+typedef struct { int steps, rejects, newton, jacobians, failed; } fo_chem_stats;
+/* f = d(Y_1..Y_KK, T)/dt at fixed density rho (g/cm^3). */
+void fo_chem_rhs(const fo_mech *m, double rho, const double *y, double *f);
+/* J = df/dy (n x n row-major, n = KK+1). */
+void fo_chem_jac(const fo_mech *m, double rho, const double *y, double *J);
+/* macks(dt, nspe+1, yiT, rho) (SRC/source.f90:58): yT = (Y, T) in/out, rho kg/m^3.
+ * Returns 0, or -1 when the step budget ran out (state = last accepted). */
+int fo_chem_step(const fo_mech *m, double dt, double rho, double *yT, double rtol, double atol,
+                 fo_chem_stats *st);
+/* reaction_macks (SRC/source.f90:17-84) over a domain's interior cells. */
+void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *hrr,
+                     double *hrr_dot, int *nfail);
 
 #ifdef __cplusplus
 }

@@ -235,6 +235,43 @@ class Oracle:
         return c, d
 
 
+class ChemStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("rejects", C.c_int), ("newton", C.c_int),
+                ("jacobians", C.c_int), ("failed", C.c_int)]
+
+
+def _chem_methods():
+    def ncf(self):
+        mm = self.lib.fo_mech_mm(self.m)
+        out = np.zeros(self.kk * mm, np.int32)
+        self.lib.fo_mech_ncf(self.m, out.ctypes.data_as(_I))
+        return out.reshape(self.kk, mm)
+
+    def chem_rhs(self, rho_cgs, yT):
+        f = np.zeros(self.kk + 1)
+        self.lib.fo_chem_rhs(self.m, C.c_double(rho_cgs), _p(_f64(yT)), _p(f))
+        return f
+
+    def chem_jac(self, rho_cgs, yT):
+        n = self.kk + 1
+        J = np.zeros(n * n)
+        self.lib.fo_chem_jac(self.m, C.c_double(rho_cgs), _p(_f64(yT)), _p(J))
+        return J.reshape(n, n)
+
+    def chem_step(self, dt, rho_si, yT, rtol=1e-5, atol=1e-13):
+        """macks(dt, nspe+1, yiT, rho): returns (yT_new, stats dict, rc)."""
+        y = _f64(yT).copy()
+        st = ChemStats()
+        rc = self.lib.fo_chem_step(self.m, C.c_double(dt), C.c_double(rho_si), _p(y),
+                                   C.c_double(rtol), C.c_double(atol), C.byref(st))
+        return y, {k: getattr(st, k) for k, _ in ChemStats._fields_}, rc
+
+    return ncf, chem_rhs, chem_jac, chem_step
+
+
+Oracle.ncf, Oracle.chem_rhs, Oracle.chem_jac, Oracle.chem_step = _chem_methods()
+
+
 class Domain:
     """Multi-block wall-bounded domain stepped by the oracle (reference storage layout)."""
 
@@ -272,3 +309,11 @@ class Domain:
 
     def refresh(self):
         self.orc.lib.fo_refresh(self.ptr)
+
+    def solve_source(self, dt, rtol=1e-5, atol=1e-13):
+        n = self.nblk * self.ncell
+        hrr, hrd = np.zeros(n), np.zeros(n)
+        nf = C.c_int()
+        self.orc.lib.fo_solve_source(self.ptr, C.c_double(dt), C.c_double(rtol), C.c_double(atol),
+                                     _p(hrr), _p(hrd), C.byref(nf))
+        return hrr, hrd, nf.value
