@@ -250,7 +250,31 @@ int fcfd_mech_load(const char *chem_bin, const char *tran_bin, fcfd_mech **out) 
     return FCFD_OK;
 }
 
-void fcfd_mech_free(fcfd_mech *m) { delete m; }
+void fcfd_mech_free(fcfd_mech *m) {
+    if (!m) return;
+    (void)hipFree(m->d_thermo);
+    (void)hipFree(m->d_kin);
+    (void)hipFree(m->d_counter);
+    delete m;
+}
+
+int fcfd_mech_species(const fcfd_mech *m, int k, char *name, int len, double *wt, int *ncf) {
+    if (!m || k < 0 || k >= m->host.kk) { set_error("fcfd_mech_species: invalid argument"); return FCFD_E_ARG; }
+    if (name && len > 0) {
+        std::strncpy(name, m->host.names[k].c_str(), size_t(len) - 1);
+        name[len - 1] = '\0';
+    }
+    if (wt) *wt = m->host.wt[k];
+    if (ncf)
+        for (int e = 0; e < m->host.mm; ++e) ncf[e] = m->host.ncf[size_t(k) * m->host.mm + e];
+    return FCFD_OK;
+}
+
+int fcfd_mech_elements(const fcfd_mech *m, int *mm) {
+    if (!m || !mm) { set_error("fcfd_mech_elements: invalid argument"); return FCFD_E_ARG; }
+    *mm = m->host.mm;
+    return FCFD_OK;
+}
 
 int fcfd_mech_dims(const fcfd_mech *m, int *kk, int *ii) {
     if (!m) { set_error("mech is NULL"); return FCFD_E_ARG; }

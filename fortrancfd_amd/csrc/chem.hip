@@ -83,9 +83,8 @@ __device__ __forceinline__ double smh_species(const Nasa &a, double T) {
 
 template <int NSP>
 struct Shared {
-    double J[(NSP + 1) * LDJ];
-    double c[64], smh[64], u[64], prow[64];
-    double rop[MAXR], rr[MAXR];
+    double J[(NSP + 1) * LDJ];   // Jacobian, overwritten by the LU factors of M
+    double c[64], smh[64], u[64], prow[64], wdot[64], cdot[64], ddot[64];
     int perm[64];
     int cell;
 };
@@ -169,15 +168,18 @@ struct RhsOut {
 };
 
 // f = dy/dt at (y_l, T); y_l is this lane's component (lane < NSP: Y_lane).  Leaves the
-// concentrations in sh.c, S/R-H/RT in sh.smh, u_k in sh.u, rates of progress in sh.rop (net) and
-// sh.rr (reverse) when keep_split.
+// concentrations in sh.c, S/R-H/RT in sh.smh, u_k in sh.u and the production rates in sh.wdot, or,
+// with split, CKCDYR's creation / destruction rates in sh.cdot / sh.ddot (cklib.f:1497-1517).
+// Rates of progress are scattered into the species sums with LDS atomics; one wave owns the
+// workgroup, so the order of those additions is fixed and the result reproducible.
 template <int NSP>
 __device__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP> &sh,
-                      double rho, double y_l, double T, bool keep_split) {
+                      double rho, double y_l, double T, bool split) {
     const int lane = lane_id();
-    double uk = 0.0, cpk = 0.0;
+    double uk = 0.0, cpk = 0.0, ck = 0.0;
     if (lane < NSP) {
-        sh.c[lane] = rho * y_l / th->wt[lane];
+        ck = rho * y_l / th->wt[lane];
+        sh.c[lane] = ck;
         const Nasa a = nasa(th, lane, T);
         const TPow p = tpow(T, th->ru);
         uk = h_mol(a, p) - p.rut;
@@ -185,21 +187,36 @@ __device__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict
         sh.u[lane] = uk;
         sh.smh[lane] = smh_species(a, T);
     }
+    sh.wdot[lane] = 0.0;
+    sh.cdot[lane] = 0.0;
+    sh.ddot[lane] = 0.0;
+    const double ctot = wsum(ck);
     __syncthreads();
-    double ctot = 0.0;
-    for (int k = 0; k < NSP; ++k) ctot = ctot + sh.c[k];
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
     for (int i = lane; i < K->ii; i += 64) {
         const Rop r = reaction(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
-        sh.rop[i] = keep_split ? r.qf : r.qf - r.qr;
-        if (keep_split) sh.rr[i] = r.qr;
+        const int *nu = K->nu[i], *nk = K->nk[i];
+        if (!split) {
+            const double q = r.qf - r.qr;
+            for (int n = 0; n < 6; ++n)
+                if (nk[n] >= 0) atomicAdd(&sh.wdot[nk[n]], q * (double)nu[n]);
+        } else {
+            for (int n = 0; n < 3; ++n) {
+                if (nk[n] >= 0) {
+                    const double a = (double)abs(nu[n]);
+                    atomicAdd(&sh.cdot[nk[n]], r.qr * a);
+                    atomicAdd(&sh.ddot[nk[n]], r.qf * a);
+                }
+                if (nk[n + 3] >= 0) {
+                    const double a = (double)nu[n + 3];
+                    atomicAdd(&sh.cdot[nk[n + 3]], r.qf * a);
+                    atomicAdd(&sh.ddot[nk[n + 3]], r.qr * a);
+                }
+            }
+        }
     }
     __syncthreads();
-    double wdot = 0.0;
-    if (lane < NSP && !keep_split) {
-        for (int e = K->csr_start[lane]; e < K->csr_start[lane + 1]; ++e)
-            wdot = wdot + sh.rop[K->csr_rxn[e]] * K->csr_nu[e];
-    }
+    const double wdot = lane < NSP ? sh.wdot[lane] : 0.0;
     const double cvk = cpk - th->ru;
     const double cvm = wsum(lane < NSP ? y_l * cvk / th->wt[lane] : 0.0);
     const double su = wsum(lane < NSP ? uk * wdot : 0.0);
@@ -219,9 +236,8 @@ __device__ void jacobian(const DevThermo *__restrict__ th, const DevKin *__restr
     constexpr int N = NSP + 1;
     const RhsOut f0 = rhs<NSP>(th, K, sh, rho, y_l, T, false);
     for (int e = lane; e < N * LDJ; e += 64) sh.J[e] = 0.0;
+    const double ctot = wsum(lane < NSP ? sh.c[lane] : 0.0);
     __syncthreads();
-    double ctot = 0.0;
-    for (int k = 0; k < NSP; ++k) ctot = ctot + sh.c[k];
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
     for (int i = lane; i < K->ii; i += 64) {
         const int *nu = K->nu[i], *nk = K->nk[i];
@@ -267,13 +283,15 @@ __device__ void jacobian(const DevThermo *__restrict__ th, const DevKin *__restr
     __syncthreads();
 }
 
-// LU with partial pivoting of M = I - hg*J, one row per lane in m[]; rank = elimination step of
-// this lane's row (N: never pivoted yet, -1: lane beyond the system).  Returns false if singular.
+// LU with partial pivoting of M = I - hg*J: rows are factored one per lane in VGPRs and the factors
+// written back over sh.J; rank = elimination step of this lane's row (N: not pivoted yet, -1: lane
+// beyond the system).  Returns false if a pivot is zero.
 template <int NSP>
-__device__ bool lu(Shared<NSP> &sh, double hg, double (&m)[NSP + 1], int &rank) {
+__device__ bool lu(Shared<NSP> &sh, double hg, int &rank) {
     constexpr int N = NSP + 1;
     const int lane = lane_id();
     rank = lane < N ? N : -1;
+    double m[N];
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         m[j] = 0.0;
@@ -303,26 +321,30 @@ __device__ bool lu(Shared<NSP> &sh, double hg, double (&m)[NSP + 1], int &rank) 
         }
         __syncthreads();
     }
+    if (lane < N) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) sh.J[lane * LDJ + j] = m[j];
+    }
+    __syncthreads();
     return ok;
 }
 
-// Solves M x = b with the factors above; b and x distributed one component per lane.
+// Solves M x = b with the factors in sh.J; b and x distributed one component per lane.
 template <int NSP>
-__device__ double solve(const Shared<NSP> &sh, const double (&m)[NSP + 1], int rank, double b) {
+__device__ double solve(const Shared<NSP> &sh, int rank, double b) {
     constexpr int N = NSP + 1;
     const int lane = lane_id();
-#pragma unroll
+    const double *row = sh.J + (lane < N ? lane : 0) * LDJ;
     for (int k = 0; k < N; ++k) {
         const double bp = bcast(b, sh.perm[k]);
-        if (rank > k) b = b - m[k] * bp;
+        if (rank > k) b = b - row[k] * bp;
     }
     double x = 0.0;
-#pragma unroll
     for (int k = N - 1; k >= 0; --k) {
         const int p = sh.perm[k];
-        const double xk = bcast(b / m[k], p);
+        const double xk = bcast(b / row[k], p);
         if (lane == k) x = xk;
-        if (rank >= 0 && rank < k) b = b - m[k] * xk;
+        if (rank >= 0 && rank < k) b = b - row[k] * xk;
     }
     return x;
 }
@@ -388,20 +410,36 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
         energy<NSP>(th, adjusted(y, lane, NSP), T0, e0, cv0);
         if (lane == NSP) y = T0;
 
-        // ---- SDIRK3 over [0, dt] ----
         double t = 0.0, h = dt;
         int steps = 0, rejects = 0, newton = 0;
-        double m[N];
         int rank = -1;
+        // ---- explicit pre-step: Heun over the whole dt, accepted when the embedded Euler error
+        //      estimate is within tolerance (quiescent cells never build a Jacobian) ----
+        {
+            const double w = lane < NSP ? 1.0 / (a.atol + a.rtol * fabs(y))
+                                        : (lane == NSP ? 1.0 / (a.rtol * fabs(y)) : 0.0);
+            const RhsOut f0 = rhs<NSP>(th, K, sh, rho, y, bcast(y, NSP), false);
+            const double y1 = lane < N ? y + dt * f0.f : 0.0;
+            const RhsOut f1 = rhs<NSP>(th, K, sh, rho, y1, bcast(y1, NSP), false);
+            const double err = wnorm(lane < N ? (dt * 0.5) * (f1.f - f0.f) : 0.0, w, N);
+            if (err <= 1.0) {
+                if (lane < N) y = y + (dt * 0.5) * (f0.f + f1.f);
+                t = dt;
+                steps = 1;
+            }
+        }
+        // ---- SDIRK3 over [0, dt] ----
         while (t < dt && steps + rejects < CH_MAXSTEPS) {
             if (h > dt - t) h = dt - t;
             const double Tn = bcast(y, NSP);
             const double w = lane < NSP ? 1.0 / (a.atol + a.rtol * fabs(y))
                                         : (lane == NSP ? 1.0 / (a.rtol * fabs(y)) : 0.0);
-            jacobian<NSP>(th, K, sh, rho, y, Tn);
+            bool have_j = false;
             for (;;) {
+                if (!have_j) jacobian<NSP>(th, K, sh, rho, y, Tn);   // lu() overwrites it
+                have_j = false;
                 const double hg = h * CH_G;
-                bool ok = lu<NSP>(sh, hg, m, rank);
+                bool ok = lu<NSP>(sh, hg, rank);
                 double k0 = 0.0, k1 = 0.0, k2 = 0.0, z = 0.0;
                 for (int st = 0; st < 3 && ok; ++st) {
                     double base = y;
@@ -414,7 +452,7 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
                         const double Tz = bcast(z, NSP);
                         const RhsOut f = rhs<NSP>(th, K, sh, rho, z, Tz, false);
                         double r = lane < N ? -(z - base - hg * f.f) : 0.0;
-                        r = solve<NSP>(sh, m, rank, r);
+                        r = solve<NSP>(sh, rank, r);
                         if (lane < N) z = z + r;
                         ++newton;
                         const double nrm = wnorm(lane < N ? r : 0.0, w, N);
@@ -433,7 +471,7 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
                     continue;
                 }
                 double ev = lane < N ? h * (CH_E1 * k0 + CH_E2 * k1 + CH_E3 * k2) : 0.0;
-                ev = solve<NSP>(sh, m, rank, ev);
+                ev = solve<NSP>(sh, rank, ev);
                 const double err = wnorm(lane < N ? ev : 0.0, w, N);
                 const double fac = 0.9 * pow(fmax(err, 1.0e-10), -1.0 / 3.0);
                 if (err <= 1.0) {
@@ -471,15 +509,8 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
                 const double s1 = wsum(term);
                 // heat_release_rate_omegadot via CKCDYR at the new state (SRC/thermo.f90:128-132)
                 rhs<NSP>(th, K, sh, rho, ya, T, true);
-                double cd = 0.0, dd = 0.0;
-                if (lane < NSP) {
-                    for (int e = K->csr_start[lane]; e < K->csr_start[lane + 1]; ++e) {
-                        const int i = K->csr_rxn[e];
-                        const double nuv = K->csr_nu[e];
-                        if (nuv < 0.0) { cd = cd + sh.rr[i] * -nuv; dd = dd + sh.rop[i] * -nuv; }
-                        else { cd = cd + sh.rop[i] * nuv; dd = dd + sh.rr[i] * nuv; }
-                    }
-                }
+                const double cd = lane < NSP ? sh.cdot[lane] : 0.0;
+                const double dd = lane < NSP ? sh.ddot[lane] : 0.0;
                 const double s2 = wsum(lane < NSP ? th->wt[lane] * th->h298[lane] * (cd - dd) : 0.0);
                 if (lane == 0) {
                     a.hrr[cell] = -s1 * 1.e-1;

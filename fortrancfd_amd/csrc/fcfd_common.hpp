@@ -36,6 +36,7 @@ struct Mech {
     int kk = 0, ii = 0, mm = 0, mxsp = 6, mxtb = 0, mxtp = 3, ncp = 5, npar = 3, nfar = 8;
     double ru = 8.314510e7, pa = 1.01325e6;
     std::vector<std::string> names;
+    std::vector<int> ncf;                       // [kk*mm] element counts
     std::vector<double> wt, tt, aa;             // [kk], [kk*mxtp], [kk*(mxtp-1)*(ncp+2)]
     std::vector<int> nt;
     std::vector<int> ns, nr, nu, nk;            // [ii], [ii], [ii*mxsp], [ii*mxsp]

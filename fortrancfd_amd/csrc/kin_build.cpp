@@ -1,11 +1,16 @@
 // kin_build.cpp -- device kinetics tables from the CHEMKIN-II link-file data (host side).
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "chem.hpp"
 
 namespace fcfd {
 
+// Device rows are the mechanism's reactions re-ordered by kind -- fall-off first, then plain
+// third-body, then the rest -- so a wave, which evaluates 64 consecutive rows at once, runs the
+// fall-off and third-body code paths in as few row chunks as possible.  Every per-reaction result
+// is independent of the row order; only the order of the species-sum additions changes.
 void build_dev_kin(const Mech &m, DevKin &k) {
     std::memset(&k, 0, sizeof(DevKin));
     k.ii = m.ii;
@@ -14,25 +19,32 @@ void build_dev_kin(const Mech &m, DevKin &k) {
     k.nfal = int(m.ifl.size());
     k.small = std::pow(10.0, -300);        // CKINIT SMALL / EXPARG (cklib.f:4054-4057)
     k.exparg = std::log(std::pow(10.0, 300));
+    std::vector<int> tb_of(m.ii, -1), fo_of(m.ii, -1), order;
+    for (int t = 0; t < k.nthb; ++t) tb_of[m.itb[t] - 1] = t;
+    for (int f = 0; f < k.nfal; ++f) fo_of[m.ifl[f] - 1] = f;
+    for (int pass = 0; pass < 3; ++pass)
+        for (int i = 0; i < m.ii; ++i) {
+            const int kind = fo_of[i] >= 0 ? 0 : (tb_of[i] >= 0 ? 1 : 2);
+            if (kind == pass) order.push_back(i);
+        }
     const int np1 = m.npar + 1;
-    for (int i = 0; i < m.ii; ++i) {
-        k.A[i] = m.par[size_t(i) * np1 + 0];
-        k.B[i] = m.par[size_t(i) * np1 + 1];
-        k.E[i] = m.par[size_t(i) * np1 + 2];
-        k.rev[i] = m.ns[i] > 0 ? 1 : 0;
+    for (int r = 0; r < m.ii; ++r) {
+        const int i = order[r];
+        k.A[r] = m.par[size_t(i) * np1 + 0];
+        k.B[r] = m.par[size_t(i) * np1 + 1];
+        k.E[r] = m.par[size_t(i) * np1 + 2];
+        k.rev[r] = m.ns[i] > 0 ? 1 : 0;
         int s = 0;
         for (int n = 0; n < 6; ++n) {
-            k.nu[i][n] = m.nu[size_t(i) * 6 + n];
-            k.nk[i][n] = m.nk[size_t(i) * 6 + n] - 1;
-            s += k.nu[i][n];
+            k.nu[r][n] = m.nu[size_t(i) * 6 + n];
+            k.nk[r][n] = m.nk[size_t(i) * 6 + n] - 1;
+            s += k.nu[r][n];
         }
-        k.nus[i] = s;
-        k.tb[i] = -1;
-        k.fo[i] = -1;
+        k.nus[r] = s;
+        k.tb[r] = tb_of[i];
+        k.fo[r] = fo_of[i];
     }
     for (int t = 0; t < k.nthb; ++t) {
-        const int i = m.itb[t] - 1;
-        k.tb[i] = t;
         k.ntbs[t] = m.ntbs[t];
         for (int l = 0; l < m.mxtb && l < MAXTB; ++l) {
             k.tbk[t][l] = m.nktb[size_t(t) * m.mxtb + l] - 1;
@@ -40,21 +52,21 @@ void build_dev_kin(const Mech &m, DevKin &k) {
         }
     }
     for (int f = 0; f < k.nfal; ++f) {
-        const int i = m.ifl[f] - 1;
-        k.fo[i] = f;
         k.ifop[f] = m.ifo[f];
         k.kfal[f] = m.kfl[f] - 1;
         for (int c = 0; c < 8 && c < m.nfar; ++c) k.fpar[f][c] = m.fpar[size_t(f) * m.nfar + c];
     }
-    // production-rate lists per species in CKWYR's order: slot-major, reaction-minor
+    // production-rate lists per species (CKWYR's slot-major, reaction-minor order)
     int e = 0;
     for (int sp = 0; sp < m.kk; ++sp) {
         k.csr_start[sp] = e;
         for (int n = 0; n < 6; ++n)
             for (int i = 0; i < m.ii; ++i)
-                if (k.nk[i][n] == sp) {
-                    k.csr_rxn[e] = i;
-                    k.csr_nu[e] = double(k.nu[i][n]);
+                if (m.nk[size_t(i) * 6 + n] - 1 == sp) {
+                    int r = 0;
+                    while (order[r] != i) ++r;
+                    k.csr_rxn[e] = r;
+                    k.csr_nu[e] = double(m.nu[size_t(i) * 6 + n]);
                     ++e;
                 }
     }

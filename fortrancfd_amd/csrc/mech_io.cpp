@@ -78,11 +78,11 @@ int load_mech(const char *chem_bin, const char *tran_bin, Mech &m, std::string &
     if (!r.next()) { err = "chem.bin: missing element record"; return -2; }
     if (!r.next()) { err = "chem.bin: missing species record"; return -2; }
     const int kk = m.kk, ncp2 = m.ncp + 2;
-    m.names.resize(kk); m.wt.resize(kk); m.nt.resize(kk);
+    m.names.resize(kk); m.wt.resize(kk); m.nt.resize(kk); m.ncf.resize(size_t(kk) * m.mm);
     m.tt.resize(size_t(kk) * m.mxtp); m.aa.resize(size_t(kk) * (m.mxtp - 1) * ncp2);
     for (int k = 0; k < kk; ++k) {
         m.names[k] = r.chars(16);
-        for (int e = 0; e < m.mm; ++e) r.i32();    // element composition
+        for (int e = 0; e < m.mm; ++e) m.ncf[size_t(k) * m.mm + e] = r.i32();
         r.i32(); r.i32();                           // phase, charge
         m.wt[k] = r.f64();
         m.nt[k] = r.i32();

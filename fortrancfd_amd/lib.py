@@ -74,6 +74,20 @@ class Mechanism:
         kk, ii = C.c_int(), C.c_int()
         _check(L.fcfd_mech_dims(self.handle, C.byref(kk), C.byref(ii)))
         self.kk, self.ii, self.nequ = kk.value, ii.value, 3 + kk.value
+        mm = C.c_int()
+        _check(L.fcfd_mech_elements(self.handle, C.byref(mm)))
+        self.mm = mm.value
+        names, wts, ncf = [], np.zeros(self.kk), np.zeros((self.kk, self.mm), np.int32)
+        buf = C.create_string_buffer(32)
+        for k in range(self.kk):
+            w = C.c_double()
+            row = np.zeros(self.mm, np.int32)
+            _check(L.fcfd_mech_species(self.handle, C.c_int(k), buf, C.c_int(32), C.byref(w),
+                                       row.ctypes.data_as(C.POINTER(C.c_int))))
+            names.append(buf.value.decode())
+            wts[k] = w.value
+            ncf[k] = row
+        self.names, self.weights, self.ncf = names, wts, ncf
 
     def __del__(self):
         try:

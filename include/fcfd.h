@@ -55,6 +55,12 @@ typedef struct fcfd_opts {
 int fcfd_mech_load(const char *chem_bin, const char *tran_bin, fcfd_mech **out);
 void fcfd_mech_free(fcfd_mech *m);
 int fcfd_mech_dims(const fcfd_mech *m, int *kk, int *ii);
+/* Species k (0-based): symbol (NUL-terminated, at most len-1 chars), molar mass g/mol and element
+ * counts (ncf[MM]), as CKSYMS / CKWT / CKNCF return them (cklib.f CKINIT species record). Each
+ * output may be NULL. */
+int fcfd_mech_species(const fcfd_mech *m, int k, char *name, int len, double *wt, int *ncf);
+/* Number of elements MM. */
+int fcfd_mech_elements(const fcfd_mech *m, int *mm);
 
 /* ---- domain context -----------------------------------------------------------------------
  * nblk equal blocks of ncell interior cells and spacing dx, all on the current HIP device

@@ -1097,6 +1097,8 @@ void fo_ckcdyr(const fo_mech *m, double rho, double T, const double *Y, double *
 /*   analytic mass-action Jacobian                                                               */
 /*   with frozen third-body / fall-off factors and a finite-difference temperature column;       */
 /*   T finally corrected so that sum Y_k u_k(T)/W_k keeps its initial value (energy exact).      */
+/*   Before any implicit step an explicit Heun step over dt is tried and accepted when its        */
+/*   embedded Euler error estimate is within tolerance (quiescent cells skip the Jacobian).       */
 /* ============================================================================================ */
 #define CH_GAMMA 0.435866521508459
 #define CH_A21 0.2820667392457705
@@ -1247,6 +1249,20 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
     fo_adjust_sum(kk, ya);
     ch_energy(m, ya, y[kk], &e0, &cv0);
     double t = 0.0, h = dt;
+    {   /* explicit pre-step: Heun over dt, accepted when the embedded Euler error is in tolerance */
+        double f0[65], f1[65], y1[65];
+        for (int i = 0; i < kk; ++i) w[i] = 1.0 / (atol + rtol * fabs(y[i]));
+        w[kk] = 1.0 / (rtol * fabs(y[kk]));
+        fo_chem_rhs(m, rho, y, f0);
+        for (int i = 0; i < n; ++i) y1[i] = y[i] + dt * f0[i];
+        fo_chem_rhs(m, rho, y1, f1);
+        for (int i = 0; i < n; ++i) e[i] = (dt * 0.5) * (f1[i] - f0[i]);
+        if (ch_norm(e, w, n) <= 1.0) {
+            for (int i = 0; i < n; ++i) y[i] = y[i] + (dt * 0.5) * (f0[i] + f1[i]);
+            t = dt;
+            s.steps = 1;
+        }
+    }
     while (t < dt && s.steps + s.rejects < CH_MAXSTEPS) {
         if (h > dt - t) h = dt - t;
         for (int i = 0; i < kk; ++i) w[i] = 1.0 / (atol + rtol * fabs(y[i]));
