@@ -9,6 +9,7 @@
 #include "fcfd_common.hpp"
 #include "fluid.hpp"
 #include "chem.hpp"
+#include "surrogate_impl.hpp"
 
 namespace fcfd {
 
@@ -30,7 +31,7 @@ using namespace fcfd;
 
 namespace {
 
-constexpr int NTIMERS = 5;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry, 4 source Qc refresh
+constexpr int NTIMERS = 6;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry, 4 source Qc refresh, 5 surrogate network
 
 int lvir_of(const fcfd_opts &o) {                                  // SRC/params.f90:161-173
     if (o.flag_acu == 0) return 1;
@@ -81,15 +82,17 @@ struct fcfd_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_pending;   // (timer id, event index of the start event)
     size_t ev_used = 0;
-    double ms[NTIMERS] = {0, 0, 0, 0, 0};
-    long long cnt[NTIMERS] = {0, 0, 0, 0, 0};
+    double ms[NTIMERS] = {};
+    long long cnt[NTIMERS] = {};
     // chemistry
     unsigned int *d_counter = nullptr;
     double *d_hrr = nullptr, *d_hrr_dot = nullptr;
     ChemCellStats *d_stats = nullptr;
+    fcfd_surrogate *sur = nullptr;
+    int *d_accepted = nullptr;
 };
 
-namespace {
+namespace fcfd {
 
 // Device copies of a mechanism's tables on the current device (created once).
 int ensure_dev_tables(const fcfd_mech *cm) {
@@ -113,7 +116,7 @@ int ensure_dev_tables(const fcfd_mech *cm) {
 // Persistent grid size: four one-wave workgroups per CU (LDS-limited).
 int chem_workgroups(const fcfd_mech *m) { return 4 * (m->ncu > 0 ? m->ncu : 256); }
 
-}  // namespace
+}  // namespace fcfd
 
 namespace {
 
@@ -213,14 +216,34 @@ hipError_t enqueue_refresh(fcfd_ctx *c) {
 }
 
 // reaction_macks (SRC/source.f90:17-84): chemistry on every interior cell, then Qc = qfToQc(qf).
-hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev) {
+// The heat-release diagnostics cost one more rate evaluation per cell and are only computed when
+// the caller asks for them.  With a surrogate attached, its accepted cells skip the integrator.
+hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool want_hrr) {
     ChemArgs a{};
     a.thermo = c->thermo; a.kin = static_cast<const DevKin *>(c->mech->d_kin); a.counter = c->d_counter;
     a.ncells = c->nblk * c->ncell; a.rtol = c->o.rtol; a.atol = c->o.atol;
     a.qf = c->qf; a.P = c->P; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S;
-    a.dt = dt; a.dt_dev = dt_dev; a.hrr = c->d_hrr; a.hrr_dot = c->d_hrr_dot; a.stats = c->d_stats;
+    a.dt = dt; a.dt_dev = dt_dev; a.stats = c->d_stats;
+    if (want_hrr) { a.hrr = c->d_hrr; a.hrr_dot = c->d_hrr_dot; }
     a.layout = 0;
-    hipError_t e = timed(c, 3, [&]() { return launch_chem(a, c->kk, chem_workgroups(c->mech), c->stream); });
+    hipError_t e;
+    if (c->sur) {
+        fcfd_surrogate *s = c->sur;
+        const int n = c->nblk * c->ncell, npad = int(round_up(n, 64));
+        e = timed(c, 5, [&]() {
+            hipError_t e2 = mlp_features_domain(s->d_norm, c->thermo, c->kk, c->nblk, c->ncell, c->lv, c->S, c->P,
+                                                c->qf, dt_dev, dt, npad, s->X, s->in_range, c->stream);
+            if (e2 != hipSuccess) return e2;
+            return s->forward(s->X, npad, s->Yh, false, c->stream);
+        });
+        if (e != hipSuccess) return e;
+        ApplyArgs ap{};
+        ap.n = n; ap.layout = 0; ap.qf = c->qf; ap.P = c->P; ap.ncell = c->ncell; ap.lv = c->lv; ap.S = c->S;
+        ap.accepted = c->d_accepted;
+        e = timed(c, 3, [&]() { return s->apply_and_fallback(ap, a, chem_workgroups(c->mech), c->stream); });
+    } else {
+        e = timed(c, 3, [&]() { return launch_chem(a, c->kk, chem_workgroups(c->mech), c->stream); });
+    }
     if (e != hipSuccess) return e;
     return timed(c, 4, [&]() {
         return launch_qf_to_qc_interior(c->thermo, c->kk, c->nblk, c->ncell, c->lv, c->S, c->qf,
@@ -332,6 +355,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->qf); (void)hipFree(c->F); (void)hipFree(c->god); (void)hipFree(c->cells);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
+    (void)hipFree(c->d_accepted);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -435,7 +459,7 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
         for (int s = 0; s < nsteps; ++s) {
             HIP_TRY(enqueue_dt(c, cfl, c->d_time));
             if (c->o.do_hydro == 1) HIP_TRY(enqueue_fluid(c, 0.0, c->d_dt));
-            if (refresh) HIP_TRY(enqueue_source(c, 0.0, c->d_dt));
+            if (refresh) HIP_TRY(enqueue_source(c, 0.0, c->d_dt, false));
             if (refresh) HIP_TRY(enqueue_refresh(c));
         }
         HIP_TRY(collect_timing(c));
@@ -448,7 +472,7 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
                 HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 hipError_t e = enqueue_dt(c, cfl, c->d_time);
                 if (e == hipSuccess && c->o.do_hydro == 1) e = enqueue_fluid(c, 0.0, c->d_dt);
-                if (e == hipSuccess && refresh) e = enqueue_source(c, 0.0, c->d_dt);
+                if (e == hipSuccess && refresh) e = enqueue_source(c, 0.0, c->d_dt, false);
                 if (e == hipSuccess && refresh) e = enqueue_refresh(c);
                 hipError_t e2 = hipStreamEndCapture(c->stream, &g);
                 HIP_TRY(e);
@@ -470,7 +494,8 @@ int fcfd_solve_source(fcfd_ctx *c, double dt, double time, double *hrr, double *
     (void)time;
     if (!c) { set_error("fcfd_solve_source: invalid argument"); return FCFD_E_ARG; }
     if (c->o.do_reaction != 1) return FCFD_OK;
-    HIP_TRY(enqueue_source(c, dt, nullptr));
+    if (c->sur && (hrr || hrr_dot)) { set_error("fcfd_solve_source: hrr / hrr_dot are not available in surrogate mode"); return FCFD_E_UNSUPPORTED; }
+    HIP_TRY(enqueue_source(c, dt, nullptr, hrr || hrr_dot));
     const size_t nint = size_t(c->nblk) * c->ncell;
     if (hrr) HIP_TRY(hipMemcpyAsync(hrr, c->d_hrr, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
     if (hrr_dot) HIP_TRY(hipMemcpyAsync(hrr_dot, c->d_hrr_dot, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
@@ -502,6 +527,38 @@ int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double 
     a.layout = 1;
     HIP_TRY(launch_chem(a, m->host.kk, chem_workgroups(m), static_cast<hipStream_t>(stream)));
     HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return FCFD_OK;
+}
+
+int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s) {
+    if (!c) { set_error("fcfd_set_surrogate: invalid argument"); return FCFD_E_ARG; }
+    if (s && c->o.do_reaction != 1) { set_error("fcfd_set_surrogate: the context has no chemistry"); return FCFD_E_ARG; }
+    if (s && s->mech != c->mech) { set_error("fcfd_set_surrogate: surrogate and context use different mechanisms"); return FCFD_E_ARG; }
+    if (s) {
+        const int n = c->nblk * c->ncell;
+        HIP_TRY(s->reserve(int(round_up(n, 64)), false));
+        if (!c->d_accepted) {
+            HIP_TRY(hipMalloc(&c->d_accepted, sizeof(int) * size_t(n)));
+            HIP_TRY(hipMemset(c->d_accepted, 0, sizeof(int) * size_t(n)));
+        }
+    }
+    drop_graphs(c);
+    c->sur = s;
+    return FCFD_OK;
+}
+
+int fcfd_source_accepted(fcfd_ctx *c, int *per_cell, int *n_fallback) {
+    if (!c || !c->d_accepted) { set_error("fcfd_source_accepted: no surrogate solve_source on this context"); return FCFD_E_ARG; }
+    const size_t n = size_t(c->nblk) * c->ncell;
+    std::vector<int> h(n);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->d_accepted, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (per_cell) std::memcpy(per_cell, h.data(), sizeof(int) * n);
+    if (n_fallback) {
+        long nf = 0;
+        for (int v : h) nf += v == 0;
+        *n_fallback = int(nf);
+    }
     return FCFD_OK;
 }
 

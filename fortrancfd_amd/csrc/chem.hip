@@ -384,9 +384,11 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
     for (;;) {
         if (lane == 0) sh.cell = int(atomicAdd(a.counter, 1u));
         __syncthreads();
-        const int cell = sh.cell;
+        const int item = sh.cell;
         __syncthreads();
-        if (cell >= a.ncells) break;
+        const int nitems = a.list_n ? int(*a.list_n) : a.ncells;
+        if (item >= nitems) break;
+        const int cell = a.list ? a.list[item] : item;
 
         // ---- load (reaction_macks, SRC/source.f90:44-56) ----
         long col = 0;

@@ -49,6 +49,9 @@ struct ChemArgs {
     const double *b_rho, *b_dt;
     double *b_T, *b_y;
     ChemCellStats *stats;        // may be null
+    // optional work list: cells list[0 .. *list_n) instead of 0 .. ncells (both device memory)
+    const int *list;
+    const unsigned *list_n;
     int layout;                  // 0: domain, 1: batch
 };
 

@@ -1,0 +1,57 @@
+// surrogate.hpp -- device-side records of the chemistry surrogate and the launchers of mlp.hip.
+#pragma once
+
+#include <cstdint>
+
+#include "fcfd_common.hpp"
+#include "mlp.hpp"
+
+namespace fcfd {
+
+// Feature / target scaling and the training range (raw feature units), columns as the network's.
+struct SurrogateNorm {
+    double in_mean[MLP_INP], in_std[MLP_INP];
+    double lo[MLP_INP], hi[MLP_INP];
+    double out_mean[MLP_OUTP], out_std[MLP_OUTP];
+};
+
+// Inference epilogue (k_apply): network output -> accepted state or a fallback-list entry.
+struct ApplyArgs {
+    const DevThermo *thermo;
+    const SurrogateNorm *norm;
+    const double *ncf;          // [KK][MM] element counts
+    int mm, n, layout;          // layout 0: padded SoA domain (qf), 1: batch arrays
+    double *qf;
+    long P;
+    int ncell, lv, S;
+    const double *rho;          // layout 1
+    double *T, *Y;              // layout 1: T[n], Y[KK][n]
+    const float *Yh;            // [n][MLP_OUTP] normalised network output
+    const int *in_range;        // [n]
+    int *accepted;              // [n] or null
+    unsigned *nfallback;        // fallback-list length (zeroed before the launch)
+    int *fallback;              // fallback list (cell indices)
+    double resid_tol, t_tol;
+};
+
+hipError_t mlp_features_batch(const SurrogateNorm *nm, const DevThermo *th, int kk, int n, int npad, const double *rho,
+                              const double *dt, const double *T, const double *Y, uint16_t *X, int *in_range,
+                              hipStream_t st);
+hipError_t mlp_features_domain(const SurrogateNorm *nm, const DevThermo *th, int kk, int nblk, int ncell, int lv, int S,
+                               long P, const double *qf, const double *dt_dev, double dt, int npad, uint16_t *X,
+                               int *in_range, hipStream_t st);
+hipError_t mlp_targets(const SurrogateNorm *nm, int kk, int n, int npad, const double *T0, const double *Y0,
+                       const double *T1, const double *Y1, float *Tg, hipStream_t st);
+hipError_t mlp_gather_bf16(const uint16_t *src, uint16_t *dst, const int *idx, int rows, int w, hipStream_t st);
+hipError_t mlp_gather_f32(const float *src, float *dst, const int *idx, int rows, int w, hipStream_t st);
+hipError_t mlp_mse(const float *Yh, const float *Yt, uint16_t *dY, int rows, float *partial, float *loss,
+                   hipStream_t st);
+hipError_t mlp_reduce_splits(const float *parts, float *out, int nsplit, long len, hipStream_t st);
+hipError_t mlp_colsum(const uint16_t *D, float *partial, float *out, int rows, int cols, hipStream_t st);
+hipError_t mlp_adam(float *w, const float *g, float *m, float *v, uint16_t *w16, long n, float lr, float b1, float b2,
+                    float eps, float c1, float c2, hipStream_t st);
+hipError_t mlp_to_bf16(const float *w, uint16_t *w16, long n, hipStream_t st);
+hipError_t mlp_zero_u32(unsigned *p, hipStream_t st);
+hipError_t mlp_apply(const ApplyArgs &a, int nsp, hipStream_t st);
+
+}  // namespace fcfd

@@ -203,13 +203,31 @@ class Solver:
     def refresh(self):
         _check(lib().fcfd_refresh(self.handle))
 
-    def solve_source(self, dt, time=0.0):
-        """reaction_macks on every interior cell; returns (HRR, HRR_dot) per interior cell."""
+    def solve_source(self, dt, time=0.0, diagnostics=True):
+        """reaction_macks on every interior cell; returns (HRR, HRR_dot) per interior cell when
+        diagnostics is set (not available with a surrogate attached), else None."""
         n = self.nblk * self.ncell
+        if not diagnostics:
+            _check(lib().fcfd_solve_source(self.handle, C.c_double(dt), C.c_double(time), None,
+                                           None))
+            return None
         hrr, hrd = np.zeros(n), np.zeros(n)
         _check(lib().fcfd_solve_source(self.handle, C.c_double(dt), C.c_double(time), _p(hrr),
                                        _p(hrd)))
         return hrr, hrd
+
+    def set_surrogate(self, sur):
+        """Route solve_source through a Surrogate (None: the FP64 integrator everywhere)."""
+        _check(lib().fcfd_set_surrogate(self.handle, sur.handle if sur is not None else None))
+        self._sur = sur
+
+    def source_accepted(self):
+        """(per-cell accepted flags, number of FP64 fallback cells) of the last surrogate call."""
+        acc = np.zeros(self.nblk * self.ncell, np.int32)
+        nf = C.c_int()
+        _check(lib().fcfd_source_accepted(self.handle, acc.ctypes.data_as(C.POINTER(C.c_int)),
+                                          C.byref(nf)))
+        return acc, nf.value
 
     def chem_stats(self):
         st = np.zeros((self.nblk * self.ncell, 4), np.int32)
@@ -228,3 +246,94 @@ class Solver:
         ms, n = C.c_double(), C.c_longlong()
         _check(lib().fcfd_get_timing(self.handle, C.c_int(kernel_id), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+def _vp(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def bf16_gemm(A, sam, sak, B, sbk, sbn, M, N, K, C32, ldc, ksplit=1, stream=None):
+    """C32 = A B with the surrogate's BF16 MFMA kernel; A, B, C32 are device tensors (A, B
+    bfloat16 or int16 bit patterns) addressed through the given element strides."""
+    L = lib()
+    L.fcfd_bf16_gemm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_long, C.c_long,
+                                 C.c_void_p, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_int,
+                                 C.c_void_p]
+    _check(L.fcfd_bf16_gemm(M, N, K, _vp(A), sam, sak, _vp(B), sbk, sbn, _vp(C32), ldc, ksplit,
+                            stream))
+
+
+class Surrogate:
+    """The chemistry surrogate (56 -> 4 x 256 GELU -> 54, BF16 MFMA, FP32 accumulate) with its
+    training step and its surrogate + FP64-fallback inference.  Array arguments are device tensors
+    (torch) unless stated otherwise."""
+
+    NORM_FIELDS = ("in_mean", "in_std", "lo", "hi", "out_mean", "out_std")
+
+    def __init__(self, mech: Mechanism):
+        self.mech = mech
+        self.handle = C.c_void_p()
+        _check(lib().fcfd_sur_create(mech.handle, C.byref(self.handle)))
+        self.nparams = lib().fcfd_sur_nparams()
+
+    def __del__(self):
+        try:
+            lib().fcfd_sur_free(self.handle)
+        except Exception:
+            pass
+
+    # parameters (host numpy float32, flat, in the documented order)
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        assert flat.size == self.nparams
+        _check(lib().fcfd_sur_set_params(self.handle, flat.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def get_params(self):
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().fcfd_sur_get_params(self.handle, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def get_grads(self):
+        out = np.zeros(self.nparams, np.float32)
+        _check(lib().fcfd_sur_get_grads(self.handle, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_norm(self, norm: dict):
+        arr = np.ascontiguousarray(np.stack([np.asarray(norm[f], np.float64)
+                                             for f in self.NORM_FIELDS]))
+        assert arr.shape == (6, 64)
+        _check(lib().fcfd_sur_set_norm(self.handle, _p(arr)))
+
+    def set_thresholds(self, resid_tol=1e-5, t_tol=2e-3):
+        _check(lib().fcfd_sur_set_thresholds(self.handle, C.c_double(resid_tol), C.c_double(t_tol)))
+
+    def features(self, rho, dt, T, Y, X, in_range=None, stream=None):
+        n = int(T.numel())
+        _check(lib().fcfd_sur_features(self.handle, C.c_int(n), _vp(rho), _vp(dt), _vp(T), _vp(Y),
+                                       _vp(X), _vp(in_range), stream))
+
+    def targets(self, T0, Y0, T1, Y1, Tg, stream=None):
+        n = int(T0.numel())
+        _check(lib().fcfd_sur_targets(self.handle, C.c_int(n), _vp(T0), _vp(Y0), _vp(T1), _vp(Y1),
+                                      _vp(Tg), stream))
+
+    def forward(self, X, Yh, stream=None):
+        _check(lib().fcfd_sur_forward(self.handle, C.c_int(int(X.shape[0])), _vp(X), _vp(Yh),
+                                      stream))
+
+    def train_step(self, X_all, Tg_all, idx, lr, want_loss=False, stream=None):
+        loss = C.c_float()
+        _check(lib().fcfd_sur_train_step(self.handle, _vp(X_all), _vp(Tg_all), _vp(idx),
+                                         C.c_int(int(idx.numel())), C.c_float(lr),
+                                         C.byref(loss) if want_loss else None, stream))
+        return loss.value if want_loss else None
+
+    def infer_batch(self, rho, dt, T, Y, accepted=None, stats=None, rtol=1e-5, atol=1e-13,
+                    stream=None):
+        """Surrogate + FP64 fallback in place on (T, Y); returns the number of fallback cells."""
+        o = opts(atol=atol, rtol=rtol, do_reaction=1)
+        nf = C.c_int()
+        _check(lib().fcfd_sur_infer_batch(self.handle, C.byref(o), C.c_int(int(T.numel())),
+                                          _vp(rho), _vp(dt), _vp(T), _vp(Y), _vp(accepted),
+                                          C.byref(nf), _vp(stats), stream))
+        return nf.value

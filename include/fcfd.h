@@ -138,6 +138,63 @@ int fcfd_chem_step_host(const fcfd_mech *m, const fcfd_opts *o, int n, const dou
 int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double *rho,
                     const double *dt, double *T, double *Y, int *stats, void *stream);
 
+/* ---- chemistry surrogate (no reference counterpart: stands in for the macks call of
+ * SRC/source.f90:58 where its answer is accepted) ----------------------------------------------
+ * A 56 -> 4 x 256 GELU -> 54 multilayer perceptron with BF16 weights and activations and FP32
+ * accumulation.  Inputs per cell: log max(Y_k, 1e-30) (KK = 53), T, log(p / 1 atm), log dt,
+ * z-scored; outputs (Y1 - Y0, T1 - T0) z-scored.  A cell's answer is accepted when every raw
+ * input lies in the training range, the element-conservation residual of the final state (after
+ * projection onto element conservation and clipping of negative mass fractions) is at most
+ * resid_tol relative, and the network temperature agrees with the energy-consistent temperature
+ * to t_tol relative; every other cell is integrated by the FP64 integrator.  The final T of an
+ * accepted cell comes from conservation of the specific internal energy, as in the FP64 path.
+ *
+ * Parameters (fcfd_sur_nparams() floats) are ordered W0 (64 x 256, row-major, k-major), b0 (256),
+ * W1 (256 x 256), b1, W2, b2, W3, b3, W4 (256 x 64), b4 (64); input rows 56..63 of W0 and output
+ * columns 54..63 of W4 / b4 are padding.  The norm record is six arrays of 64 doubles: input mean,
+ * input scale, training-range low, training-range high (raw units), output mean, output scale.
+ * Device pointers are arrays on the mechanism's device; X is BF16 [rows][64], Tg and Yh FP32
+ * [rows][64]; stream is a hipStream_t or NULL.  Calls on one surrogate must not overlap. */
+typedef struct fcfd_surrogate fcfd_surrogate;
+int fcfd_sur_create(const fcfd_mech *m, fcfd_surrogate **out);
+void fcfd_sur_free(fcfd_surrogate *s);
+int fcfd_sur_nparams(void);
+int fcfd_sur_set_params(fcfd_surrogate *s, const float *params);   /* host; resets Adam state */
+int fcfd_sur_get_params(fcfd_surrogate *s, float *params);
+int fcfd_sur_get_grads(fcfd_surrogate *s, float *grads);           /* of the last train step */
+int fcfd_sur_set_norm(fcfd_surrogate *s, const double *norm);
+int fcfd_sur_set_thresholds(fcfd_surrogate *s, double resid_tol, double t_tol);
+/* Features of n cells (device rho[n] kg/m^3, dt[n], T[n], Y[KK][n]) into X (rows padded to a
+ * multiple of 64 with zeros); in_range[n] (may be NULL) = 1 where every input is in range. */
+int fcfd_sur_features(fcfd_surrogate *s, int n, const double *rho, const double *dt,
+                      const double *T, const double *Y, void *X, int *in_range, void *stream);
+/* Normalised targets of n (state, result) pairs into Tg (rows padded like X). */
+int fcfd_sur_targets(fcfd_surrogate *s, int n, const double *T0, const double *Y0,
+                     const double *T1, const double *Y1, float *Tg, void *stream);
+/* Network output for rows (multiple of 64) rows of X. */
+int fcfd_sur_forward(fcfd_surrogate *s, int rows, const void *X, float *Yh, void *stream);
+/* One Adam step (b1 0.9, b2 0.999, eps 1e-8) on the mean squared error over the first 54 output
+ * columns of rows idx[0..batch) of (X_all, Tg_all); lr = 0 computes the gradients only.  loss
+ * (may be NULL; then the call does not wait for the device) receives the batch's MSE. */
+int fcfd_sur_train_step(fcfd_surrogate *s, const void *X_all, const float *Tg_all, const int *idx,
+                        int batch, float lr, float *loss, void *stream);
+/* Surrogate + FP64 fallback on n independent cells, arrays as fcfd_chem_batch; accepted[n] and
+ * stats[4n] may be NULL; *n_fallback (may be NULL) receives the number of FP64 cells. */
+int fcfd_sur_infer_batch(fcfd_surrogate *s, const fcfd_opts *o, int n, const double *rho,
+                         const double *dt, double *T, double *Y, int *accepted, int *n_fallback,
+                         int *stats, void *stream);
+/* solve_source of a context through the surrogate (NULL: the FP64 integrator on every cell).
+ * hrr / hrr_dot of fcfd_solve_source are not available in surrogate mode. */
+int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s);
+/* Per interior cell of the last surrogate solve_source: 1 accepted, 0 FP64 fallback. */
+int fcfd_source_accepted(fcfd_ctx *c, int *per_cell, int *n_fallback);
+
+/* Plain BF16 GEMM C (FP32) = A B on device arrays with element (m, k) of A at A[m*sam + k*sak]
+ * and (k, n) of B at B[k*sbk + n*sbn]; M, N multiples of 64, K / ksplit a multiple of 16;
+ * split s of K writes C + s*M*ldc.  The kernel the surrogate's layers run on. */
+int fcfd_bf16_gemm(int M, int N, int K, const void *A, long sam, long sak, const void *B,
+                   long sbk, long sbn, float *C, long ldc, int ksplit, void *stream);
+
 const char *fcfd_last_error(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,211 @@
+"""The surrogate's BF16-MFMA kernels against the CPU FP64 restatement (oracle/mlp_oracle.py).
+
+- GEMM: every operand orientation the network uses (x W, dZ W^T, H^T dZ with split K) against the
+  FP64 product of the same BF16 values; bound 4e-5 of sum |a||b| per entry (FP32 accumulation).
+- Forward and gradients: against the restatement with BF16 activations; relative L2 bounds 1e-2
+  (forward) and 3e-2 per gradient block -- BF16 storage of activations means entries whose FP32 and
+  FP64 values straddle a rounding boundary differ by one BF16 ulp (2^-8 relative).
+- Training: a few hundred Adam steps on a fixed regression problem reduce the loss fivefold.
+- Inference: with no cell in the training range every cell takes the FP64 path and the result is
+  bit-identical to the plain integrator; accepted cells satisfy element conservation within the
+  residual bound, sum Y = 1 and energy conservation.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available()
+    return t
+
+
+@pytest.fixture(scope="module")
+def mech():
+    from fortrancfd_amd.lib import Mechanism
+    return Mechanism()
+
+
+def dev_bf16(torch, a):
+    return torch.from_numpy(M.bf16_bits(a).view(np.int16)).cuda()
+
+
+@pytest.mark.parametrize("case", ["xW", "dzWT", "HTdz_split"])
+def test_gemm_orientations(torch, case):
+    from fortrancfd_amd.lib import bf16_gemm
+    rng = np.random.default_rng(5)
+    if case == "xW":
+        M_, N_, K_ = 256, 192, 64
+        A = rng.normal(size=(M_, K_)); B = rng.normal(size=(K_, N_))
+        Ad, Bd = dev_bf16(torch, A), dev_bf16(torch, B)
+        sam, sak, sbk, sbn, ks = K_, 1, N_, 1, 1
+    elif case == "dzWT":                       # B = W^T with W stored (N x K) row-major
+        M_, N_, K_ = 128, 256, 64
+        A = rng.normal(size=(M_, K_)); W = rng.normal(size=(N_, K_)); B = W.T
+        Ad, Bd = dev_bf16(torch, A), dev_bf16(torch, W)
+        sam, sak, sbk, sbn, ks = K_, 1, 1, K_, 1
+    else:                                      # A = H^T with H stored (K x M) row-major
+        M_, N_, K_ = 64, 256, 4096
+        H = rng.normal(size=(K_, M_)); A = H.T; B = rng.normal(size=(K_, N_))
+        Ad, Bd = dev_bf16(torch, H), dev_bf16(torch, B)
+        sam, sak, sbk, sbn, ks = 1, M_, N_, 1, 16
+    print(f"gemm {case} M={M_} N={N_} K={K_} ksplit={ks}", flush=True)
+    C = torch.zeros(ks * M_ * N_, dtype=torch.float32, device="cuda")
+    bf16_gemm(Ad, sam, sak, Bd, sbk, sbn, M_, N_, K_, C, N_, ks)
+    got = C.cpu().numpy().astype(np.float64).reshape(ks, M_, N_).sum(axis=0)
+    Ab, Bb = M.bf16(A), M.bf16(B)
+    ref = Ab @ Bb
+    bound = np.abs(Ab) @ np.abs(Bb)
+    assert np.all(np.abs(got - ref) <= 4e-5 * bound + 1e-30)
+
+
+def network(seed=3):
+    flat = M.init_params(seed).astype(np.float64)
+    lay = M.unpack(flat)
+    rng = np.random.default_rng(seed)
+    for l, (W, b) in enumerate(lay):
+        b[:] = rng.normal(0, 0.05, b.size)
+        if l == 4:
+            b[M.OUT:] = 0
+    return M.pack(lay).astype(np.float32).astype(np.float64)
+
+
+def inputs(rows, seed=4):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((rows, M.INP))
+    X[:, :M.IN] = rng.normal(size=(rows, M.IN))
+    return M.bf16(X)
+
+
+def test_forward_vs_oracle(torch, mech):
+    from fortrancfd_amd.lib import Surrogate
+    s = Surrogate(mech)
+    flat = network()
+    s.set_params(flat)
+    X = inputs(512)
+    Y = torch.zeros((512, 64), dtype=torch.float32, device="cuda")
+    print("forward 512 rows", flush=True)
+    s.forward(dev_bf16(torch, X), Y)
+    got = Y.cpu().numpy().astype(np.float64)
+    ref, _ = M.forward(flat, X, bf16_acts=True)
+    assert np.linalg.norm(got - ref) <= 1e-2 * np.linalg.norm(ref)
+    assert np.all(got[:, M.OUT:] == 0.0)
+
+
+def test_gradients_vs_oracle(torch, mech):
+    from fortrancfd_amd.lib import Surrogate
+    s = Surrogate(mech)
+    flat = network(6)
+    s.set_params(flat)
+    rows = 1024
+    X = inputs(rows, 7)
+    T = np.random.default_rng(8).normal(size=(rows, M.OUTP)).astype(np.float32)
+    T[:, M.OUT:] = 0
+    idx = torch.arange(rows, dtype=torch.int32, device="cuda")
+    print("train step lr=0, 1024 rows", flush=True)
+    loss = s.train_step(dev_bf16(torch, X), torch.from_numpy(T).cuda(), idx, 0.0, want_loss=True)
+    g = s.get_grads().astype(np.float64)
+    ref_loss, ref_g = M.loss_and_grads(flat, X, T.astype(np.float64), bf16_acts=True)
+    assert abs(loss - ref_loss) <= 1e-3 * ref_loss
+    gl, rl = M.unpack(g), M.unpack(ref_g)
+    for l in range(5):
+        for a, b in zip(gl[l], rl[l]):
+            rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+            cos = (a * b).sum() / (np.linalg.norm(a) * np.linalg.norm(b))
+            assert rel <= 3e-2 and cos >= 0.999, (l, rel, cos)
+    # the FP64 (no BF16) gradient is the same direction
+    _, g64 = M.loss_and_grads(flat, X, T.astype(np.float64))
+    assert (g * g64).sum() / (np.linalg.norm(g) * np.linalg.norm(g64)) >= 0.995
+
+
+def test_training_reduces_loss(torch, mech):
+    """Regression onto a smooth fixed map of the inputs: 400 Adam steps reduce the loss fivefold."""
+    from fortrancfd_amd.lib import Surrogate
+    s = Surrogate(mech)
+    s.set_params(M.init_params(11))
+    rows = 16384
+    X = inputs(rows, 12)
+    R = np.random.default_rng(13).normal(size=(M.INP, M.OUTP)) / np.sqrt(M.IN)
+    T = np.tanh(X @ R).astype(np.float32)
+    T[:, M.OUT:] = 0
+    Xd, Td = dev_bf16(torch, X), torch.from_numpy(T).cuda()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    losses = []
+    print("train 400 steps", flush=True)
+    for step in range(400):
+        idx = torch.randint(0, rows, (2048,), generator=g, device="cuda", dtype=torch.int32)
+        l = s.train_step(Xd, Td, idx, 1e-3, want_loss=(step % 50 == 0 or step == 399))
+        if l is not None:
+            losses.append(l)
+    assert all(np.isfinite(losses))
+    assert losses[-1] < 0.2 * losses[0], losses
+
+
+def config3_cells(mech, torch, n, seed):
+    from fortrancfd_amd import workloads
+    rho, dt, T, Y = workloads.chemistry_batch(mech, n, seed=seed)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return d(rho), d(dt), d(T), d(Y.T.copy())
+
+
+def test_out_of_range_cells_take_the_fp64_path_bitwise(torch, mech):
+    from fortrancfd_amd.lib import Surrogate
+    s = Surrogate(mech)
+    s.set_params(M.init_params(1))
+    norm = {"in_mean": np.zeros(64), "in_std": np.ones(64), "lo": np.full(64, -1e300),
+            "hi": np.full(64, 1e300), "out_mean": np.zeros(64), "out_std": np.ones(64)}
+    norm["lo"][M.IN - 3] = 1e9                     # T >= 1e9 K: no cell is in range
+    s.set_norm(norm)
+    rho, dt, T, Y = config3_cells(mech, torch, 256, 21)
+    T2, Y2 = T.clone(), Y.clone()
+    acc = torch.zeros(256, dtype=torch.int32, device="cuda")
+    print("infer_batch all-fallback, 256 cells", flush=True)
+    nf = s.infer_batch(rho, dt, T, Y, accepted=acc)
+    assert nf == 256 and int(acc.sum()) == 0
+    mech.chem_batch_device(rho, dt, T2, Y2)
+    assert torch.equal(T, T2) and torch.equal(Y, Y2)
+
+
+def test_accepted_cells_conserve_elements_and_energy(torch, mech, orc):
+    """Accepted cells satisfy the acceptance invariants whatever the network predicts: a network
+    with zero output weights predicts dY = out_mean, dT = out_mean (here zero), so the projection,
+    clipping and energy inversion are exercised on the identity map."""
+    from fortrancfd_amd.lib import Surrogate
+    s = Surrogate(mech)
+    flat = M.init_params(2)
+    lay = M.unpack(flat)
+    lay[4][0][:] = 0
+    s.set_params(M.pack(lay))
+    rng = np.random.default_rng(3)
+    out_mean = np.zeros(64)
+    out_mean[:M.OUT - 1] = rng.normal(0, 1e-7, M.OUT - 1)   # small composition shift
+    norm = {"in_mean": np.zeros(64), "in_std": np.ones(64), "lo": np.full(64, -1e300),
+            "hi": np.full(64, 1e300), "out_mean": out_mean, "out_std": np.ones(64)}
+    s.set_norm(norm)
+    s.set_thresholds(1e-5, 1e-1)
+    n = 512
+    rho, dt, T, Y = config3_cells(mech, torch, n, 22)
+    T0, Y0 = T.cpu().numpy(), Y.cpu().numpy().T.copy()
+    acc = torch.zeros(n, dtype=torch.int32, device="cuda")
+    print("infer_batch identity network, 512 cells", flush=True)
+    nf = s.infer_batch(rho, dt, T, Y, accepted=acc)
+    a = acc.cpu().numpy().astype(bool)
+    assert a.sum() > n // 2 and nf == n - a.sum()
+    T1, Y1 = T.cpu().numpy(), Y.cpu().numpy().T.copy()
+    W = orc.weights()
+    ncf = orc.ncf()
+    el0 = (Y0 / W) @ ncf
+    el1 = (Y1 / W) @ ncf
+    live = el0 > 0
+    assert np.all(np.abs(el1 - el0)[a][live[a]] <= 1.01e-5 * el0[a][live[a]])
+    assert np.all(np.abs(Y1[a].sum(axis=1) - 1.0) <= 1e-12)
+    assert np.all(Y1[a] >= 0)
+    for c in np.flatnonzero(a)[:64]:
+        e0 = orc.specific_energy(T0[c], orc.adjust_sum(orc.ytx(Y0[c])))
+        e1 = orc.specific_energy(T1[c], orc.adjust_sum(orc.ytx(Y1[c])))
+        assert abs(e1 - e0) <= 1e-10 * abs(e0)
