@@ -31,7 +31,7 @@ using namespace fcfd;
 
 namespace {
 
-constexpr int NTIMERS = 6;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry, 4 source Qc refresh, 5 surrogate network
+constexpr int NTIMERS = 7;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry SDIRK, 4 source Qc refresh, 5 surrogate, 6 explicit screen
 
 int lvir_of(const fcfd_opts &o) {                                  // SRC/params.f90:161-173
     if (o.flag_acu == 0) return 1;
@@ -90,6 +90,8 @@ struct fcfd_ctx {
     ChemCellStats *d_stats = nullptr;
     fcfd_surrogate *sur = nullptr;
     int *d_accepted = nullptr;
+    int *d_list = nullptr;               // cells the explicit screen sends to the implicit integrator
+    unsigned *d_list_n = nullptr;
 };
 
 namespace fcfd {
@@ -226,22 +228,36 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
     a.dt = dt; a.dt_dev = dt_dev; a.stats = c->d_stats;
     if (want_hrr) { a.hrr = c->d_hrr; a.hrr_dot = c->d_hrr_dot; }
     a.layout = 0;
-    hipError_t e;
+    a.route = c->sur ? c->d_accepted : nullptr;
+    const int n = c->nblk * c->ncell;
+    // explicit screen, one cell per lane: quiescent cells are final, the rest form the stiff list
+    hipError_t e = timed(c, 6, [&]() {
+        hipError_t e2 = launch_zero_u32(c->d_list_n, c->stream);
+        if (e2 != hipSuccess) return e2;
+        a.stiff = c->d_list;
+        a.stiff_n = c->d_list_n;
+        return launch_chem_screen(a, c->kk, c->stream);
+    });
+    if (e != hipSuccess) return e;
     if (c->sur) {
         fcfd_surrogate *s = c->sur;
-        const int n = c->nblk * c->ncell, npad = int(round_up(n, 64));
         e = timed(c, 5, [&]() {
-            hipError_t e2 = mlp_features_domain(s->d_norm, c->thermo, c->kk, c->nblk, c->ncell, c->lv, c->S, c->P,
-                                                c->qf, dt_dev, dt, npad, s->X, s->in_range, c->stream);
+            FeatArgs fa{};
+            fa.layout = 0; fa.n = n; fa.qf = c->qf; fa.P = c->P; fa.ncell = c->ncell; fa.lv = c->lv; fa.S = c->S;
+            fa.dt_dev = dt_dev; fa.dt = dt;
+            hipError_t e2 = s->network(fa, c->d_list, c->d_list_n, n, c->stream);
             if (e2 != hipSuccess) return e2;
-            return s->forward(s->X, npad, s->Yh, false, c->stream);
+            ApplyArgs ap{};
+            ap.layout = 0; ap.qf = c->qf; ap.P = c->P; ap.ncell = c->ncell; ap.lv = c->lv; ap.S = c->S;
+            ap.route = c->d_accepted;
+            return s->accept(ap, c->d_list, c->d_list_n, n, c->stream);
         });
         if (e != hipSuccess) return e;
-        ApplyArgs ap{};
-        ap.n = n; ap.layout = 0; ap.qf = c->qf; ap.P = c->P; ap.ncell = c->ncell; ap.lv = c->lv; ap.S = c->S;
-        ap.accepted = c->d_accepted;
-        e = timed(c, 3, [&]() { return s->apply_and_fallback(ap, a, chem_workgroups(c->mech), c->stream); });
+        e = timed(c, 3, [&]() { return s->fallback(a, chem_workgroups(c->mech), c->stream); });
     } else {
+        a.list = c->d_list;
+        a.list_n = c->d_list_n;
+        a.skip_explicit = 1;
         e = timed(c, 3, [&]() { return launch_chem(a, c->kk, chem_workgroups(c->mech), c->stream); });
     }
     if (e != hipSuccess) return e;
@@ -278,6 +294,8 @@ void fcfd_mech_free(fcfd_mech *m) {
     (void)hipFree(m->d_thermo);
     (void)hipFree(m->d_kin);
     (void)hipFree(m->d_counter);
+    (void)hipFree(m->d_list);
+    (void)hipFree(m->d_list_n);
     delete m;
 }
 
@@ -339,6 +357,8 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
         HIP_TRY(hipMalloc(&c->d_hrr, sizeof(double) * nint));
         HIP_TRY(hipMalloc(&c->d_hrr_dot, sizeof(double) * nint));
         HIP_TRY(hipMalloc(&c->d_stats, sizeof(ChemCellStats) * nint));
+        HIP_TRY(hipMalloc(&c->d_list, sizeof(int) * nint));
+        HIP_TRY(hipMalloc(&c->d_list_n, sizeof(unsigned)));
         HIP_TRY(hipMemset(c->d_hrr, 0, sizeof(double) * nint));
         HIP_TRY(hipMemset(c->d_hrr_dot, 0, sizeof(double) * nint));
         HIP_TRY(hipMemset(c->d_stats, 0, sizeof(ChemCellStats) * nint));
@@ -355,7 +375,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->qf); (void)hipFree(c->F); (void)hipFree(c->god); (void)hipFree(c->cells);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
-    (void)hipFree(c->d_accepted);
+    (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_list_n);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -525,7 +545,17 @@ int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double 
     a.b_rho = rho; a.b_dt = dt; a.b_T = T; a.b_y = Y;
     a.stats = reinterpret_cast<ChemCellStats *>(stats);
     a.layout = 1;
-    HIP_TRY(launch_chem(a, m->host.kk, chem_workgroups(m), static_cast<hipStream_t>(stream)));
+    fcfd_mech *mm = const_cast<fcfd_mech *>(m);
+    if (size_t(n) > mm->list_cap) {
+        if (mm->d_list) HIP_TRY(hipFree(mm->d_list));
+        mm->d_list = nullptr;
+        mm->list_cap = 0;
+        HIP_TRY(hipMalloc(&mm->d_list, sizeof(int) * size_t(n)));
+        mm->list_cap = size_t(n);
+    }
+    if (!mm->d_list_n) HIP_TRY(hipMalloc(&mm->d_list_n, sizeof(unsigned)));
+    HIP_TRY(launch_chem_full(a, m->host.kk, chem_workgroups(m), mm->d_list, mm->d_list_n,
+                             static_cast<hipStream_t>(stream)));
     HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     return FCFD_OK;
 }

@@ -14,6 +14,7 @@
 // 54x54 Newton matrix lives one row per lane in VGPRs and is factored by pivoted elimination
 // across lanes; vectors of the state are distributed one component per lane.
 #include "chem.hpp"
+#include "chem_device.hpp"
 #include "fcfd_device.hpp"
 
 namespace fcfd {
@@ -58,29 +59,6 @@ __device__ __forceinline__ int wave_argmax(double v) {
     return __builtin_amdgcn_readfirstlane(bi);
 }
 
-__device__ __forceinline__ double ipow(double x, int n) {
-    if (n < 0) return 1.0 / ipow(x, -n);
-    double r = 1.0;
-    bool first = true;
-    while (n) {
-        if (n & 1) { r = first ? x : r * x; first = false; }
-        n >>= 1;
-        if (n) x = x * x;
-    }
-    return r;
-}
-
-// CKSMH (cklib.f:7540-7560): S/R - H/RT of one species.
-__device__ __forceinline__ double smh_species(const Nasa &a, double T) {
-    const double t2 = T * T, t3 = T * t2, t4 = t2 * t2;
-    double s = (log(T) - 1.0) * a.a0;
-    s = s + T / 2.0 * a.a1;
-    s = s + t2 / 6.0 * a.a2;
-    s = s + t3 / 12.0 * a.a3;
-    s = s + t4 / 20.0 * a.a4;
-    return s + a.a6 - a.a5 / T;
-}
-
 template <int NSP>
 struct Shared {
     double J[(NSP + 1) * LDJ];   // Jacobian, overwritten by the LU factors of M
@@ -88,77 +66,6 @@ struct Shared {
     int perm[64];
     int cell;
 };
-
-// Effective rate constants (after fall-off and third-body factors) and rates of progress of
-// reaction i: CKRATT + CKRATX (cklib.f:6662-6718, 6851-6948).
-struct Rop {
-    double kf, kr, qf, qr;
-};
-
-__device__ Rop reaction(const DevKin *__restrict__ K, int i, double T, double alogt, double pfac,
-                        double ctot, const double *__restrict__ c, const double *__restrict__ smh) {
-    double kf = K->A[i] * exp(K->B[i] * alogt - K->E[i] / T);
-    double s = 0.0;
-    for (int n = 0; n < 6; ++n) {
-        const int k = K->nk[i][n];
-        if (k >= 0) s = s + (double)K->nu[i][n] * smh[k];
-    }
-    double eqk = s != 0.0 ? exp(fmin(s, K->exparg)) : 1.0;
-    eqk = eqk * ipow(pfac, K->nus[i]);
-    double kr = K->rev[i] ? kf / fmax(eqk, K->small) : 0.0;
-    double ctb = 1.0;
-    const int t = K->tb[i];
-    if (t >= 0) {
-        ctb = ctot;
-        for (int l = 0; l < K->ntbs[t]; ++l) ctb = ctb + (K->tba[t][l] - 1.0) * c[K->tbk[t][l]];
-    }
-    const int f = K->fo[i];
-    if (f >= 0) {
-        const double *fp = K->fpar[f];
-        const double rklow = fp[0] * exp(fp[1] * alogt - fp[2] / T);
-        double pr;
-        if (K->kfal[f] < 0) { pr = rklow * ctb / kf; ctb = 1.0; }
-        else pr = rklow * c[K->kfal[f]] / kf;
-        double pcor = pr / (1.0 + pr);
-        if (K->ifop[f] > 1) {
-            const double prlog = log10(fmax(pr, K->small));
-            double fc;
-            if (K->ifop[f] == 2) {
-                const double xp = 1.0 / (1.0 + prlog * prlog);
-                fc = pow(fp[3] * exp(-fp[4] / T) + exp(-T / fp[5]), xp) * fp[6] * pow(T, fp[7]);
-            } else {
-                double fcent = (1.0 - fp[3]) * exp(-T / fp[4]) + fp[3] * exp(-T / fp[5]);
-                if (K->ifop[f] == 4) fcent = fcent + exp(-fp[6] / T);
-                const double fclog = log10(fmax(fcent, K->small));
-                // single-precision literals of cklib.f:6915-6917
-                const double xn = 0.75 - (double)1.27f * fclog;
-                const double cprlog = prlog - ((double)0.4f + (double)0.67f * fclog);
-                const double q = cprlog / (xn - (double)0.14f * cprlog);
-                fc = pow(10.0, fclog / (1.0 + q * q));
-            }
-            pcor = fc * pcor;
-        }
-        kf = kf * pcor;
-        kr = kr * pcor;
-    }
-    kf = kf * ctb * 1.0;
-    kr = kr * ctb * 1.0;
-    Rop r{kf, kr, 0.0, 0.0};
-    const int *nu = K->nu[i], *nk = K->nk[i];
-    if (nu[0] != 0) {
-        r.qf = kf * ipow(c[nk[0]], abs(nu[0]));
-        r.qr = kr * ipow(c[nk[3]], nu[3]);
-        if (nk[1] >= 0) {
-            r.qf = r.qf * ipow(c[nk[1]], abs(nu[1]));
-            if (nk[2] >= 0) r.qf = r.qf * ipow(c[nk[2]], abs(nu[2]));
-        }
-        if (nk[4] >= 0) {
-            r.qr = r.qr * ipow(c[nk[4]], nu[4]);
-            if (nk[5] >= 0) r.qr = r.qr * ipow(c[nk[5]], nu[5]);
-        }
-    }
-    return r;
-}
 
 struct RhsOut {
     double f;      // this lane's component of dy/dt
@@ -194,7 +101,7 @@ __device__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict
     __syncthreads();
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
     for (int i = lane; i < K->ii; i += 64) {
-        const Rop r = reaction(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
+        const Rop r = reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
         const int *nu = K->nu[i], *nk = K->nk[i];
         if (!split) {
             const double q = r.qf - r.qr;
@@ -242,7 +149,7 @@ __device__ void jacobian(const DevThermo *__restrict__ th, const DevKin *__restr
     for (int i = lane; i < K->ii; i += 64) {
         const int *nu = K->nu[i], *nk = K->nk[i];
         if (nu[0] == 0) continue;
-        const Rop r = reaction(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
+        const Rop r = reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
         double dq[6];
         for (int s = 0; s < 6; ++s) {
             dq[s] = 0.0;
@@ -417,7 +324,7 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
         int rank = -1;
         // ---- explicit pre-step: Heun over the whole dt, accepted when the embedded Euler error
         //      estimate is within tolerance (quiescent cells never build a Jacobian) ----
-        {
+        if (!a.skip_explicit) {
             const double w = lane < NSP ? 1.0 / (a.atol + a.rtol * fabs(y))
                                         : (lane == NSP ? 1.0 / (a.rtol * fabs(y)) : 0.0);
             const RhsOut f0 = rhs<NSP>(th, K, sh, rho, y, bcast(y, NSP), false);
@@ -532,6 +439,25 @@ __global__ void k_zero_counter(unsigned int *c) {
 }
 
 }  // namespace
+
+hipError_t launch_zero_u32(unsigned *p, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_chem_full(ChemArgs a, int nsp, int n_workgroups, int *list, unsigned *list_n, hipStream_t st) {
+    hipError_t e = launch_zero_u32(list_n, st);
+    if (e != hipSuccess) return e;
+    a.stiff = list;
+    a.stiff_n = list_n;
+    a.list = nullptr;
+    a.list_n = nullptr;
+    if ((e = launch_chem_screen(a, nsp, st)) != hipSuccess) return e;
+    a.list = list;
+    a.list_n = list_n;
+    a.skip_explicit = 1;
+    return launch_chem(a, nsp, n_workgroups, st);
+}
 
 hipError_t launch_chem(const ChemArgs &a, int nsp, int n_workgroups, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, a.counter);

@@ -52,9 +52,22 @@ struct ChemArgs {
     // optional work list: cells list[0 .. *list_n) instead of 0 .. ncells (both device memory)
     const int *list;
     const unsigned *list_n;
+    int skip_explicit;           // 1: the explicit pre-step already failed for every listed cell
+    // explicit screen output: cells that need the implicit integrator (list + length)
+    int *stiff;
+    unsigned *stiff_n;
+    int *route;                  // optional per-cell route code: 2 = accepted by the explicit screen
     int layout;                  // 0: domain, 1: batch
 };
 
+// Wave-per-cell SDIRK integrator over all cells or over a work list (persistent grid).
 hipError_t launch_chem(const ChemArgs &a, int nsp, int n_workgroups, hipStream_t st);
+// Lane-per-cell explicit screen over cells 0 .. ncells: accepted cells are final, the others are
+// appended to a.stiff (a.stiff_n must be zero before the launch).
+hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st);
+hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
+// Screen, then SDIRK on the cells the screen rejected; list / list_n are scratch for ncells cells.
+hipError_t launch_chem_full(ChemArgs a, int nsp, int n_workgroups, int *list, unsigned *list_n,
+                            hipStream_t st);
 
 }  // namespace fcfd

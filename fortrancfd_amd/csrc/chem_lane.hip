@@ -1,0 +1,315 @@
+// chem_lane.hip -- explicit screen of the chemistry source: ONE CELL PER LANE.
+//
+// Every cell first tries the integrator's explicit pre-step (Heun over the whole dt with the
+// embedded Euler error estimate, accepted when the weighted RMS error is <= 1, then the
+// energy-consistent temperature).  Quiescent cells -- the cold bulk of a reacting domain -- end
+// here; the others are appended to a work list for the wave-per-cell SDIRK kernel (chem.hip),
+// which then starts from the unchanged state.  The arithmetic is the pre-step of that kernel; only
+// the order of the species sums differs.
+//
+// Mapping: a workgroup of four waves owns 64 cells (lane l of every wave is cell l).  Concentrations
+// and S/R - H/RT of the 64 cells live in LDS as species-major [k][64] blocks; each wave evaluates a
+// quarter of the reactions (i = wave mod 4) for all 64 cells and scatters its rates of progress
+// into its own production-rate block, so no two threads ever add into the same word and the four
+// partial sums are combined in a fixed order.  Species-wise work (thermo, state update, norms) is
+// split over the waves by k = wave mod 4; cell-wise scalars (T, sums) are combined through LDS.
+// Reaction tables are read with a wave-uniform reaction index (scalar loads).
+#include "chem.hpp"
+#include "chem_device.hpp"
+#include "fcfd_device.hpp"
+
+namespace fcfd {
+
+namespace {
+
+constexpr int NW = 4;          // waves per workgroup
+
+template <int NSP>
+struct LaneShared {
+    double C[NSP * 64];        // concentrations, mol/cm^3
+    double G[NSP * 64];        // S/R - H/RT; reused for cross-wave sums after the reaction loop
+    double W[NW][NSP * 64];    // per-wave production-rate partial sums; W[0] start doubles as sum slots
+};
+
+// Species slots of wave w: k = w + NW * j.
+template <int NSP>
+constexpr int slots() { return (NSP + NW - 1) / NW; }
+
+__device__ __forceinline__ Nasa nasa_lane(const DevThermo *__restrict__ m, int k, double T) {
+    const double *lo = m->alo[k], *hi = m->ahi[k];
+    const bool up = T > m->tmid[k];
+    return Nasa{up ? hi[0] : lo[0], up ? hi[1] : lo[1], up ? hi[2] : lo[2], up ? hi[3] : lo[3],
+                up ? hi[4] : lo[4], up ? hi[5] : lo[5], up ? hi[6] : lo[6]};
+}
+
+// Sum over the four waves of one value per lane, in wave order; `slot` is a [NW][64] LDS area that
+// no wave uses between the surrounding barriers.
+__device__ __forceinline__ double sum4(double *slot, int w, int lane, double v) {
+    slot[w * 64 + lane] = v;
+    __syncthreads();
+    const double s = ((slot[lane] + slot[64 + lane]) + slot[128 + lane]) + slot[192 + lane];
+    __syncthreads();
+    return s;
+}
+
+// f = dy/dt at (y, T) for this wave's species slots (f[j] of species w + NW*j) and fT for all
+// waves.  rho in g/cm^3.  y[] beyond NSP is ignored.
+template <int NSP>
+__device__ void rhs_lane(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, LaneShared<NSP> &sh,
+                         int w, int lane, double rho, const double (&y)[slots<NSP>()], double T,
+                         double (&f)[slots<NSP>()], double &fT) {
+    constexpr int NS = slots<NSP>();
+    const TPow p = tpow(T, th->ru);
+    double uk[NS];
+    double ctot_w = 0.0, cvm_w = 0.0;
+    const double lnT = log(T);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const int k = w + NW * j;
+        uk[j] = 0.0;
+        if (k < NSP) {
+            const double ck = rho * y[j] / th->wt[k];
+            sh.C[k * 64 + lane] = ck;
+            ctot_w = ctot_w + ck;
+            const Nasa a = nasa_lane(th, k, T);
+            uk[j] = h_mol(a, p) - p.rut;
+            cvm_w = cvm_w + y[j] * (cp_mol(a, p) - th->ru) / th->wt[k];
+            const double t2 = T * T, t3 = T * t2, t4 = t2 * t2;     // CKSMH, as smh_species
+            double s = (lnT - 1.0) * a.a0;
+            s = s + T / 2.0 * a.a1;
+            s = s + t2 / 6.0 * a.a2;
+            s = s + t3 / 12.0 * a.a3;
+            s = s + t4 / 20.0 * a.a4;
+            sh.G[k * 64 + lane] = s + a.a6 - a.a5 / T;
+        }
+    }
+    double *slot = sh.W[0];
+    const double ctot = sum4(slot, w, lane, ctot_w);
+    const double cvm = sum4(slot, w, lane, cvm_w);
+    double *Wp = sh.W[w] + lane;
+    for (int k = 0; k < NSP; ++k) Wp[k * 64] = 0.0;
+    __syncthreads();
+    const double pfac = th->pa / (th->ru * T);
+    const double *Cp = sh.C + lane, *Gp = sh.G + lane;
+    for (int i = w; i < K->ii; i += NW) {
+        const Rop r = reaction<64>(K, i, T, lnT, pfac, ctot, Cp, Gp);
+        const double q = r.qf - r.qr;
+        const int *nu = K->nu[i], *nk = K->nk[i];
+#pragma unroll
+        for (int n = 0; n < 6; ++n)
+            if (nk[n] >= 0) Wp[nk[n] * 64] = Wp[nk[n] * 64] + q * (double)nu[n];
+    }
+    __syncthreads();
+    double su_w = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const int k = w + NW * j;
+        f[j] = 0.0;
+        if (k < NSP) {
+            const int o = k * 64 + lane;
+            const double wd = ((sh.W[0][o] + sh.W[1][o]) + sh.W[2][o]) + sh.W[3][o];
+            f[j] = wd * th->wt[k] / rho;
+            su_w = su_w + uk[j] * wd;
+        }
+    }
+    __syncthreads();                       // every wave has read W before G is reused
+    const double su = sum4(sh.G, w, lane, su_w);
+    fT = -su / (rho * cvm);
+}
+
+// Mixture e = sum Y u / W and cv (erg/g) over all species at T for this cell.
+template <int NSP>
+__device__ void energy_lane(const DevThermo *__restrict__ th, double *slot, int w, int lane,
+                            const double (&y)[slots<NSP>()], double T, double &e, double &cv) {
+    constexpr int NS = slots<NSP>();
+    const TPow p = tpow(T, th->ru);
+    double eu = 0.0, ec = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const int k = w + NW * j;
+        if (k < NSP) {
+            const Nasa a = nasa_lane(th, k, T);
+            eu = eu + y[j] * (h_mol(a, p) - p.rut) / th->wt[k];
+            ec = ec + y[j] * (cp_mol(a, p) - th->ru) / th->wt[k];
+        }
+    }
+    e = sum4(slot, w, lane, eu);
+    cv = sum4(slot, w, lane, ec);
+}
+
+// adjust_sum (SRC/thermo.f90:136-149) of this wave's slots: clip at 0, divide by the cell's sum.
+template <int NSP>
+__device__ void adjust_lane(double *slot, int w, int lane, double (&y)[slots<NSP>()]) {
+    constexpr int NS = slots<NSP>();
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        y[j] = (w + NW * j < NSP) ? fmax(y[j], 0.0) : 0.0;
+        s = s + y[j];
+    }
+    const double tot = sum4(slot, w, lane, s);
+#pragma unroll
+    for (int j = 0; j < NS; ++j) y[j] = y[j] / tot;
+}
+
+template <int NSP>
+__global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
+    constexpr int NS = slots<NSP>();
+    __shared__ LaneShared<NSP> sh;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const DevThermo *__restrict__ th = a.thermo;
+    const DevKin *__restrict__ K = a.kin;
+    const int cell0 = blockIdx.x * 64;
+    const int cell = cell0 + lane;
+    const bool live = cell < a.ncells;
+    const int c = live ? cell : cell0;          // idle lanes shadow a live cell and store nothing
+    double *slot = sh.W[0];
+
+    long col = 0;
+    double rho_si, dt, T0;
+    double y[NS], yraw[NS];
+    if (a.layout == 0) {
+        const int b = c / a.ncell, i = c % a.ncell;
+        col = long(b) * a.S + a.lv + i;
+        rho_si = a.qf[col];
+        T0 = a.qf[2 * a.P + col];
+        dt = a.dt_dev ? *a.dt_dev : a.dt;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int k = w + NW * j;
+            yraw[j] = k < NSP ? a.qf[(3 + k) * a.P + col] : 0.0;
+            y[j] = yraw[j];
+        }
+        adjust_lane<NSP>(slot, w, lane, y);
+    } else {
+        rho_si = a.b_rho[c];
+        dt = a.b_dt[c];
+        T0 = a.b_T[c];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int k = w + NW * j;
+            yraw[j] = k < NSP ? a.b_y[long(k) * a.ncells + c] : 0.0;
+            y[j] = yraw[j];
+        }
+    }
+    const double rho = rho_si * 1.e-3;
+    double e0, cv0;
+    {
+        double ya[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) ya[j] = y[j];
+        adjust_lane<NSP>(slot, w, lane, ya);
+        energy_lane<NSP>(th, slot, w, lane, ya, T0, e0, cv0);
+    }
+
+    // ---- Heun over dt with the embedded Euler error estimate ----
+    double f0[NS], f0T;
+    rhs_lane<NSP>(th, K, sh, w, lane, rho, y, T0, f0, f0T);
+    double y1[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) y1[j] = y[j] + dt * f0[j];
+    const double T1 = T0 + dt * f0T;
+    double f1[NS], f1T;
+    rhs_lane<NSP>(th, K, sh, w, lane, rho, y1, T1, f1, f1T);
+    double err_w = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        if (w + NW * j < NSP) {
+            const double wt = 1.0 / (a.atol + a.rtol * fabs(y[j]));
+            const double t = ((dt * 0.5) * (f1[j] - f0[j])) * wt;
+            err_w = err_w + t * t;
+        }
+    }
+    if (w == 0) {
+        const double t = ((dt * 0.5) * (f1T - f0T)) * (1.0 / (a.rtol * fabs(T0)));
+        err_w = err_w + t * t;
+    }
+    const double err = sqrt(sum4(slot, w, lane, err_w) / double(NSP + 1));
+    const bool ok = live && err <= 1.0;
+
+    if (!ok) {
+        if (live && w == 0) {
+            const unsigned long long bal = __ballot(1);
+            const int leader = __ffsll((long long)bal) - 1;
+            const int rank = __popcll(bal & ((1ull << lane) - 1ull));
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(a.stiff_n, unsigned(__popcll(bal)));
+            base = __shfl(base, leader, 64);
+            a.stiff[base + rank] = cell;
+        }
+    }
+    // ---- accepted cells: state update and energy-consistent temperature ----
+#pragma unroll
+    for (int j = 0; j < NS; ++j) y[j] = y[j] + (dt * 0.5) * (f0[j] + f1[j]);
+    double T = T0 + (dt * 0.5) * (f0T + f1T);
+    adjust_lane<NSP>(slot, w, lane, y);
+    bool done = !ok;
+    for (int it = 0; it < 20; ++it) {
+        if (!__syncthreads_or(done ? 0 : 1)) break;
+        double ek, cvk;
+        energy_lane<NSP>(th, slot, w, lane, y, T, ek, cvk);
+        if (!done) {
+            const double d = (e0 - ek) / cvk;
+            T = T + d;
+            if (fabs(d) <= 1.0e-12 * T) done = true;
+        }
+    }
+    double hrr_s = 0.0, hrr_d = 0.0;
+    if (a.hrr) {
+        // heat_release_rate (SRC/thermo.f90:98-101) and heat_release_rate_omegadot
+        // (SRC/thermo.f90:128-132): the creation minus destruction rates of CKCDYR are the net
+        // production rates, evaluated at the new state.
+        double fn[NS], fnT;
+        rhs_lane<NSP>(th, K, sh, w, lane, rho, y, T, fn, fnT);
+        double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int k = w + NW * j;
+            if (k < NSP) {
+                s1 = s1 + rho * th->h298[k] * (y[j] - yraw[j]) / dt;
+                s2 = s2 + th->h298[k] * fn[j] * rho;     // W_k h_k wdot_k = h_k (W_k wdot_k / rho) rho
+            }
+        }
+        hrr_s = sum4(slot, w, lane, s1);
+        hrr_d = sum4(slot, w, lane, s2);
+    }
+    if (!ok) return;
+    if (a.layout == 0) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int k = w + NW * j;
+            if (k < NSP) a.qf[(3 + k) * a.P + col] = y[j];
+        }
+        if (w == 0) {
+            a.qf[2 * a.P + col] = T;
+            if (a.hrr) {
+                a.hrr[cell] = -hrr_s * 1.e-1;
+                a.hrr_dot[cell] = -hrr_d * 1.e-1;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int k = w + NW * j;
+            if (k < NSP) a.b_y[long(k) * a.ncells + cell] = y[j];
+        }
+        if (w == 0) a.b_T[cell] = T;
+    }
+    if (w == 0) {
+        if (a.stats) a.stats[cell] = ChemCellStats{1, 0, 0, 0};
+        if (a.route) a.route[cell] = 2;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st) {
+    const unsigned nb = unsigned((a.ncells + 63) / 64);
+    if (nb == 0) return hipSuccess;
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_screen<N>), dim3(nb), dim3(256), 0, st, a); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace fcfd

@@ -68,4 +68,7 @@ struct fcfd_mech {
     fcfd::DevThermo *d_thermo = nullptr;
     void *d_kin = nullptr;               // fcfd::DevKin
     unsigned int *d_counter = nullptr;   // work queue of batched chemistry calls
+    int *d_list = nullptr;               // stiff-cell list of batched chemistry calls (grown on demand)
+    unsigned *d_list_n = nullptr;
+    size_t list_cap = 0;
 };

@@ -40,6 +40,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs a) {
     const int tiles_n = a.N / 64;
     const int tile = blockIdx.x;
     const int m0 = (tile / tiles_n) * 64, n0 = (tile % tiles_n) * 64;
+    if (a.m_dev && a.m_off + m0 >= int(*a.m_dev)) return;   // uniform per workgroup, before any barrier
     const int split = blockIdx.y;
     const int kchunk = a.K / a.ksplit;
     const int kbeg = split * kchunk, kend = kbeg + kchunk;
@@ -133,15 +134,40 @@ __device__ __forceinline__ void features_of(const SurrogateNorm *__restrict__ nm
     if (in_range) *in_range = ok ? 1 : 0;
 }
 
-__global__ void k_features_batch(const SurrogateNorm *nm, const DevThermo *th, int kk, int n, int npad, const double *rho,
-                                 const double *dt, const double *T, const double *Y, uint16_t *X, int *in_range) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= npad) return;
-    if (c >= n) {
-        for (int j = 0; j < MLP_INP; ++j) X[long(c) * MLP_INP + j] = 0;
-        return;
+// One row of features per cell: rows 0 .. npad of all cells (list == null; rows >= n zero-filled)
+// or rows 0 .. *list_n for the cells list[row].
+__global__ void k_features(FeatArgs f) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int cell = t;
+    if (f.list) {
+        if (t >= int(*f.list_n)) return;
+        cell = f.list[t];
+    } else {
+        if (t >= f.npad) return;
+        if (t >= f.n) {
+            for (int j = 0; j < MLP_INP; ++j) f.X[long(t) * MLP_INP + j] = 0;
+            if (f.in_range) f.in_range[t] = 0;
+            return;
+        }
     }
-    features_of(nm, th, kk, rho[c], T[c], dt[c], Y + c, n, X + long(c) * MLP_INP, in_range ? in_range + c : nullptr);
+    const int kk = f.kk;
+    double y[MAXK];
+    double rho, T, dt;
+    if (f.layout == 0) {
+        const long col = long(cell / f.ncell) * f.S + f.lv + (cell % f.ncell);
+        double s = 0.0;
+        for (int k = 0; k < kk; ++k) { y[k] = fmax(f.qf[(3 + k) * f.P + col], 0.0); s += y[k]; }
+        for (int k = 0; k < kk; ++k) y[k] /= s;
+        rho = f.qf[col];
+        T = f.qf[2 * f.P + col];
+        dt = f.dt_dev ? *f.dt_dev : f.dt;
+    } else {
+        for (int k = 0; k < kk; ++k) y[k] = f.Y[long(k) * f.n + cell];
+        rho = f.rho[cell];
+        T = f.T[cell];
+        dt = f.dtv[cell];
+    }
+    features_of(f.norm, f.thermo, kk, rho, T, dt, y, 1, f.X + long(t) * MLP_INP, f.in_range ? f.in_range + t : nullptr);
 }
 
 // Normalised targets: (Y1 - Y0, T1 - T0) z-scored, columns >= 54 zero.
@@ -250,8 +276,14 @@ __global__ void k_to_bf16(const float *w, uint16_t *w16, long n) {
 // ---------------------------------------------------------------------------------------------
 template <int NSP>
 __global__ void k_apply(ApplyArgs a) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;   // network row
+    int t = row;                                               // cell
+    if (a.list) {
+        if (row >= int(*a.list_n)) return;
+        t = a.list[row];
+    } else if (row >= a.n) {
+        return;
+    }
     const DevThermo *__restrict__ th = a.thermo;
     const SurrogateNorm *__restrict__ nm = a.norm;
     long col = 0;
@@ -270,12 +302,12 @@ __global__ void k_apply(ApplyArgs a) {
 #pragma unroll
         for (int k = 0; k < NSP; ++k) y0[k] = a.Y[long(k) * a.n + t];
     }
-    const float *out = a.Yh + long(t) * MLP_OUTP;
+    const float *out = a.Yh + long(row) * MLP_OUTP;
     double y1[NSP];
 #pragma unroll
     for (int k = 0; k < NSP; ++k) y1[k] = y0[k] + (double(out[k]) * nm->out_std[k] + nm->out_mean[k]);
     const double Tnet = T0 + (double(out[NSP]) * nm->out_std[NSP] + nm->out_mean[NSP]);
-    bool ok = a.in_range[t] != 0;
+    bool ok = a.in_range[row] != 0;
     // weighted projection onto element conservation (5 x 5 normal equations)
     constexpr int ME = 8;
     const int mm = a.mm;
@@ -357,7 +389,7 @@ __global__ void k_apply(ApplyArgs a) {
         if (fabs(d) <= 1.0e-12 * T1) break;
     }
     if (!(fabs(T1 - Tnet) <= a.t_tol * T1)) ok = false;
-    if (a.accepted) a.accepted[t] = ok ? 1 : 0;
+    if (a.route) a.route[t] = ok ? 1 : 0;
     if (!ok) {
         const unsigned q = atomicAdd(a.nfallback, 1u);
         a.fallback[q] = t;
@@ -374,43 +406,14 @@ __global__ void k_apply(ApplyArgs a) {
     }
 }
 
-__global__ void k_features_domain(const SurrogateNorm *nm, const DevThermo *th, int kk, int nblk, int ncell, int lv, int S,
-                                  long P, const double *qf, const double *dt_dev, double dt, int npad, uint16_t *X,
-                                  int *in_range) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npad) return;
-    const int n = nblk * ncell;
-    if (t >= n) {
-        for (int j = 0; j < MLP_INP; ++j) X[long(t) * MLP_INP + j] = 0;
-        return;
-    }
-    const long col = long(t / ncell) * S + lv + (t % ncell);
-    double y[MAXK];
-    double s = 0.0;
-    for (int k = 0; k < kk; ++k) { y[k] = fmax(qf[(3 + k) * P + col], 0.0); s += y[k]; }
-    for (int k = 0; k < kk; ++k) y[k] /= s;
-    features_of(nm, th, kk, qf[col], qf[2 * P + col], dt_dev ? *dt_dev : dt, y, 1, X + long(t) * MLP_INP,
-                in_range + t);
-}
-
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
 static unsigned blocks(long n, int b) { return unsigned((n + b - 1) / b); }
 
-hipError_t mlp_features_batch(const SurrogateNorm *nm, const DevThermo *th, int kk, int n, int npad, const double *rho,
-                              const double *dt, const double *T, const double *Y, uint16_t *X, int *in_range,
-                              hipStream_t st) {
-    hipLaunchKernelGGL(k_features_batch, dim3(blocks(npad, 128)), dim3(128), 0, st, nm, th, kk, n, npad, rho, dt, T, Y,
-                       X, in_range);
-    return hipGetLastError();
-}
-
-hipError_t mlp_features_domain(const SurrogateNorm *nm, const DevThermo *th, int kk, int nblk, int ncell, int lv, int S,
-                               long P, const double *qf, const double *dt_dev, double dt, int npad, uint16_t *X,
-                               int *in_range, hipStream_t st) {
-    hipLaunchKernelGGL(k_features_domain, dim3(blocks(npad, 128)), dim3(128), 0, st, nm, th, kk, nblk, ncell, lv, S, P,
-                       qf, dt_dev, dt, npad, X, in_range);
+hipError_t mlp_features(const FeatArgs &f, int rows, hipStream_t st) {
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_features, dim3(blocks(rows, 128)), dim3(128), 0, st, f);
     return hipGetLastError();
 }
 
@@ -467,6 +470,7 @@ hipError_t mlp_to_bf16(const float *w, uint16_t *w16, long n, hipStream_t st) {
 }
 
 hipError_t mlp_apply(const ApplyArgs &a, int nsp, hipStream_t st) {
+    if (a.n <= 0) return hipSuccess;
 #define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_apply<N>), dim3(blocks(a.n, 64)), dim3(64), 0, st, a); return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE

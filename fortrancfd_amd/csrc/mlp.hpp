@@ -38,6 +38,8 @@ struct GemmArgs {
     const float *bias;
     long ldc;
     int epilogue;
+    const unsigned *m_dev;   // optional device row count: 64-row tiles with m_off + m0 >= *m_dev are skipped
+    int m_off;
 };
 
 hipError_t launch_gemm(const GemmArgs &a, hipStream_t st);

@@ -53,6 +53,7 @@ hipError_t Surrogate::reserve(int rows, bool training) {
     if ((e = grow(Yh, capYh, r * MLP_OUTP)) != hipSuccess) return e;
     if ((e = grow(in_range, capIR, r)) != hipSuccess) return e;
     if ((e = grow(fb_list, capFB, r)) != hipSuccess) return e;
+    if ((e = grow(stiff, capST, r)) != hipSuccess) return e;
     const size_t hr = std::min<size_t>(r, size_t(INFER_CHUNK));
     for (int i = 0; i < 2; ++i)
         if ((e = grow(Hping[i], capH[i], hr * MLP_HID)) != hipSuccess) return e;
@@ -72,8 +73,10 @@ hipError_t Surrogate::reserve(int rows, bool training) {
 
 hipError_t Surrogate::gemm(const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, int M,
                            int N, int K, int ksplit, int ep, float *C32, uint16_t *C16, float *P32,
-                           const float *bias, long ldc, hipStream_t st) const {
+                           const float *bias, long ldc, hipStream_t st, const unsigned *m_dev, int m_off) const {
     GemmArgs g{};
+    g.m_dev = m_dev;
+    g.m_off = m_off;
     g.A = A; g.B = B; g.sam = sam; g.sak = sak; g.sbk = sbk; g.sbn = sbn;
     g.M = M; g.N = N; g.K = K; g.ksplit = ksplit;
     g.C32 = C32; g.C16 = C16; g.P32 = P32; g.bias = bias; g.ldc = ldc; g.epilogue = ep;
@@ -82,7 +85,8 @@ hipError_t Surrogate::gemm(const uint16_t *A, long sam, long sak, const uint16_t
 
 // Forward of `rows` rows of X (rows multiple of 64).  Training keeps every layer's activations
 // and pre-activations; inference ping-pongs two activation buffers chunk by chunk.
-hipError_t Surrogate::forward(const uint16_t *x, int rows, float *out, bool training, hipStream_t st) {
+hipError_t Surrogate::forward(const uint16_t *x, int rows, float *out, bool training, hipStream_t st,
+                              const unsigned *m_dev) {
     hipError_t e;
     if (training) {
         const uint16_t *in = x;
@@ -101,12 +105,12 @@ hipError_t Surrogate::forward(const uint16_t *x, int rows, float *out, bool trai
         for (int l = 0; l < 4; ++l) {
             uint16_t *o = Hping[l & 1];
             e = gemm(in, MLP_K[l], 1, W16[l], MLP_N[l], 1, cr, MLP_N[l], MLP_K[l], 1, EP_BIAS_GELU, nullptr, o,
-                     nullptr, b32[l], MLP_N[l], st);
+                     nullptr, b32[l], MLP_N[l], st, m_dev, r0);
             if (e != hipSuccess) return e;
             in = o;
         }
         e = gemm(in, MLP_HID, 1, W16[4], MLP_OUTP, 1, cr, MLP_OUTP, MLP_HID, 1, EP_BIAS, out + long(r0) * MLP_OUTP,
-                 nullptr, nullptr, b32[4], MLP_OUTP, st);
+                 nullptr, nullptr, b32[4], MLP_OUTP, st, m_dev, r0);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -155,23 +159,43 @@ hipError_t Surrogate::adam(float lr, hipStream_t st) {
     return hipSuccess;
 }
 
-// Inference epilogue and the FP64 fallback for cells the surrogate does not accept.
-hipError_t Surrogate::apply_and_fallback(ApplyArgs &ap, ChemArgs &ca, int chem_wg, hipStream_t st) {
-    hipError_t e;
+hipError_t Surrogate::network(FeatArgs fa, const int *list, const unsigned *list_n, int rows_max, hipStream_t st) {
+    fa.norm = d_norm;
+    fa.thermo = mech->d_thermo;
+    fa.kk = mech->host.kk;
+    fa.list = list;
+    fa.list_n = list_n;
+    fa.X = X;
+    fa.in_range = in_range;
+    const int rows = int((long(rows_max) + 63) / 64 * 64);
+    hipError_t e = mlp_features(fa, rows, st);
+    if (e != hipSuccess) return e;
+    return forward(X, rows, Yh, false, st, list_n);
+}
+
+hipError_t Surrogate::accept(ApplyArgs ap, const int *list, const unsigned *list_n, int rows_max, hipStream_t st) {
     ap.thermo = mech->d_thermo;
     ap.norm = d_norm;
     ap.ncf = d_ncf;
     ap.mm = mech->host.mm;
+    ap.list = list;
+    ap.list_n = list_n;
     ap.Yh = Yh;
     ap.in_range = in_range;
     ap.nfallback = d_nfb;
     ap.fallback = fb_list;
     ap.resid_tol = resid_tol;
     ap.t_tol = t_tol;
-    if ((e = mlp_zero_u32(d_nfb, st)) != hipSuccess) return e;
-    if ((e = mlp_apply(ap, mech->host.kk, st)) != hipSuccess) return e;
+    ap.n = rows_max;
+    hipError_t e = launch_zero_u32(d_nfb, st);
+    if (e != hipSuccess) return e;
+    return mlp_apply(ap, mech->host.kk, st);
+}
+
+hipError_t Surrogate::fallback(ChemArgs ca, int chem_wg, hipStream_t st) {
     ca.list = fb_list;
     ca.list_n = d_nfb;
+    ca.skip_explicit = 1;
     return launch_chem(ca, mech->host.kk, chem_wg, st);
 }
 
@@ -183,7 +207,7 @@ Surrogate::~Surrogate() {
     for (int l = 0; l < 4; ++l) { f(H[l]); f(Pre[l]); }
     for (int i = 0; i < 2; ++i) { f(Hping[i]); f(dZ[i]); }
     f(X); f(Yh); f(Yt); f(dY); f(gpart); f(cs_part); f(loss_part); f(d_loss); f(in_range); f(fb_list);
-    f(d_nfb); f(d_norm); f(d_ncf);
+    f(stiff); f(d_stiff_n); f(d_nfb); f(d_norm); f(d_ncf);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -232,6 +256,7 @@ int fcfd_sur_create(const fcfd_mech *m, fcfd_surrogate **out) {
     HIP_TRY(hipMalloc(&s->gpart, gp * 4));
     HIP_TRY(hipMalloc(&s->d_loss, 4));
     HIP_TRY(hipMalloc(&s->d_nfb, sizeof(unsigned)));
+    HIP_TRY(hipMalloc(&s->d_stiff_n, sizeof(unsigned)));
     HIP_TRY(hipMalloc(&s->d_norm, sizeof(SurrogateNorm)));
     SurrogateNorm nm{};
     for (int c = 0; c < MLP_INP; ++c) { nm.in_std[c] = 1.0; nm.lo[c] = -1e300; nm.hi[c] = 1e300; }
@@ -324,9 +349,12 @@ int fcfd_sur_features(fcfd_surrogate *s, int n, const double *rho, const double 
                       const double *Y, void *X, int *in_range, void *stream) {
     if (!s || n < 1 || !rho || !dt || !T || !Y || !X) { set_error("fcfd_sur_features: invalid argument"); return FCFD_E_ARG; }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    const int npad = int((long(n) + 63) / 64 * 64);
-    HIP_TRY(mlp_features_batch(s->d_norm, s->mech->d_thermo, s->mech->host.kk, n, npad, rho, dt, T, Y,
-                               static_cast<uint16_t *>(X), in_range, st));
+    FeatArgs f{};
+    f.norm = s->d_norm; f.thermo = s->mech->d_thermo; f.kk = s->mech->host.kk; f.layout = 1;
+    f.n = n; f.npad = int((long(n) + 63) / 64 * 64);
+    f.rho = rho; f.dtv = dt; f.T = T; f.Y = Y;
+    f.X = static_cast<uint16_t *>(X); f.in_range = in_range;
+    HIP_TRY(mlp_features(f, f.npad, st));
     HIP_TRY(hipStreamSynchronize(st));
     return FCFD_OK;
 }
@@ -377,18 +405,23 @@ int fcfd_sur_infer_batch(fcfd_surrogate *s, const fcfd_opts *o, int n, const dou
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int npad = int((long(n) + 63) / 64 * 64);
     HIP_TRY(s->reserve(npad, false));
-    HIP_TRY(mlp_features_batch(s->d_norm, s->mech->d_thermo, s->mech->host.kk, n, npad, rho, dt, T, Y, s->X,
-                               s->in_range, st));
-    HIP_TRY(s->forward(s->X, npad, s->Yh, false, st));
-    ApplyArgs ap{};
-    ap.n = n; ap.layout = 1; ap.rho = rho; ap.T = T; ap.Y = Y; ap.accepted = accepted;
     ChemArgs ca{};
     ca.thermo = s->mech->d_thermo; ca.kin = static_cast<const DevKin *>(s->mech->d_kin); ca.counter = s->mech->d_counter;
     ca.ncells = n; ca.rtol = o->rtol; ca.atol = o->atol;
     ca.b_rho = rho; ca.b_dt = dt; ca.b_T = T; ca.b_y = Y;
     ca.stats = reinterpret_cast<ChemCellStats *>(stats);
     ca.layout = 1;
-    HIP_TRY(s->apply_and_fallback(ap, ca, chem_workgroups(s->mech), st));
+    ca.route = accepted;
+    HIP_TRY(launch_zero_u32(s->d_stiff_n, st));
+    ca.stiff = s->stiff; ca.stiff_n = s->d_stiff_n;
+    HIP_TRY(launch_chem_screen(ca, s->mech->host.kk, st));
+    FeatArgs fa{};
+    fa.layout = 1; fa.n = n; fa.rho = rho; fa.dtv = dt; fa.T = T; fa.Y = Y;
+    HIP_TRY(s->network(fa, s->stiff, s->d_stiff_n, n, st));
+    ApplyArgs ap{};
+    ap.layout = 1; ap.rho = rho; ap.T = T; ap.Y = Y; ap.route = accepted;
+    HIP_TRY(s->accept(ap, s->stiff, s->d_stiff_n, n, st));
+    HIP_TRY(s->fallback(ca, chem_workgroups(s->mech), st));
     if (n_fallback) {
         unsigned nf = 0;
         HIP_TRY(hipMemcpyAsync(&nf, s->d_nfb, sizeof nf, hipMemcpyDeviceToHost, st));

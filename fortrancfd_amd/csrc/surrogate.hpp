@@ -15,6 +15,24 @@ struct SurrogateNorm {
     double out_mean[MLP_OUTP], out_std[MLP_OUTP];
 };
 
+// Feature rows (k_features).
+struct FeatArgs {
+    const SurrogateNorm *norm;
+    const DevThermo *thermo;
+    int kk, layout;
+    int n, npad;                 // cells; rows written when list is null
+    const int *list;             // optional: row r <- cell list[r], r < *list_n
+    const unsigned *list_n;
+    const double *qf;            // layout 0
+    long P;
+    int ncell, lv, S;
+    const double *dt_dev;
+    double dt;
+    const double *rho, *dtv, *T, *Y;   // layout 1 (Y [KK][n])
+    uint16_t *X;
+    int *in_range;
+};
+
 // Inference epilogue (k_apply): network output -> accepted state or a fallback-list entry.
 struct ApplyArgs {
     const DevThermo *thermo;
@@ -26,20 +44,17 @@ struct ApplyArgs {
     int ncell, lv, S;
     const double *rho;          // layout 1
     double *T, *Y;              // layout 1: T[n], Y[KK][n]
-    const float *Yh;            // [n][MLP_OUTP] normalised network output
-    const int *in_range;        // [n]
-    int *accepted;              // [n] or null
+    const int *list;            // optional: row r is cell list[r], r < *list_n (else row = cell < n)
+    const unsigned *list_n;
+    const float *Yh;            // [rows][MLP_OUTP] normalised network output
+    const int *in_range;        // [rows]
+    int *route;                 // [n] or null: 1 surrogate accepted, 0 sent to the FP64 integrator
     unsigned *nfallback;        // fallback-list length (zeroed before the launch)
     int *fallback;              // fallback list (cell indices)
     double resid_tol, t_tol;
 };
 
-hipError_t mlp_features_batch(const SurrogateNorm *nm, const DevThermo *th, int kk, int n, int npad, const double *rho,
-                              const double *dt, const double *T, const double *Y, uint16_t *X, int *in_range,
-                              hipStream_t st);
-hipError_t mlp_features_domain(const SurrogateNorm *nm, const DevThermo *th, int kk, int nblk, int ncell, int lv, int S,
-                               long P, const double *qf, const double *dt_dev, double dt, int npad, uint16_t *X,
-                               int *in_range, hipStream_t st);
+hipError_t mlp_features(const FeatArgs &f, int rows, hipStream_t st);
 hipError_t mlp_targets(const SurrogateNorm *nm, int kk, int n, int npad, const double *T0, const double *Y0,
                        const double *T1, const double *Y1, float *Tg, hipStream_t st);
 hipError_t mlp_gather_bf16(const uint16_t *src, uint16_t *dst, const int *idx, int rows, int w, hipStream_t st);

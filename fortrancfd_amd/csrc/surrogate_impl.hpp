@@ -26,9 +26,9 @@ struct Surrogate {
     uint16_t *H[4] = {}, *Hping[2] = {}, *dY = nullptr, *dZ[2] = {};
     float *Pre[4] = {};
     float *gpart = nullptr, *cs_part = nullptr, *loss_part = nullptr, *d_loss = nullptr;
-    int *in_range = nullptr, *fb_list = nullptr;
-    unsigned *d_nfb = nullptr;
-    size_t capX = 0, capYh = 0, capYt = 0, capdY = 0, capIR = 0, capFB = 0, capCS = 0, capLP = 0;
+    int *in_range = nullptr, *fb_list = nullptr, *stiff = nullptr;
+    unsigned *d_nfb = nullptr, *d_stiff_n = nullptr;
+    size_t capX = 0, capYh = 0, capYt = 0, capdY = 0, capIR = 0, capFB = 0, capCS = 0, capLP = 0, capST = 0;
     size_t capH[2] = {}, capHl[4] = {}, capPre[4] = {}, capdZ[2] = {};
     SurrogateNorm h_norm{};
     SurrogateNorm *d_norm = nullptr;
@@ -39,11 +39,17 @@ struct Surrogate {
     hipError_t reserve(int rows, bool training);
     hipError_t gemm(const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, int M, int N, int K,
                     int ksplit, int ep, float *C32, uint16_t *C16, float *P32, const float *bias, long ldc,
-                    hipStream_t st) const;
-    hipError_t forward(const uint16_t *x, int rows, float *out, bool training, hipStream_t st);
+                    hipStream_t st, const unsigned *m_dev = nullptr, int m_off = 0) const;
+    // rows: rows allocated in x / out (multiple of 64); m_dev: optional device count of live rows
+    hipError_t forward(const uint16_t *x, int rows, float *out, bool training, hipStream_t st,
+                       const unsigned *m_dev = nullptr);
     hipError_t backward(int rows, hipStream_t st);
     hipError_t adam(float lr, hipStream_t st);
-    hipError_t apply_and_fallback(ApplyArgs &ap, ChemArgs &ca, int chem_wg, hipStream_t st);
+    // Surrogate branch of the chemistry source for the cells list[0 .. *list_n) that the explicit
+    // screen rejected: features + network (rows <= rows_max), acceptance, FP64 SDIRK for the rest.
+    hipError_t network(FeatArgs fa, const int *list, const unsigned *list_n, int rows_max, hipStream_t st);
+    hipError_t accept(ApplyArgs ap, const int *list, const unsigned *list_n, int rows_max, hipStream_t st);
+    hipError_t fallback(ChemArgs ca, int chem_wg, hipStream_t st);
     ~Surrogate();
 };
 

@@ -178,15 +178,19 @@ int fcfd_sur_forward(fcfd_surrogate *s, int rows, const void *X, float *Yh, void
  * (may be NULL; then the call does not wait for the device) receives the batch's MSE. */
 int fcfd_sur_train_step(fcfd_surrogate *s, const void *X_all, const float *Tg_all, const int *idx,
                         int batch, float lr, float *loss, void *stream);
-/* Surrogate + FP64 fallback on n independent cells, arrays as fcfd_chem_batch; accepted[n] and
- * stats[4n] may be NULL; *n_fallback (may be NULL) receives the number of FP64 cells. */
+/* Chemistry of n independent cells (arrays as fcfd_chem_batch) through the explicit screen, the
+ * surrogate for the cells the screen rejects, and the FP64 implicit integrator for the cells the
+ * surrogate does not accept.  route[n] (may be NULL) receives 2 (explicit screen), 1 (surrogate)
+ * or 0 (implicit integrator) per cell; stats[4n] may be NULL; *n_fallback (may be NULL) receives
+ * the number of implicit-integrator cells. */
 int fcfd_sur_infer_batch(fcfd_surrogate *s, const fcfd_opts *o, int n, const double *rho,
-                         const double *dt, double *T, double *Y, int *accepted, int *n_fallback,
+                         const double *dt, double *T, double *Y, int *route, int *n_fallback,
                          int *stats, void *stream);
 /* solve_source of a context through the surrogate (NULL: the FP64 integrator on every cell).
  * hrr / hrr_dot of fcfd_solve_source are not available in surrogate mode. */
 int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s);
-/* Per interior cell of the last surrogate solve_source: 1 accepted, 0 FP64 fallback. */
+/* Per interior cell of the last surrogate solve_source: 2 explicit screen, 1 surrogate, 0 FP64
+ * implicit integrator; *n_fallback = number of cells with 0. */
 int fcfd_source_accepted(fcfd_ctx *c, int *per_cell, int *n_fallback);
 
 /* Plain BF16 GEMM C (FP32) = A B on device arrays with element (m, k) of A at A[m*sam + k*sak]

@@ -6,8 +6,8 @@
   (forward) and 3e-2 per gradient block -- BF16 storage of activations means entries whose FP32 and
   FP64 values straddle a rounding boundary differ by one BF16 ulp (2^-8 relative).
 - Training: a few hundred Adam steps on a fixed regression problem reduce the loss fivefold.
-- Inference: with no cell in the training range every cell takes the FP64 path and the result is
-  bit-identical to the plain integrator; accepted cells satisfy element conservation within the
+- Inference: with no cell in the training range every cell the explicit screen rejects takes the
+  FP64 implicit path and the result is bit-identical to the plain integrator; accepted cells satisfy element conservation within the
   residual bound, sum Y = 1 and energy conservation.
 """
 import numpy as np
@@ -166,24 +166,26 @@ def test_out_of_range_cells_take_the_fp64_path_bitwise(torch, mech):
     acc = torch.zeros(256, dtype=torch.int32, device="cuda")
     print("infer_batch all-fallback, 256 cells", flush=True)
     nf = s.infer_batch(rho, dt, T, Y, accepted=acc)
-    assert nf == 256 and int(acc.sum()) == 0
+    route = acc.cpu().numpy()
+    assert not (route == 1).any() and nf == (route == 0).sum() and nf > 128
     mech.chem_batch_device(rho, dt, T2, Y2)
     assert torch.equal(T, T2) and torch.equal(Y, Y2)
 
 
 def test_accepted_cells_conserve_elements_and_energy(torch, mech, orc):
-    """Accepted cells satisfy the acceptance invariants whatever the network predicts: a network
-    with zero output weights predicts dY = out_mean, dT = out_mean (here zero), so the projection,
-    clipping and energy inversion are exercised on the identity map."""
+    """Accepted cells satisfy the acceptance invariants: a network with zero output weights
+    predicts dY = out_mean, dT = out_mean[53] = 0; out_mean moves O2 and N2 by 1e-6 in opposite
+    directions, which breaks O and N balance, so the projection, the clipping and the energy
+    inversion all act."""
     from fortrancfd_amd.lib import Surrogate
     s = Surrogate(mech)
     flat = M.init_params(2)
     lay = M.unpack(flat)
     lay[4][0][:] = 0
     s.set_params(M.pack(lay))
-    rng = np.random.default_rng(3)
     out_mean = np.zeros(64)
-    out_mean[:M.OUT - 1] = rng.normal(0, 1e-7, M.OUT - 1)   # small composition shift
+    out_mean[mech.names.index("O2")] = 1e-6
+    out_mean[mech.names.index("N2")] = -1e-6
     norm = {"in_mean": np.zeros(64), "in_std": np.ones(64), "lo": np.full(64, -1e300),
             "hi": np.full(64, 1e300), "out_mean": out_mean, "out_std": np.ones(64)}
     s.set_norm(norm)
@@ -194,9 +196,11 @@ def test_accepted_cells_conserve_elements_and_energy(torch, mech, orc):
     acc = torch.zeros(n, dtype=torch.int32, device="cuda")
     print("infer_batch identity network, 512 cells", flush=True)
     nf = s.infer_batch(rho, dt, T, Y, accepted=acc)
-    a = acc.cpu().numpy().astype(bool)
-    assert a.sum() > n // 2 and nf == n - a.sum()
+    route = acc.cpu().numpy()
+    a = route == 1                               # accepted by the surrogate (2: explicit screen)
+    assert a.sum() > n // 4 and nf == (route == 0).sum()
     T1, Y1 = T.cpu().numpy(), Y.cpu().numpy().T.copy()
+    assert np.abs(Y1[a] - Y0[a]).max() > 1e-7          # the state moved
     W = orc.weights()
     ncf = orc.ncf()
     el0 = (Y0 / W) @ ncf
