@@ -41,6 +41,14 @@ def opts(do_hydro=1, do_tran=0, do_reaction=0, select_time=1, select_riemann=2, 
 def lib():
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so.7 / HSA runtime.
+        # Loading torch first lets this library bind to the runtime already in the process
+        # (same soname); loading it first would bring in a second HSA runtime that torch then
+        # cannot initialise.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise FcfdError(f"{LIB_PATH} is missing: build it with `python -m fortrancfd_amd.build`")
         L = C.CDLL(LIB_PATH)
