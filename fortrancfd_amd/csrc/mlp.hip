@@ -27,6 +27,16 @@ __device__ __forceinline__ uint16_t f2bf(float f) {        // round to nearest e
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
+// Box-Cox transform of a mass fraction, lambda = 1/10: b(Y) = 10 (Y^0.1 - 1), b(0) = -10, and its
+// inverse Y = (b / 10 + 1)^10 (0 where the base is not positive).
+__device__ __forceinline__ double boxcox(double y) { return 10.0 * (pow(fmax(y, 0.0), 0.1) - 1.0); }
+__device__ __forceinline__ double boxcox_inv(double b) {
+    const double v = 0.1 * b + 1.0;
+    if (!(v > 0.0)) return 0.0;
+    const double v2 = v * v, v4 = v2 * v2, v8 = v4 * v4;
+    return v8 * v2;
+}
+
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_d(float x) {
     return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
@@ -108,7 +118,7 @@ hipError_t launch_gemm(const GemmArgs &a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Features: x = ((log max(Y,1e-30))_k, T, log(p/1 atm), log dt) z-scored; pad to 64 columns.
+// Features: x = (b(Y_k), T, log(p/1 atm), log dt) z-scored (b: Box-Cox above); pad to 64 columns.
 // in_range: every raw feature inside the training range [lo, hi].
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ void features_of(const SurrogateNorm *__restrict__ nm, const DevThermo *__restrict__ th,
@@ -120,7 +130,7 @@ __device__ __forceinline__ void features_of(const SurrogateNorm *__restrict__ nm
     bool ok = rho > 0.0 && T > 0.0 && dt > 0.0;
     for (int c = 0; c < MLP_INP; ++c) {
         double raw = 0.0;
-        if (c < kk) raw = log(fmax(y[c * stride], 1.0e-30));
+        if (c < kk) raw = boxcox(y[c * stride]);
         else if (c == kk) raw = T;
         else if (c == kk + 1) raw = log(p / 101325.0);
         else if (c == kk + 2) raw = log(dt);
@@ -170,14 +180,15 @@ __global__ void k_features(FeatArgs f) {
     features_of(f.norm, f.thermo, kk, rho, T, dt, y, 1, f.X + long(t) * MLP_INP, f.in_range ? f.in_range + t : nullptr);
 }
 
-// Normalised targets: (Y1 - Y0, T1 - T0) z-scored, columns >= 54 zero.
+// Normalised targets: (b(Y1) - b(Y0), T1 - T0) z-scored, columns >= 54 zero.
 __global__ void k_targets(const SurrogateNorm *nm, int kk, int n, int npad, const double *T0, const double *Y0,
                           const double *T1, const double *Y1, float *Tg) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= npad) return;
     for (int j = 0; j < MLP_OUTP; ++j) {
         double v = 0.0;
-        if (c < n && j < kk) v = (Y1[long(j) * n + c] - Y0[long(j) * n + c] - nm->out_mean[j]) / nm->out_std[j];
+        if (c < n && j < kk)
+            v = (boxcox(Y1[long(j) * n + c]) - boxcox(Y0[long(j) * n + c]) - nm->out_mean[j]) / nm->out_std[j];
         else if (c < n && j == kk) v = (T1[c] - T0[c] - nm->out_mean[j]) / nm->out_std[j];
         Tg[long(c) * MLP_OUTP + j] = float(v);
     }
@@ -305,7 +316,7 @@ __global__ void k_apply(ApplyArgs a) {
     const float *out = a.Yh + long(row) * MLP_OUTP;
     double y1[NSP];
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) y1[k] = y0[k] + (double(out[k]) * nm->out_std[k] + nm->out_mean[k]);
+    for (int k = 0; k < NSP; ++k) y1[k] = boxcox_inv(boxcox(y0[k]) + (double(out[k]) * nm->out_std[k] + nm->out_mean[k]));
     const double Tnet = T0 + (double(out[NSP]) * nm->out_std[NSP] + nm->out_mean[NSP]);
     bool ok = a.in_range[row] != 0;
     // weighted projection onto element conservation (5 x 5 normal equations)

@@ -8,7 +8,7 @@
 
 namespace fcfd {
 
-constexpr int MLP_IN = 56;       // log Y (53), T, log p, log dt
+constexpr int MLP_IN = 56;       // Box-Cox Y (53), T, log p, log dt
 constexpr int MLP_OUT = 54;      // dY (53), dT
 constexpr int MLP_HID = 256;
 constexpr int MLP_LAYERS = 5;

@@ -7,7 +7,8 @@ Training pairs come from the batched-chemistry generator (workloads.chemistry_ba
 2600) K, p ~ log-U(1, 50) atm, dt ~ log-U(1e-8, 1e-6) s, mixing-line compositions) solved by this
 package's FP64 integrator on the GPU.  The network is trained on the GPU by the library's own
 forward / backward / Adam kernels (fcfd_sur_train_step): Adam, learning rate 1e-3 on a cosine
-schedule, batch 8192.  Inputs are z-scored with the training set's mean and standard deviation;
+schedule, batch 8192.  Species enter and leave the network through the Box-Cox transform
+b(Y) = 10 (Y^0.1 - 1); inputs are z-scored with the training set's mean and standard deviation;
 the training range used by the fallback test is the training set's per-feature [min, max]
 widened by 2 % of its width on each side.
 
@@ -29,7 +30,6 @@ from .lib import DATA_DIR, Mechanism, Surrogate
 
 DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "surrogate_gri30.npz")
 RANGE_MARGIN = 0.02
-LOG_Y_FLOOR = 1e-30
 P_ATM = 101325.0
 RU_SI = 8.31451
 
@@ -41,10 +41,15 @@ def _torch():
     return torch
 
 
+def boxcox(Y):
+    """Box-Cox transform of mass fractions, lambda = 1/10 (the network's species coordinate)."""
+    return 10.0 * (np.maximum(Y, 0.0) ** 0.1 - 1.0)
+
+
 def raw_features(mech, rho, dt, T, Y):
-    """(n, 56) raw inputs: log max(Y, 1e-30), T, log(p / 1 atm), log dt (host numpy, FP64)."""
+    """(n, 56) raw inputs: Box-Cox(Y), T, log(p / 1 atm), log dt (host numpy, FP64)."""
     p = rho * RU_SI * T * (Y / mech.weights).sum(axis=1) * 1e3
-    return np.column_stack([np.log(np.maximum(Y, LOG_Y_FLOOR)), T, np.log(p / P_ATM), np.log(dt)])
+    return np.column_stack([boxcox(Y), T, np.log(p / P_ATM), np.log(dt)])
 
 
 def solve_pairs(mech, rho, dt, T, Y, chunk=131072, log=print):
@@ -118,7 +123,7 @@ def train(mech, n_pairs, epochs, batch=8192, lr=1e-3, seed=workloads.SEED + 4, l
     T1, Y1 = solve_pairs(mech, rho, dt, T0, Y0, log=log)
     t_solve = time.time() - t0
     feats = raw_features(mech, rho, dt, T0, Y0)
-    norm = fit_norm(mech, feats, Y1 - Y0, T1 - T0)
+    norm = fit_norm(mech, feats, boxcox(Y1) - boxcox(Y0), T1 - T0)
     s = Surrogate(mech)
     s.set_norm(norm)
     s.set_params(he_uniform(seed))

@@ -141,8 +141,8 @@ int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double 
 /* ---- chemistry surrogate (no reference counterpart: stands in for the macks call of
  * SRC/source.f90:58 where its answer is accepted) ----------------------------------------------
  * A 56 -> 4 x 256 GELU -> 54 multilayer perceptron with BF16 weights and activations and FP32
- * accumulation.  Inputs per cell: log max(Y_k, 1e-30) (KK = 53), T, log(p / 1 atm), log dt,
- * z-scored; outputs (Y1 - Y0, T1 - T0) z-scored.  A cell's answer is accepted when every raw
+ * accumulation.  Inputs per cell: b(Y_k) (KK = 53), T, log(p / 1 atm), log dt, z-scored, with the
+ * Box-Cox transform b(Y) = 10 (Y^0.1 - 1); outputs (b(Y1) - b(Y0), T1 - T0) z-scored.  A cell's answer is accepted when every raw
  * input lies in the training range, the element-conservation residual of the final state (after
  * projection onto element conservation and clipping of negative mass fractions) is at most
  * resid_tol relative, and the network temperature agrees with the energy-consistent temperature

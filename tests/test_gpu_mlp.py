@@ -213,3 +213,40 @@ def test_accepted_cells_conserve_elements_and_energy(torch, mech, orc):
         e0 = orc.specific_energy(T0[c], orc.adjust_sum(orc.ytx(Y0[c])))
         e1 = orc.specific_energy(T1[c], orc.adjust_sum(orc.ytx(Y1[c])))
         assert abs(e1 - e0) <= 1e-10 * abs(e0)
+
+
+def test_features_and_targets_match_host(torch, mech):
+    """Feature rows and normalised targets against the host restatement of the same transforms
+    (Box-Cox species, T, log p/atm, log dt, z-scored, BF16): equal to within one BF16 ulp (the
+    device pow and the host pow may round differently); padding rows and columns are zero."""
+    from fortrancfd_amd import surrogate as S
+    from fortrancfd_amd import workloads
+    from fortrancfd_amd.lib import Surrogate
+    s = Surrogate(mech)
+    n = 1000
+    rho, dt, T, Y = workloads.chemistry_batch(mech, n, seed=31)
+    raw = S.raw_features(mech, rho, dt, T, Y)
+    Y1 = Y.copy()
+    Y1[:, 5] *= 1.1
+    T1 = T + 3.0
+    norm = S.fit_norm(mech, raw, S.boxcox(Y1) - S.boxcox(Y), T1 - T)
+    s.set_norm(norm)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    X = torch.zeros((1024, 64), dtype=torch.int16, device="cuda")
+    rng_ok = torch.zeros(n, dtype=torch.int32, device="cuda")
+    print("features / targets of 1000 cells", flush=True)
+    s.features(d(rho), d(dt), d(T), d(Y.T), X, rng_ok)
+    got = X.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    got = got.astype(np.uint32).view(np.float32).astype(np.float64)
+    ref = M.bf16((raw - norm["in_mean"][:56]) / norm["in_std"][:56])
+    assert np.all(got[n:] == 0) and np.all(got[:, 56:] == 0)
+    ulp = np.abs(ref) * 2.0 ** -7 + 1e-30
+    assert np.all(np.abs(got[:n, :56] - ref) <= ulp)
+    assert rng_ok.cpu().numpy().all()              # every cell lies inside its own range
+    Tg = torch.zeros((1024, 64), dtype=torch.float32, device="cuda")
+    s.targets(d(T), d(Y.T), d(T1), d(Y1.T), Tg)
+    tg = Tg.cpu().numpy().astype(np.float64)
+    out = np.column_stack([S.boxcox(Y1) - S.boxcox(Y), T1 - T])
+    reft = (out - norm["out_mean"][:54]) / norm["out_std"][:54]
+    assert np.all(tg[n:] == 0) and np.all(tg[:, 54:] == 0)
+    np.testing.assert_allclose(tg[:n, :54], reft, rtol=1e-5, atol=1e-5)
