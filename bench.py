@@ -1,11 +1,16 @@
 """Benchmark of the per-timestep cell and interface update on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload coupled65536|inert65536|inert1m]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload coupled65536|inert65536|inert1m|chem1m|surrogate1m]
 
-Prints one JSON line.  The default workload is the 65536-cell GRI-3.0 domain of the coupled run
-(16 blocks x 4096 cells, dx = 25 um, stoichiometric CH4/air at 1 atm and 300 K with three seeded
-32-cell hot spots at 1800-2200 K, WENO5-JS + HLLC + SSP-RK3, CFL 0.5, reflective walls), stepped
-with its hydrodynamics on the GPU.  Synthetic data, FP64 throughout.
+Prints one JSON line.  The default workload is the coupled reacting run: 65536 cells in 16 blocks
+of 4096 (dx = 25 um), stoichiometric CH4/air at 1 atm and 300 K with three seeded 32-cell hot spots
+at 1800-2200 K, GRI-3.0 (53 species, 325 reactions), WENO5-JS + HLLC + SSP-RK3 at CFL 0.5 between
+reflective walls, chemistry split after each fluid step (calc_dt, solve_fluid, solve_source,
+refresh, replayed from a HIP graph).  K steps are timed with the chemistry through the trained
+surrogate + FP64 fallback (`value`), then K steps from the same initial state with the plain FP64
+integrator; the line reports both rates, their ratio and the final-state errors of the surrogate run
+against the FP64 run.  Synthetic data, FP64 state throughout (BF16 / FP32 inside the network).
 """
 from __future__ import annotations
 
@@ -23,6 +28,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TF = 78.6
 SEED = 20261202
 
 
@@ -80,19 +86,262 @@ def reference_cpu_baseline(cells, nblk, ncell, dx, steps, cfl):
 
 
 WORKLOADS = {
-    # name: (nblk, ncell, dx, hot spots)
-    "inert65536": (16, 4096, 25e-6, 3),
-    "inert1m": (256, 4096, 25e-6, 48),
+    # name: (nblk, ncell, dx, hot spots, chemistry)
+    "coupled65536": (16, 4096, 25e-6, 3, True),
+    "inert65536": (16, 4096, 25e-6, 3, False),
+    "inert1m": (256, 4096, 25e-6, 48, False),
 }
+MAJOR = ("CH4", "O2", "N2", "CO2", "H2O", "CO", "H2")
+TIMERS = {0: "k_flux", 1: "k_stage_end", 2: "k_wave_speed+k_dt_finalize", 3: "k_chem (SDIRK)",
+          4: "k_qf_to_qc_interior", 5: "surrogate (features+GEMMs+apply)", 6: "k_chem_screen"}
+
+# GRI-3.0 reaction census (chem.inp: 325 reactions, 29 fall-off of which 26 Troe, 12 plain +M,
+# 309 reversible) and the FP64 operation model of one right-hand side, counting each exp / log /
+# pow as one operation: per species 40 (thermo, S/R-H/RT, concentration), per reaction 14 (rate
+# constant, equilibrium constant, rates of progress, six production-rate updates), per fall-off
+# reaction 30 more, per third-body efficiency 2 (about 8 per third-body reaction).
+def rhs_flops(kk=53, ii=325, nfall=29, ntb=41):
+    return 40 * kk + 14 * ii + 30 * nfall + 2 * 8 * ntb
+
+
+def kernel_table(s):
+    out = {}
+    for i, name in TIMERS.items():
+        ms, n = s.timing(i)
+        if n:
+            out[name] = {"avg_ms": ms / n, "launches": n, "total_ms": ms}
+    return out
+
+
+def fluid_bytes(nequ, ntot, nblk, ncell):
+    """Algorithmic HBM bytes of one k_flux launch: the primitive state once (nequ doubles per cell)
+    in, one flux vector per interface (ncell + 1 per block) out."""
+    return 8.0 * nequ * (ntot + nblk * (ncell + 1))
+
+
+def roofline_for(kern, mech, ntot, nblk, ncell, screen_cells):
+    """Roofline of the kernel with the largest total time in the timed pass."""
+    name, rec = max(kern.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_s = rec["avg_ms"] / 1e3
+    if name == "k_flux":
+        b = fluid_bytes(mech.nequ, ntot, nblk, ncell)
+        a = b / avg_s / 1e9
+        return {"bound": "hbm", "kernel": name, "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": a / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": rec["avg_ms"],
+                "algorithmic_bytes_per_launch": b}
+    if name == "k_stage_end":
+        # RK stage: Q0, Qin and the fluxes in, Qout and the primitives out (all cells and ghosts)
+        b = 8.0 * mech.nequ * (5 * ntot)
+        a = b / avg_s / 1e9
+        return {"bound": "hbm", "kernel": name, "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": a / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": rec["avg_ms"],
+                "algorithmic_bytes_per_launch": b}
+    # chemistry kernels: FP64 vector arithmetic (Heun screen = 2 right-hand sides per cell)
+    f = 2.0 * rhs_flops() * screen_cells
+    a = f / avg_s / 1e12
+    return {"bound": "valu-fp64", "kernel": name, "achieved": a, "peak": FP64_PEAK_TF,
+            "unit": "TFLOP/s", "frac": a / FP64_PEAK_TF, "traffic": None, "avg_launch_ms": rec["avg_ms"],
+            "algorithmic_flops_per_launch": f}
+
+
+def oracle_coupled_baseline(cells, nblk, ncell, dx, cfl, steps):
+    """The CPU restatement (oracle, gcc -O2, one core) on `steps` coupled steps of the same domain:
+    calc_dt, solve_fluid, solve_source (no heat-release diagnostics), refresh."""
+    import ctypes as C
+    from oracle.pyoracle import Domain, Oracle
+    orc = Oracle()
+    d = Domain(orc, cells, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2,
+               select_time=1, do_reaction=1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt = d.calc_dt(cfl)
+        d.solve_fluid(dt)
+        orc.lib.fo_solve_source(d.ptr, C.c_double(dt), C.c_double(1e-5), C.c_double(1e-13), None, None,
+                                None)
+        d.refresh()
+    return time.perf_counter() - t0
+
+
+def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
+    import torch
+    from fortrancfd_amd.lib import Solver
+    from fortrancfd_amd.surrogate import load_surrogate
+    ntot = nblk * ncell
+    cells = ch4_air_cells(names, ntot, nhot, SEED)
+    cfl = 0.5
+    sur = load_surrogate(mech)
+
+    def run(with_sur):
+        s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1,
+                   do_reaction=1)
+        s.init_cells(cells)
+        if with_sur:
+            s.set_surrogate(sur)
+        t = s.advance(cfl, args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t = s.advance(cfl, args.steps, t)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        return s, el, t
+
+    s_sur, el_sur, t_sur = run(True)
+    _, qf_sur = s_sur.download()
+    route, nfb = s_sur.source_accepted()
+    s_fp, el_fp, t_fp = run(False)
+    _, qf_fp = s_fp.download()
+    lv = s_sur.lv
+    inner = (slice(None), slice(lv, lv + ncell))
+    Ts, Tf = qf_sur[inner][..., 2], qf_fp[inner][..., 2]
+    maj = [3 + names.index(k) for k in MAJOR]
+    Ys, Yf = qf_sur[inner][..., maj], qf_fp[inner][..., maj]
+    t_err = float(np.max(np.abs(Ts - Tf) / Tf))
+    y_err = float(np.max(np.abs(Ys - Yf)))
+
+    # per-kernel times: a further pass of the surrogate run with HIP events around every launch
+    kt = min(args.steps, 200)
+    s_sur.set_timing(True)
+    s_sur.advance(cfl, kt, t_sur)
+    kern = kernel_table(s_sur)
+    s_sur.set_timing(False)
+    screen_cells = ntot
+    roof = roofline_for(kern, mech, ntot, nblk, ncell, screen_cells)
+
+    cpu = None
+    if not args.no_cpu:
+        secs = oracle_coupled_baseline(cells, nblk, ncell, dx, cfl, args.cpu_steps)
+        cpu = {"value": ntot * args.cpu_steps / secs, "unit": "cell-steps/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} coupled step(s) of the same {ntot}-cell domain by the C "
+                         f"restatement (gcc -O2, one core; the reference ships no chemistry "
+                         f"integrator): {secs:.2f} s"}
+    value = ntot * args.steps / el_sur
+    fp64_value = ntot * args.steps / el_fp
+    return {
+        "metric": "cell-steps/s, coupled 65536-cell GRI-3.0 reacting flow, surrogate + FP64 "
+                  "fallback (vs plain FP64 integrator, at stated error)",
+        "value": value,
+        "unit": "cell-steps/s",
+        "ms_per_step": el_sur / args.steps * 1e3,
+        "fp64": {"value": fp64_value, "ms_per_step": el_fp / args.steps * 1e3, "sim_time_s": t_fp},
+        "speedup_vs_fp64": value / fp64_value,
+        "error_vs_fp64": {"T_rel_max": t_err, "major_species_abs_max": y_err,
+                          "major_species": list(MAJOR), "after_steps": args.warmup + args.steps},
+        "chemistry_routes_last_step": {"explicit_screen": int((route == 2).sum()),
+                                       "surrogate": int((route == 1).sum()),
+                                       "fp64_implicit": int((route == 0).sum())},
+        "sim_time_s": t_sur,
+        "config": {"workload": f"coupled65536: {nblk} blocks x {ncell} cells, dx {dx} m, GRI-3.0 "
+                               f"CH4/air 1 atm 300 K, {nhot} hot spots 1800-2200 K, WENO5-JS+HLLC+"
+                               f"SSP-RK3, CFL {cfl}, walls, split chemistry (atol 1e-13, rtol 1e-5)",
+                   "cells": ntot, "species": mech.kk, "reactions": mech.ii,
+                   "surrogate": "56-4x256 GELU-54 BF16/FP32 MLP, "
+                                f"{sur.meta.get('pairs', '?')} training pairs"},
+        "kernels": kern,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_inert(args, mech, nblk, ncell, dx, nhot, names):
+    import torch
+    from fortrancfd_amd.lib import Solver
+    ntot = nblk * ncell
+    cells = ch4_air_cells(names, ntot, nhot, SEED)
+    s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1)
+    s.init_cells(cells)
+    cfl = 0.5
+    t = s.advance(cfl, args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t = s.advance(cfl, args.steps, t)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s.set_timing(True)
+    s.advance(cfl, min(args.steps, 200), t)
+    kern = kernel_table(s)
+    s.set_timing(False)
+    roof = roofline_for({k: v for k, v in kern.items() if k == "k_flux"}, mech, ntot, nblk, ncell, 0)
+    cpu = None
+    if not args.no_cpu:
+        secs = reference_cpu_baseline(cells, nblk, ncell, dx, args.cpu_steps, cfl)
+        if secs is not None:
+            cpu = {"value": ntot * args.cpu_steps / secs, "unit": "cell-steps/s", "cores": 1,
+                   "kind": "reference",
+                   "sample": f"{args.cpu_steps} steps of the same {ntot}-cell domain on the "
+                             f"reference Fortran (amdflang -O2, one core): {secs:.2f} s"}
+    return {
+        "metric": "cell-steps/s (GRI-3.0 WENO5+HLLC+SSP-RK3 hydrodynamics, no chemistry)",
+        "value": ntot * args.steps / elapsed,
+        "unit": "cell-steps/s",
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "sim_time_s": t,
+        "config": {"workload": f"{args.workload}: {nblk} blocks x {ncell} cells, GRI-3.0 CH4/air "
+                               f"1 atm 300 K, {nhot} hot spots, WENO5-JS+HLLC+SSP-RK3, CFL {cfl}, "
+                               f"walls", "cells": ntot, "species": mech.kk},
+        "kernels": kern,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_chem(args, mech, surrogate):
+    """Batched chemistry on 2^20 independent cells (T ~ U(1000, 2600) K, p ~ log-U(1, 50) atm,
+    dt ~ log-U(1e-8, 1e-6) s): FP64 integrator, or surrogate + fallback with errors vs FP64."""
+    import torch
+    from fortrancfd_amd import workloads
+    from fortrancfd_amd.surrogate import load_surrogate
+    n = args.cells
+    seed = workloads.SEED + (5 if surrogate else 3)
+    rho, dt, T, Y = workloads.chemistry_batch(mech, n, seed=seed)
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    tr, tdt = d(rho), d(dt)
+    Tf, Yf = d(T), d(Y.T)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mech.chem_batch_device(tr, tdt, Tf, Yf)
+    torch.cuda.synchronize()
+    el_fp = time.perf_counter() - t0
+    line = {"unit": "chemistry updates/s", "cells": n}
+    if surrogate:
+        sur = load_surrogate(mech)
+        Ts, Ys = d(T), d(Y.T)
+        route = torch.zeros(n, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nf = sur.infer_batch(tr, tdt, Ts, Ys, accepted=route)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        r = route.cpu().numpy()
+        maj = [names_index(mech, k) for k in MAJOR]
+        terr = (torch.abs(Ts - Tf) / Tf).max().item()
+        yerr = torch.abs(Ys - Yf)[maj].max().item()
+        line.update({"metric": "chemistry updates/s, surrogate + FP64 fallback, 2^20 fresh cells",
+                     "value": n / el, "fp64_value": n / el_fp, "speedup_vs_fp64": el_fp / el,
+                     "routes": {"explicit_screen": int((r == 2).sum()), "surrogate": int((r == 1).sum()),
+                                "fp64_implicit": int(nf)},
+                     "error_vs_fp64": {"T_rel_max": terr, "major_species_abs_max": yerr}})
+    else:
+        line.update({"metric": "chemistry updates/s, FP64 implicit integrator, 2^20 cells",
+                     "value": n / el_fp})
+    line["config"] = {"workload": f"{'surrogate1m' if surrogate else 'chem1m'}: {n} independent "
+                                  "GRI-3.0 cells, T 1000-2600 K, p 1-50 atm, dt 1e-8-1e-6 s"}
+    return line
+
+
+def names_index(mech, k):
+    return mech.names.index(k)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="inert65536", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="coupled65536",
+                    choices=sorted(WORKLOADS) + ["chem1m", "surrogate1m"])
+    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -103,73 +352,24 @@ def main():
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
-    from fortrancfd_amd.lib import GRI30, Mechanism, Solver
+    from fortrancfd_amd.lib import GRI30, Mechanism
 
-    nblk, ncell, dx, nhot = WORKLOADS[args.workload]
-    ntot = nblk * ncell
-    names = species_names(GRI30)
-    cells = ch4_air_cells(names, ntot, nhot, SEED)
     mech = Mechanism()
-    s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1)
-    s.init_cells(cells)
-    cfl = 0.5
-    t = s.advance(cfl, args.warmup)
-
-    s.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    t = s.advance(cfl, args.steps, t)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    flux_ms, flux_n = s.timing(0)
-    stage_ms, stage_n = s.timing(1)
-    dt_ms, dt_n = s.timing(2)
-    s.set_timing(False)
-
-    cell_steps = ntot * args.steps
-    value = cell_steps / elapsed
-    nequ = mech.nequ
-    # flux kernel: reads the primitive state once (nequ doubles per cell) and writes one flux
-    # vector per interface (nequ doubles); ncell+1 interfaces per block
-    flux_bytes = 8.0 * nequ * (ntot + nblk * (ncell + 1))
-    flux_avg_s = flux_ms / flux_n / 1e3
-    achieved = flux_bytes / flux_avg_s / 1e9
-
-    cpu = None
-    if not args.no_cpu:
-        csteps = args.cpu_steps
-        secs = reference_cpu_baseline(cells, nblk, ncell, dx, csteps, cfl)
-        if secs is not None:
-            cpu = {"value": ntot * csteps / secs, "unit": "cell-steps/s", "cores": 1,
-                   "kind": "reference",
-                   "sample": f"{csteps} steps of the same {ntot}-cell domain on the reference "
-                             f"Fortran (amdflang -O2, one core): {secs:.2f} s"}
-
-    line = {
-        "metric": "cell-steps/s (GRI-3.0 WENO5+HLLC+SSP-RK3 hydrodynamics)",
-        "value": value,
-        "unit": "cell-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {nblk} blocks x {ncell} cells, GRI-3.0 "
-                               f"CH4/air 1 atm 300 K, {nhot} hot spots, WENO5-JS+HLLC+SSP-RK3, "
-                               f"CFL {cfl}, walls", "cells": ntot, "species": mech.kk},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_flux<53,WENO5JS>", "avg_launch_ms": flux_avg_s * 1e3,
-                     "stage_end_avg_ms": stage_ms / max(stage_n, 1),
-                     "cfl_avg_ms": dt_ms / max(dt_n, 1)},
-        "cpu_baseline": cpu,
-        "sim_time_s": t,
-    }
-    print(json.dumps(line), flush=True)
+    names = species_names(GRI30)
+    if args.workload in ("chem1m", "surrogate1m"):
+        line = bench_chem(args, mech, args.workload == "surrogate1m")
+        steps = 1
+    else:
+        nblk, ncell, dx, nhot, react = WORKLOADS[args.workload]
+        fn = bench_coupled if react else bench_inert
+        line = fn(args, mech, nblk, ncell, dx, nhot, names)
+        steps = args.steps
+    out = {"metric": line.pop("metric"), "value": line.pop("value"), "unit": line.pop("unit"),
+           "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": line.pop("ms_per_step", None), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    out.update(line)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

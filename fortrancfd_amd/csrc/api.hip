@@ -10,6 +10,7 @@
 #include "fluid.hpp"
 #include "chem.hpp"
 #include "surrogate_impl.hpp"
+#include "tran.hpp"
 
 namespace fcfd {
 
@@ -31,7 +32,7 @@ using namespace fcfd;
 
 namespace {
 
-constexpr int NTIMERS = 7;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry SDIRK, 4 source Qc refresh, 5 surrogate, 6 explicit screen
+constexpr int NTIMERS = 8;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry SDIRK, 4 source Qc refresh, 5 surrogate, 6 explicit screen, 7 transport flux
 
 int lvir_of(const fcfd_opts &o) {                                  // SRC/params.f90:161-173
     if (o.flag_acu == 0) return 1;
@@ -49,7 +50,6 @@ int check_opts(const fcfd_opts &o, int kk) {
     if (o.flag_acu < 0 || o.flag_acu > 2) { set_error("flag_acu must be 0, 1 or 2"); return FCFD_E_ARG; }
     if (o.flag_acu == 1 && (o.flag_muscl < 1 || o.flag_muscl > 4)) { set_error("flag_muscl must be 1..4"); return FCFD_E_ARG; }
     if (o.flag_acu == 2 && (o.flag_weno < 1 || o.flag_weno > 4)) { set_error("flag_weno must be 1..4"); return FCFD_E_ARG; }
-    if (o.do_tran) { set_error("do_tran = 1 is not available in this build"); return FCFD_E_UNSUPPORTED; }
     if (o.do_reaction && !(o.rtol > 0.0 && o.atol > 0.0)) { set_error("do_reaction needs atol > 0 and rtol > 0"); return FCFD_E_ARG; }
     bool ok = false;
 #define FCFD_CASE(N) ok = ok || (kk == N);
@@ -92,6 +92,10 @@ struct fcfd_ctx {
     int *d_accepted = nullptr;
     int *d_list = nullptr;               // cells the explicit screen sends to the implicit integrator
     unsigned *d_list_n = nullptr;
+    // transport
+    DevTran *d_tran = nullptr;
+    double *props = nullptr;
+    long PP = 0;
 };
 
 namespace fcfd {
@@ -183,23 +187,37 @@ StageArgs stage_args(fcfd_ctx *c, int mode, const double *Q0, const double *Qin,
 // One solve_fluid on the stream; dt from the host (dt_dev == nullptr) or from device memory.
 hipError_t enqueue_fluid(fcfd_ctx *c, double dt, const double *dt_dev) {
     const int nsp = c->kk;
-    const FluxArgs fa = flux_args(c, false);
-    auto flux = [&]() { return timed(c, 0, [&]() { return launch_flux(fa, nsp, c->o.flag_acu, c->o.flag_weno, c->stream); }); };
+    FluxArgs fa = flux_args(c, false);
+    fa.accumulate = c->o.do_tran == 1 ? 1 : 0;
+    // Fplus of one stage (SRC/timeintegration.f90:138-146): transport from the conserved state
+    // Qin, then advection from the primitives on top
+    auto flux = [&](const double *Qin) {
+        if (c->o.do_tran == 1) {
+            TranArgs ta{};
+            ta.thermo = c->thermo; ta.tran = c->d_tran; ta.Qc = Qin; ta.props = c->props; ta.F = c->F;
+            ta.nblk = c->nblk; ta.ncell = c->ncell; ta.lv = c->lv; ta.S = c->S; ta.P = c->P; ta.PF = c->PF;
+            ta.PP = c->PP; ta.dx = c->dx;
+            hipError_t e2 = timed(c, 7, [&]() { return launch_flux_tran(ta, nsp, c->stream); });
+            if (e2 != hipSuccess) return e2;
+        }
+        if (c->o.do_hydro != 1) return hipSuccess;
+        return timed(c, 0, [&]() { return launch_flux(fa, nsp, c->o.flag_acu, c->o.flag_weno, c->stream); });
+    };
     auto stage = [&](const StageArgs &s) { return timed(c, 1, [&]() { return launch_stage_end(s, nsp, c->stream); }); };
     hipError_t e;
     const int a = c->cur, b = (c->cur + 1) % 3, d = (c->cur + 2) % 3;
     if (c->o.select_time == 0) {                                   // explicit_euler
-        if ((e = flux()) != hipSuccess) return e;
+        if ((e = flux(c->Q[a])) != hipSuccess) return e;
         if ((e = stage(stage_args(c, 0, nullptr, c->Q[a], c->Q[b], dt, dt_dev, 1.0, 1.0))) != hipSuccess) return e;
         c->cur = b;
         return hipSuccess;
     }
     // TVDRK3 (SRC/timeintegration.f90:132-217)
-    if ((e = flux()) != hipSuccess) return e;
+    if ((e = flux(c->Q[a])) != hipSuccess) return e;
     if ((e = stage(stage_args(c, 0, nullptr, c->Q[a], c->Q[b], dt, dt_dev, 1.0, 1.0))) != hipSuccess) return e;
-    if ((e = flux()) != hipSuccess) return e;
+    if ((e = flux(c->Q[b])) != hipSuccess) return e;
     if ((e = stage(stage_args(c, 1, c->Q[a], c->Q[b], c->Q[d], dt, dt_dev, 3.0 / 4.0, 1.0 / 4.0))) != hipSuccess) return e;
-    if ((e = flux()) != hipSuccess) return e;
+    if ((e = flux(c->Q[d])) != hipSuccess) return e;
     if ((e = stage(stage_args(c, 1, c->Q[a], c->Q[d], c->Q[b], dt, dt_dev, 1.0 / 3.0, 2.0 / 3.0))) != hipSuccess) return e;
     c->cur = b;
     return hipSuccess;
@@ -350,6 +368,17 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
     HIP_TRY(hipMalloc(&c->d_cvel, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->d_cvel, 0, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->d_dt, 0, sizeof(double)));
+    if (o->do_tran) {
+        if (!m->host.has_tran) { set_error("do_tran = 1 needs the transport link file (tran.bin)"); return FCFD_E_ARG; }
+        if (m->host.no > 4) { set_error("transport fits of order above 4 are not supported"); return FCFD_E_UNSUPPORTED; }
+        auto t = std::make_unique<DevTran>();
+        std::memset(t.get(), 0, sizeof(DevTran));
+        build_dev_tran(m->host, *t);
+        HIP_TRY(hipMalloc(&c->d_tran, sizeof(DevTran)));
+        HIP_TRY(hipMemcpy(c->d_tran, t.get(), sizeof(DevTran), hipMemcpyHostToDevice));
+        c->PP = round_up(long(nblk) * (ncell + 2), 64);
+        HIP_TRY(hipMalloc(&c->props, sizeof(double) * size_t(5 + 2 * c->kk) * c->PP));
+    }
     if (o->do_reaction) {
         if (int rc2 = ensure_dev_tables(m)) return rc2;
         const size_t nint = size_t(nblk) * ncell;
@@ -376,6 +405,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
     (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_list_n);
+    (void)hipFree(c->d_tran); (void)hipFree(c->props);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -451,7 +481,7 @@ int fcfd_calc_dt(fcfd_ctx *c, double cfl, double *dt) {
 
 int fcfd_solve_fluid(fcfd_ctx *c, double dt) {
     if (!c) { set_error("fcfd_solve_fluid: invalid argument"); return FCFD_E_ARG; }
-    if (c->o.do_hydro != 1) return FCFD_OK;
+    if (c->o.do_hydro != 1 && c->o.do_tran != 1) return FCFD_OK;
     HIP_TRY(enqueue_fluid(c, dt, nullptr));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(collect_timing(c));
@@ -475,10 +505,11 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
     // With do_reaction the step is the driver's Lie splitting (SRC/main.f90:134-163):
     // calc_dt, solve_fluid, solve_source, refresh.
     const bool refresh = c->o.do_reaction == 1;
+    const bool fluid = c->o.do_hydro == 1 || c->o.do_tran == 1;
     if (c->timing) {
         for (int s = 0; s < nsteps; ++s) {
             HIP_TRY(enqueue_dt(c, cfl, c->d_time));
-            if (c->o.do_hydro == 1) HIP_TRY(enqueue_fluid(c, 0.0, c->d_dt));
+            if (fluid) HIP_TRY(enqueue_fluid(c, 0.0, c->d_dt));
             if (refresh) HIP_TRY(enqueue_source(c, 0.0, c->d_dt, false));
             if (refresh) HIP_TRY(enqueue_refresh(c));
         }
@@ -491,7 +522,7 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
                 hipGraph_t g;
                 HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 hipError_t e = enqueue_dt(c, cfl, c->d_time);
-                if (e == hipSuccess && c->o.do_hydro == 1) e = enqueue_fluid(c, 0.0, c->d_dt);
+                if (e == hipSuccess && fluid) e = enqueue_fluid(c, 0.0, c->d_dt);
                 if (e == hipSuccess && refresh) e = enqueue_source(c, 0.0, c->d_dt, false);
                 if (e == hipSuccess && refresh) e = enqueue_refresh(c);
                 hipError_t e2 = hipStreamEndCapture(c->stream, &g);
@@ -502,7 +533,7 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
                 HIP_TRY(hipGraphDestroy(g));
             }
             HIP_TRY(hipGraphLaunch(c->graph[cur], c->stream));
-            c->cur = c->o.do_hydro == 1 ? (cur + 1) % 3 : cur;
+            c->cur = fluid ? (cur + 1) % 3 : cur;
         }
     }
     if (time) HIP_TRY(hipMemcpyAsync(time, c->d_time, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -632,6 +663,38 @@ void untranspose(const std::vector<double> &in, long rows, int cols, double *out
 }  // namespace
 
 extern "C" {
+
+int fcfd_flux_tran_host(const fcfd_mech *m, int n, int lv, double dx, const double *Qc, double *Fplus) {
+    if (!m || !Qc || !Fplus || n < 1 || lv < 1 || !(dx > 0.0)) { set_error("fcfd_flux_tran_host: invalid argument"); return FCFD_E_ARG; }
+    if (!m->host.has_tran) { set_error("fcfd_flux_tran_host: the mechanism was loaded without tran.bin"); return FCFD_E_ARG; }
+    if (m->host.no > 4) { set_error("transport fits of order above 4 are not supported"); return FCFD_E_UNSUPPORTED; }
+    if (int rc = check_opts(fcfd_opts{1, 1, 0, 1, 2, 2, 0, 2, 0, 0, 0, 1e-13, 1e-5}, m->host.kk)) return rc;
+    const int ne = 3 + m->host.kk;
+    const long cols = n + 2L * lv, nif = n + 1;
+    DevBuf th, dt, dq, dF, dp;
+    HIP_TRY(hipMalloc(&th.p, sizeof(DevThermo)));
+    HIP_TRY(hipMemcpy(th.p, &m->host.thermo, sizeof(DevThermo), hipMemcpyHostToDevice));
+    auto t = std::make_unique<DevTran>();
+    std::memset(t.get(), 0, sizeof(DevTran));
+    build_dev_tran(m->host, *t);
+    HIP_TRY(hipMalloc(&dt.p, sizeof(DevTran)));
+    HIP_TRY(hipMemcpy(dt.p, t.get(), sizeof(DevTran), hipMemcpyHostToDevice));
+    auto hq = transpose(Qc, cols, ne);
+    HIP_TRY(hipMalloc(&dq.p, sizeof(double) * hq.size()));
+    HIP_TRY(hipMemcpy(dq.p, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&dF.p, sizeof(double) * ne * nif));
+    HIP_TRY(hipMalloc(&dp.p, sizeof(double) * (5 + 2 * m->host.kk) * (n + 2)));
+    TranArgs a{};
+    a.thermo = static_cast<DevThermo *>(th.p); a.tran = static_cast<DevTran *>(dt.p);
+    a.Qc = static_cast<double *>(dq.p); a.props = static_cast<double *>(dp.p); a.F = static_cast<double *>(dF.p);
+    a.nblk = 1; a.ncell = n; a.lv = lv; a.S = int(cols); a.P = cols; a.PF = nif; a.PP = n + 2; a.dx = dx;
+    HIP_TRY(launch_flux_tran(a, m->host.kk, nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    std::vector<double> hF(size_t(ne) * nif);
+    HIP_TRY(hipMemcpy(hF.data(), dF.p, sizeof(double) * hF.size(), hipMemcpyDeviceToHost));
+    untranspose(hF, nif, ne, Fplus);
+    return FCFD_OK;
+}
 
 int fcfd_flux_advection_host(const fcfd_mech *m, const fcfd_opts *o, int n, int lv, double dx,
                              const double *qf, double *Fplus, double *godunov) {

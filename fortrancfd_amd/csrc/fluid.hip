@@ -79,19 +79,22 @@ __global__ void __launch_bounds__(64) k_flux(FluxArgs a) {
     const double fl2 = (L.E + L.p) * L.u, fr2 = (R.E + R.p) * R.u;
     double *__restrict__ F = a.F + g;
     const long PF = a.PF;
+    // Fplus accumulates (SRC/flux.f90:358): on top of the transport flux when there is one
+    const bool acc = a.accumulate != 0;
+    auto put = [acc](double &d, double v) { d = acc ? d + v : v; };
 
     if (a.riemann == 0) {                                         // LxF (SRC/flux.f90:291-294)
         const double lL = fmax(fmax(fabs(L.u - L.a), fabs(L.u)), fabs(L.u + L.a));
         const double lR = fmax(fmax(fabs(R.u - R.a), fabs(R.u)), fabs(R.u + R.a));
         const double al = fmax(lL, lR);
-        F[0] = 0.5 * (fl0 + fr0 - al * (R.rho - L.rho));
-        F[PF] = 0.5 * (fl1 + fr1 - al * (R.rho * R.u - L.rho * L.u));
-        F[2 * PF] = 0.5 * (fl2 + fr2 - al * (R.E - L.E));
+        put(F[0], 0.5 * (fl0 + fr0 - al * (R.rho - L.rho)));
+        put(F[PF], 0.5 * (fl1 + fr1 - al * (R.rho * R.u - L.rho * L.u)));
+        put(F[2 * PF], 0.5 * (fl2 + fr2 - al * (R.E - L.E)));
 #pragma unroll
         for (int k = 0; k < NSP; ++k) {
             const double QL = L.rho * (fmax(yl[k], 0.0) / L.sy), QR = R.rho * (fmax(yr[k], 0.0) / R.sy);
             const double fL = L.rho * yl[k] * L.u, fR = R.rho * yr[k] * R.u;
-            F[(3 + k) * PF] = 0.5 * (fL + fR - al * (QR - QL));
+            put(F[(3 + k) * PF], 0.5 * (fL + fR - al * (QR - QL)));
         }
     } else {
         const double rt = sqrt(R.rho / L.rho);
@@ -103,13 +106,13 @@ __global__ void __launch_bounds__(64) k_flux(FluxArgs a) {
             auto hll = [&](double fL, double fR, double QL, double QR) {
                 return br == 0 ? fL : (br == 1 ? (sR * fL - sL * fR + sL * sR * (QR - QL)) / (sR - sL) : fR);
             };
-            F[0] = hll(fl0, fr0, L.rho, R.rho);
-            F[PF] = hll(fl1, fr1, L.rho * L.u, R.rho * R.u);
-            F[2 * PF] = hll(fl2, fr2, L.E, R.E);
+            put(F[0], hll(fl0, fr0, L.rho, R.rho));
+            put(F[PF], hll(fl1, fr1, L.rho * L.u, R.rho * R.u));
+            put(F[2 * PF], hll(fl2, fr2, L.E, R.E));
 #pragma unroll
             for (int k = 0; k < NSP; ++k) {
                 const double QL = L.rho * (fmax(yl[k], 0.0) / L.sy), QR = R.rho * (fmax(yr[k], 0.0) / R.sy);
-                F[(3 + k) * PF] = hll(L.rho * yl[k] * L.u, R.rho * yr[k] * R.u, QL, QR);
+                put(F[(3 + k) * PF], hll(L.rho * yl[k] * L.u, R.rho * yr[k] * R.u, QL, QR));
             }
         } else {                                                  // HLLC (SRC/flux.f90:321-354)
             const double sM = (L.p - R.p + R.rho * R.u * (sR - R.u) - L.rho * L.u * (sL - L.u)) /
@@ -120,24 +123,24 @@ __global__ void __launch_bounds__(64) k_flux(FluxArgs a) {
             const Side S = left ? L : R;
             const double f0 = left ? fl0 : fr0, f1 = left ? fl1 : fr1, f2 = left ? fl2 : fr2;
             if (br == 0 || br == 3) {                             // supersonic: upwind flux
-                F[0] = f0;
-                F[PF] = f1;
-                F[2 * PF] = f2;
+                put(F[0], f0);
+                put(F[PF], f1);
+                put(F[2 * PF], f2);
 #pragma unroll
-                for (int k = 0; k < NSP; ++k) F[(3 + k) * PF] = S.rho * (left ? yl[k] : yr[k]) * S.u;
+                for (int k = 0; k < NSP; ++k) put(F[(3 + k) * PF], S.rho * (left ? yl[k] : yr[k]) * S.u);
             } else {                                              // star state of the upwind side
                 const double sK = left ? sL : sR;
                 const double c = S.rho * (sK - S.u) / (sK - sM);
                 const double v2 = S.E / S.rho + (sM - S.u) * (sM + S.p / (S.rho * (sK - S.u)));
-                F[0] = f0 + sK * (c * 1.0 - S.rho);
-                F[PF] = f1 + sK * (c * sM - S.rho * S.u);
-                F[2 * PF] = f2 + sK * (c * v2 - S.E);
+                put(F[0], f0 + sK * (c * 1.0 - S.rho));
+                put(F[PF], f1 + sK * (c * sM - S.rho * S.u));
+                put(F[2 * PF], f2 + sK * (c * v2 - S.E));
 #pragma unroll
                 for (int k = 0; k < NSP; ++k) {
                     const double yk = left ? yl[k] : yr[k];
                     const double Qk = S.rho * (fmax(yk, 0.0) / S.sy);
                     const double fk = S.rho * yk * S.u;
-                    F[(3 + k) * PF] = fk + sK * (c * (Qk / S.rho) - Qk);
+                    put(F[(3 + k) * PF], fk + sK * (c * (Qk / S.rho) - Qk));
                 }
             }
         }

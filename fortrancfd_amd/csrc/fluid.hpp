@@ -13,6 +13,7 @@ struct FluxArgs {
     int nblk, ncell, lv, S;
     long P, PF;
     int muscl, riemann;
+    int accumulate;       // 1: F += advection flux (transport flux already in F), 0: F = flux
 };
 
 struct StageArgs {

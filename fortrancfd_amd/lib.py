@@ -115,6 +115,15 @@ class Mechanism:
                                               C.c_double(dx), _p(qf), _p(F), _p(G)))
         return F, G
 
+    def calc_flux_tran(self, Qc, n, lv, dx):
+        """calc_flux_tran (SRC/flux.f90:18-127) on one block: conserved rows (n+2lv, nequ) ->
+        Fplus (n+1, nequ) of the transport fluxes."""
+        Qc = _f64(Qc)
+        F = np.zeros((n + 1, self.nequ))
+        _check(lib().fcfd_flux_tran_host(self.handle, C.c_int(n), C.c_int(lv), C.c_double(dx),
+                                         _p(Qc), _p(F)))
+        return F
+
     def qfToQc(self, qf):
         qf = _f64(np.atleast_2d(qf))
         out = np.zeros_like(qf)

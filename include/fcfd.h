@@ -35,7 +35,7 @@ typedef struct fcfd_ctx fcfd_ctx;
  * (SRC/params.f90:106-111) and &problem/&boundary groups. */
 typedef struct fcfd_opts {
     int do_hydro;        /* 1: advection fluxes                                   */
-    int do_tran;         /* 1: viscous/heat/diffusive fluxes                      */
+    int do_tran;         /* 1: viscous/heat/diffusive fluxes (needs tran.bin)     */
     int do_reaction;     /* 1: chemistry source                                   */
     int select_time;     /* 0 explicit Euler, 1 TVD-RK3                           */
     int select_riemann;  /* 0 LxF, 1 HLL, 2 HLLC                                  */
@@ -108,8 +108,11 @@ int fcfd_refresh(fcfd_ctx *c);
  * without output. */
 int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time);
 
-/* Kernel timing with HIP events on the context's stream (0 = off).  Kernel ids: 0 flux,
- * 1 stage update + primitive recovery, 2 CFL reduction. */
+/* Kernel timing with HIP events on the context's stream (0 = off; while on, steps are launched
+ * one kernel at a time instead of from the captured graph).  Kernel ids: 0 advection flux,
+ * 1 stage update + primitive recovery, 2 CFL reduction, 3 chemistry implicit integrator,
+ * 4 interior qfToQc after the source, 5 surrogate (features, network, acceptance), 6 chemistry
+ * explicit screen, 7 transport flux. */
 int fcfd_set_timing(fcfd_ctx *c, int enable);
 int fcfd_get_timing(fcfd_ctx *c, int kernel_id, double *ms_total, long long *launches);
 
@@ -119,6 +122,13 @@ int fcfd_get_timing(fcfd_ctx *c, int kernel_id, double *ms_total, long long *lau
  * godunov_state (n+1) x 3 (overwritten, not accumulated). */
 int fcfd_flux_advection_host(const fcfd_mech *m, const fcfd_opts *o, int n, int lv, double dx,
                              const double *qf, double *Fplus, double *godunov);
+/* calc_flux_tran (SRC/flux.f90:18-127; called at SRC/timeintegration.f90:140) on one block:
+ * Qc (n + 2*lv) cells x nequ (conserved, ghosts included) -> Fplus (n+1) x nequ, the viscous,
+ * heat-conduction and diffusive fluxes at the n+1 interfaces (overwritten).  Needs a mechanism
+ * loaded with its transport link file.  The viscosity is TRANLIB's Wilke mixture rule (MCAVIS):
+ * the reference calls MCAVIS_EA there (SRC/trans.f90:148), which it does not ship. */
+int fcfd_flux_tran_host(const fcfd_mech *m, int n, int lv, double dx, const double *Qc,
+                        double *Fplus);
 /* qfToQc / QcToqf (SRC/eos.f90:11,49) on n cells. */
 int fcfd_qf_to_qc_host(const fcfd_mech *m, int n, const double *qf, double *Qc);
 int fcfd_qc_to_qf_host(const fcfd_mech *m, int n, const double *Qc, double *qf);
