@@ -45,7 +45,7 @@ int lvir_of(const fcfd_opts &o) {                                  // SRC/params
 int check_opts(const fcfd_opts &o, int kk) {
     if (o.is_spherical != 0) { set_error("is_spherical = 1 is not supported (Cartesian only)"); return FCFD_E_UNSUPPORTED; }
     if (o.flag_lbound != 0 || o.flag_rbound != 0) { set_error("only reflective walls (flag_lbound = flag_rbound = 0) are supported"); return FCFD_E_UNSUPPORTED; }
-    if (o.select_time != 0 && o.select_time != 1) { set_error("select_time must be 0 (Euler) or 1 (TVD-RK3)"); return FCFD_E_UNSUPPORTED; }
+    if (o.select_time < 0 || o.select_time > 2) { set_error("select_time must be 0 (Euler), 1 (TVD-RK3) or 2 (RK4)"); return FCFD_E_UNSUPPORTED; }
     if (o.select_riemann < 0 || o.select_riemann > 2) { set_error("select_riemann must be 0, 1 or 2"); return FCFD_E_ARG; }
     if (o.flag_acu < 0 || o.flag_acu > 2) { set_error("flag_acu must be 0, 1 or 2"); return FCFD_E_ARG; }
     if (o.flag_acu == 1 && (o.flag_muscl < 1 || o.flag_muscl > 4)) { set_error("flag_muscl must be 1..4"); return FCFD_E_ARG; }
@@ -92,6 +92,8 @@ struct fcfd_ctx {
     int *d_accepted = nullptr;
     int *d_list = nullptr;               // cells the explicit screen sends to the implicit integrator
     unsigned *d_list_n = nullptr;
+    double *acc = nullptr;               // RK4 stage accumulator
+    int graph_next[3] = {0, 0, 0};       // state buffer after the captured step from buffer i
     // transport
     DevTran *d_tran = nullptr;
     double *props = nullptr;
@@ -210,6 +212,23 @@ hipError_t enqueue_fluid(fcfd_ctx *c, double dt, const double *dt_dev) {
         if ((e = flux(c->Q[a])) != hipSuccess) return e;
         if ((e = stage(stage_args(c, 0, nullptr, c->Q[a], c->Q[b], dt, dt_dev, 1.0, 1.0))) != hipSuccess) return e;
         c->cur = b;
+        return hipSuccess;
+    }
+    if (c->o.select_time == 2) {                                   // RK4 (:222-356)
+        auto rk4 = [&](int st, const double *Qin, double *Qout) {
+            StageArgs s = stage_args(c, st, c->Q[a], Qin, Qout, dt, dt_dev, 1.0, 1.0);
+            s.acc = c->acc;
+            return timed(c, 1, [&]() { return launch_rk4_stage(s, nsp, c->stream); });
+        };
+        if ((e = flux(c->Q[a])) != hipSuccess) return e;
+        if ((e = rk4(0, c->Q[a], c->Q[b])) != hipSuccess) return e;
+        if ((e = flux(c->Q[b])) != hipSuccess) return e;
+        if ((e = rk4(1, c->Q[b], c->Q[d])) != hipSuccess) return e;
+        if ((e = flux(c->Q[d])) != hipSuccess) return e;
+        if ((e = rk4(2, c->Q[d], c->Q[b])) != hipSuccess) return e;
+        if ((e = flux(c->Q[b])) != hipSuccess) return e;
+        if ((e = rk4(3, c->Q[b], c->Q[d])) != hipSuccess) return e;
+        c->cur = d;
         return hipSuccess;
     }
     // TVDRK3 (SRC/timeintegration.f90:132-217)
@@ -362,6 +381,10 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
     HIP_TRY(hipMalloc(&c->qf, sb));
     HIP_TRY(hipMemset(c->qf, 0, sb));
     HIP_TRY(hipMalloc(&c->F, sizeof(double) * size_t(c->nequ) * c->PF));
+    if (o->select_time == 2) {
+        HIP_TRY(hipMalloc(&c->acc, sb));
+        HIP_TRY(hipMemset(c->acc, 0, sb));
+    }
     HIP_TRY(hipMalloc(&c->god, sizeof(double) * 3 * c->PF));
     HIP_TRY(hipMalloc(&c->d_dt, sizeof(double)));
     HIP_TRY(hipMalloc(&c->d_time, sizeof(double)));
@@ -405,7 +428,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
     (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_list_n);
-    (void)hipFree(c->d_tran); (void)hipFree(c->props);
+    (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -528,12 +551,13 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
                 hipError_t e2 = hipStreamEndCapture(c->stream, &g);
                 HIP_TRY(e);
                 HIP_TRY(e2);
+                c->graph_next[cur] = c->cur;
                 c->cur = cur;                       // capture only recorded the step
                 HIP_TRY(hipGraphInstantiate(&c->graph[cur], g, nullptr, nullptr, 0));
                 HIP_TRY(hipGraphDestroy(g));
             }
             HIP_TRY(hipGraphLaunch(c->graph[cur], c->stream));
-            c->cur = fluid ? (cur + 1) % 3 : cur;
+            c->cur = c->graph_next[cur];
         }
     }
     if (time) HIP_TRY(hipMemcpyAsync(time, c->d_time, sizeof(double), hipMemcpyDeviceToHost, c->stream));

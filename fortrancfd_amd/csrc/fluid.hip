@@ -226,6 +226,56 @@ __global__ void __launch_bounds__(64) k_stage_end(StageArgs a) {
     qc_to_qf_store<NSP>(a.thermo, Qv, a.qf, a.P, c);
 }
 
+// RK4 stage (SRC/timeintegration.f90:222-356) with the reference's semantics: within the step the
+// advection fluxes keep reading the primitives of the step start (only the left wall ghosts' are
+// refreshed, by update_boundary), interior cells move by a quarter, a quarter and a full dt/dx
+// flux difference in stages 1-3, k_s = -dt/dx dF_s accumulate as k1 + 2 k2 + 2 k3 (+ k4 at the end),
+// the walls are mirrored after every stage and the block-interface ghosts are not refreshed; the
+// last stage sets Q = Q0 + (1/6)(k1 + 2 k2 + 2 k3 + k4) and recovers the primitives everywhere.
+template <int NSP>
+__global__ void __launch_bounds__(64) k_rk4_stage(StageArgs a) {
+    const long c = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= long(a.nblk) * a.S) return;
+    const int b = int(c / a.S), i = int(c % a.S) - a.lv;
+    const int st = a.mode;
+    const bool interior = i >= 0 && i < a.ncell;
+    const bool lwall = b == 0 && i < 0, rwall = b == a.nblk - 1 && i >= a.ncell;
+    double Qv[NSP + 3];
+    if (!interior && !lwall && !rwall) {                  // block-interface ghost: unchanged
+#pragma unroll
+        for (int v = 0; v < NSP + 3; ++v) {
+            Qv[v] = a.Qin[v * a.P + c];
+            a.Qout[v * a.P + c] = Qv[v];
+        }
+        if (st == 3) qc_to_qf_store<NSP>(a.thermo, Qv, a.qf, a.P, c);
+        return;
+    }
+    int sb, si;
+    double sgn;
+    ghost_source(a, b, i, sb, si, sgn);
+    const long sc = long(sb) * a.S + a.lv + si;
+    const long fj = long(sb) * (a.ncell + 1) + si;
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+#pragma unroll
+    for (int v = 0; v < NSP + 3; ++v) {
+        const double d = a.F[v * a.PF + fj + 1] - a.F[v * a.PF + fj];
+        const double k = -(dt / a.dx) * d;
+        const double qi = a.Qin[v * a.P + sc];
+        double Q;
+        if (st < 2) Q = qi - 0.5 * (0.5 * dt / a.dx) * d;
+        else if (st == 2) Q = qi - (dt / a.dx) * d;
+        else Q = a.Q0[v * a.P + sc] + (1.0 / 6.0) * (a.acc[v * a.P + sc] + k);
+        if (interior && st < 3) {
+            const double prev = st == 0 ? 0.0 : a.acc[v * a.P + c];
+            a.acc[v * a.P + c] = st == 0 ? k : prev + 2.0 * k;
+        }
+        if (v == 1) Q = sgn < 0.0 ? -Q : Q;
+        Qv[v] = Q;
+        a.Qout[v * a.P + c] = Q;
+    }
+    if (st == 3 || lwall) qc_to_qf_store<NSP>(a.thermo, Qv, a.qf, a.P, c);
+}
+
 // ---------------------------------------------------------------------------------------------
 // calc_dt (SRC/update.f90:13-59): max wave speed over interior cells -> dt on the device.
 // ---------------------------------------------------------------------------------------------
@@ -430,6 +480,15 @@ hipError_t launch_stage_end(const StageArgs &a, int nsp, hipStream_t st) {
     const long n = long(a.nblk) * a.S;
     const dim3 grid(unsigned((n + 63) / 64)), blk(64);
 #define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_stage_end<N>), grid, blk, 0, st, a); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_rk4_stage(const StageArgs &a, int nsp, hipStream_t st) {
+    const long n = long(a.nblk) * a.S;
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_rk4_stage<N>), grid, blk, 0, st, a); return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;

@@ -20,9 +20,10 @@ struct StageArgs {
     const DevThermo *thermo;
     const double *Q0, *Qin, *F;
     double *Qout, *qf;
+    double *acc;           // RK4: running k1 + 2 k2 + 2 k3
     const double *dt_dev;  // device dt (nullptr: use dt)
     double dt, dx, h1, h2;
-    int mode;              // 0 Qin - dt/dx dF, 1 h1*Q0 + h2*(Qin - dt/dx dF), 2 copy
+    int mode;              // 0 Qin - dt/dx dF, 1 h1*Q0 + h2*(Qin - dt/dx dF), 2 copy; RK4: stage 0..3
     int ghosts_only;
     int nblk, ncell, lv, S;
     long P, PF;
@@ -49,6 +50,7 @@ struct InitArgs {
 
 hipError_t launch_flux(const FluxArgs &a, int nsp, int flag_acu, int flag_weno, hipStream_t st);
 hipError_t launch_stage_end(const StageArgs &a, int nsp, hipStream_t st);
+hipError_t launch_rk4_stage(const StageArgs &a, int nsp, hipStream_t st);
 hipError_t launch_dt(const DtArgs &a, int nsp, hipStream_t st);
 hipError_t launch_init(const InitArgs &a, int nsp, hipStream_t st);
 hipError_t launch_qf_to_qc(const DevThermo *m, int nsp, int n, const double *qf, double *Qc, long P, hipStream_t st);
