@@ -52,7 +52,23 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The Fortran binding module (fortran/fcfd.f90) -> _lib/fortran/{fcfd.mod, fcfd.o}."""
+    out = os.path.join(OUT_DIR, "fortran")
+    os.makedirs(out, exist_ok=True)
+    cmd = [FLANG, "-c", "-O2", os.path.join(PKG, "fortran", "fcfd.f90"), "-module-dir", out,
+           "-o", os.path.join(out, "fcfd.o")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -1,0 +1,170 @@
+! fcfd.f90 -- Fortran 2003 binding (ISO_C_BINDING) of libfcfd_hip.so, for the fortranCFD driver.
+!
+! The reference driver's call sites keep their shape:
+!   calc_dt      (SRC/main.f90:134)  -> fcfd_calc_dt(ctx, cfl, dt)
+!   solve_fluid  (SRC/main.f90:142)  -> fcfd_solve_fluid(ctx, dt)
+!   solve_source (SRC/main.f90:145)  -> fcfd_solve_source(ctx, dt, time, hrr, hrr_dot)
+!   refresh      (SRC/main.f90:149-163) -> fcfd_refresh(ctx)
+! with the block state moved to the device once (fcfd_upload from b_obj(ib)%Qc / %qf, whose storage
+! order -- equation index fastest, ghosts included -- the library takes as is) and back for output
+! (fcfd_download).  Every function returns 0 or a negative error code; fcfd_check turns a failure
+! into `error stop` with the library's message, the reference's convention for fatal errors.
+module fcfd
+    use iso_c_binding
+    implicit none
+    private
+    public :: fcfd_opts, fcfd_check, fcfd_error_message
+    public :: fcfd_mech_load, fcfd_mech_free, fcfd_mech_dims
+    public :: fcfd_create, fcfd_destroy, fcfd_lvir, fcfd_upload, fcfd_download, fcfd_init_cells
+    public :: fcfd_calc_dt, fcfd_solve_fluid, fcfd_solve_source, fcfd_refresh, fcfd_advance
+    public :: fcfd_flux_advection_host, fcfd_flux_tran_host, fcfd_chem_step_host
+
+    ! Same fields and codes as the reference's &params switches (SRC/params.f90:106-111).
+    type, bind(C) :: fcfd_opts
+        integer(c_int) :: do_hydro = 1, do_tran = 0, do_reaction = 0
+        integer(c_int) :: select_time = 1, select_riemann = 2, flag_acu = 2
+        integer(c_int) :: flag_muscl = 0, flag_weno = 2, is_spherical = 0
+        integer(c_int) :: flag_lbound = 0, flag_rbound = 0
+        real(c_double) :: atol = 1.0d-13, rtol = 1.0d-5
+    end type fcfd_opts
+
+    interface
+        integer(c_int) function fcfd_mech_load(chem_bin, tran_bin, mech) bind(C, name="fcfd_mech_load")
+            import :: c_int, c_char, c_ptr
+            character(kind=c_char), dimension(*), intent(in) :: chem_bin, tran_bin
+            type(c_ptr), intent(out) :: mech
+        end function
+        subroutine fcfd_mech_free(mech) bind(C, name="fcfd_mech_free")
+            import :: c_ptr
+            type(c_ptr), value :: mech
+        end subroutine
+        integer(c_int) function fcfd_mech_dims(mech, kk, ii) bind(C, name="fcfd_mech_dims")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: mech
+            integer(c_int), intent(out) :: kk, ii
+        end function
+        integer(c_int) function fcfd_create(mech, opts, nblk, ncell, dx, ctx) bind(C, name="fcfd_create")
+            import :: c_int, c_double, c_ptr, fcfd_opts
+            type(c_ptr), value :: mech
+            type(fcfd_opts), intent(in) :: opts
+            integer(c_int), value :: nblk, ncell
+            real(c_double), value :: dx
+            type(c_ptr), intent(out) :: ctx
+        end function
+        subroutine fcfd_destroy(ctx) bind(C, name="fcfd_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: ctx
+        end subroutine
+        integer(c_int) function fcfd_lvir(ctx) bind(C, name="fcfd_lvir")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        ! ib is 0-based; Qc / qf are b_obj(ib+1)%Qc(:, ista-lvir:iend+lvir) etc.
+        integer(c_int) function fcfd_upload(ctx, ib, Qc, qf) bind(C, name="fcfd_upload")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ib
+            real(c_double), intent(in) :: Qc(*), qf(*)
+        end function
+        integer(c_int) function fcfd_download(ctx, ib, Qc, qf) bind(C, name="fcfd_download")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ib
+            real(c_double), intent(out) :: Qc(*), qf(*)
+        end function
+        integer(c_int) function fcfd_init_cells(ctx, cells) bind(C, name="fcfd_init_cells")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: cells(*)
+        end function
+        integer(c_int) function fcfd_calc_dt(ctx, cfl, dt) bind(C, name="fcfd_calc_dt")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: cfl
+            real(c_double), intent(out) :: dt
+        end function
+        integer(c_int) function fcfd_solve_fluid(ctx, dt) bind(C, name="fcfd_solve_fluid")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: dt
+        end function
+        integer(c_int) function fcfd_solve_source(ctx, dt, time, hrr, hrr_dot) bind(C, name="fcfd_solve_source")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: dt, time
+            real(c_double), intent(out) :: hrr(*), hrr_dot(*)
+        end function
+        integer(c_int) function fcfd_refresh(ctx) bind(C, name="fcfd_refresh")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function fcfd_advance(ctx, cfl, nsteps, time) bind(C, name="fcfd_advance")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: cfl
+            integer(c_int), value :: nsteps
+            real(c_double), intent(inout) :: time
+        end function
+        integer(c_int) function fcfd_flux_advection_host(mech, opts, n, lv, dx, qf, Fplus, godunov) &
+                bind(C, name="fcfd_flux_advection_host")
+            import :: c_int, c_ptr, c_double, fcfd_opts
+            type(c_ptr), value :: mech
+            type(fcfd_opts), intent(in) :: opts
+            integer(c_int), value :: n, lv
+            real(c_double), value :: dx
+            real(c_double), intent(in) :: qf(*)
+            real(c_double), intent(out) :: Fplus(*), godunov(*)
+        end function
+        integer(c_int) function fcfd_flux_tran_host(mech, n, lv, dx, Qc, Fplus) bind(C, name="fcfd_flux_tran_host")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: mech
+            integer(c_int), value :: n, lv
+            real(c_double), value :: dx
+            real(c_double), intent(in) :: Qc(*)
+            real(c_double), intent(out) :: Fplus(*)
+        end function
+        ! macks(dt, nspe+1, yiT, rho) (SRC/source.f90:58) on n cells
+        integer(c_int) function fcfd_chem_step_host(mech, opts, n, dt, rho, yT, stats) bind(C, name="fcfd_chem_step_host")
+            import :: c_int, c_ptr, c_double, fcfd_opts
+            type(c_ptr), value :: mech
+            type(fcfd_opts), intent(in) :: opts
+            integer(c_int), value :: n
+            real(c_double), intent(in) :: dt(*), rho(*)
+            real(c_double), intent(inout) :: yT(*)
+            integer(c_int), intent(out) :: stats(*)
+        end function
+        type(c_ptr) function fcfd_last_error_c() bind(C, name="fcfd_last_error")
+            import :: c_ptr
+        end function
+        integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+            import :: c_ptr, c_size_t
+            type(c_ptr), value :: s
+        end function
+    end interface
+
+contains
+
+    function fcfd_error_message() result(msg)
+        character(len=:), allocatable :: msg
+        type(c_ptr) :: p
+        character(kind=c_char), pointer :: chars(:)
+        integer :: n, i
+        p = fcfd_last_error_c()
+        n = int(c_strlen(p))
+        call c_f_pointer(p, chars, [n])
+        allocate(character(len=n) :: msg)
+        do i = 1, n
+            msg(i:i) = chars(i)
+        end do
+    end function fcfd_error_message
+
+    subroutine fcfd_check(rc, where)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: where
+        if (rc /= 0) then
+            write(*, '(a, a, i0, a, a)') where, ': fcfd error ', rc, ': ', fcfd_error_message()
+            error stop 1
+        end if
+    end subroutine fcfd_check
+
+end module fcfd

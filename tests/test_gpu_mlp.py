@@ -174,9 +174,9 @@ def test_out_of_range_cells_take_the_fp64_path_bitwise(torch, mech):
 
 def test_accepted_cells_conserve_elements_and_energy(torch, mech, orc):
     """Accepted cells satisfy the acceptance invariants: a network with zero output weights
-    predicts dY = out_mean, dT = out_mean[53] = 0; out_mean moves O2 and N2 by 1e-6 in opposite
-    directions, which breaks O and N balance, so the projection, the clipping and the energy
-    inversion all act."""
+    predicts increments out_mean (Box-Cox units) and dT = out_mean[53] = 0; out_mean moves O2 and
+    N2 in opposite directions, which breaks O and N balance, so the projection, the clipping and the
+    energy inversion all act."""
     from fortrancfd_amd.lib import Surrogate
     s = Surrogate(mech)
     flat = M.init_params(2)
@@ -184,8 +184,8 @@ def test_accepted_cells_conserve_elements_and_energy(torch, mech, orc):
     lay[4][0][:] = 0
     s.set_params(M.pack(lay))
     out_mean = np.zeros(64)
-    out_mean[mech.names.index("O2")] = 1e-6
-    out_mean[mech.names.index("N2")] = -1e-6
+    out_mean[mech.names.index("O2")] = 1e-4          # Box-Cox units: dY ~ Y^0.9 * 1e-4
+    out_mean[mech.names.index("N2")] = -1e-4
     norm = {"in_mean": np.zeros(64), "in_std": np.ones(64), "lo": np.full(64, -1e300),
             "hi": np.full(64, 1e300), "out_mean": out_mean, "out_std": np.ones(64)}
     s.set_norm(norm)
