@@ -52,12 +52,13 @@ __device__ __forceinline__ double sum4(double *slot, int w, int lane, double v) 
     return s;
 }
 
-// f = dy/dt at (y, T) for this wave's species slots (f[j] of species w + NW*j) and fT for all
-// waves.  rho in g/cm^3.  y[] beyond NSP is ignored.
-template <int NSP>
+// f = dy/dt at (y + s d, T) for this wave's species slots (f[j] of species w + NW*j) and fT for
+// all waves (SHIFT = false: at y).  rho in g/cm^3.  Slots beyond NSP are ignored.
+template <int NSP, bool SHIFT>
 __device__ void rhs_lane(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, LaneShared<NSP> &sh,
-                         int w, int lane, double rho, const double (&y)[slots<NSP>()], double T,
-                         double (&f)[slots<NSP>()], double &fT) {
+                         int w, int lane, double rho, const double (&y)[slots<NSP>()],
+                         const double (&d)[slots<NSP>()], double s, double T, double (&f)[slots<NSP>()],
+                         double &fT) {
     constexpr int NS = slots<NSP>();
     const TPow p = tpow(T, th->ru);
     double uk[NS];
@@ -68,12 +69,13 @@ __device__ void rhs_lane(const DevThermo *__restrict__ th, const DevKin *__restr
         const int k = w + NW * j;
         uk[j] = 0.0;
         if (k < NSP) {
-            const double ck = rho * y[j] / th->wt[k];
+            const double yj = SHIFT ? y[j] + s * d[j] : y[j];
+            const double ck = rho * yj / th->wt[k];
             sh.C[k * 64 + lane] = ck;
             ctot_w = ctot_w + ck;
             const Nasa a = nasa_lane(th, k, T);
             uk[j] = h_mol(a, p) - p.rut;
-            cvm_w = cvm_w + y[j] * (cp_mol(a, p) - th->ru) / th->wt[k];
+            cvm_w = cvm_w + yj * (cp_mol(a, p) - th->ru) / th->wt[k];
             const double t2 = T * T, t3 = T * t2, t4 = t2 * t2;     // CKSMH, as smh_species
             double s = (lnT - 1.0) * a.a0;
             s = s + T / 2.0 * a.a1;
@@ -91,13 +93,30 @@ __device__ void rhs_lane(const DevThermo *__restrict__ th, const DevKin *__restr
     __syncthreads();
     const double pfac = th->pa / (th->ru * T);
     const double *Cp = sh.C + lane, *Gp = sh.G + lane;
-    for (int i = w; i < K->ii; i += NW) {
-        const Rop r = reaction<64>(K, i, T, lnT, pfac, ctot, Cp, Gp);
-        const double q = r.qf - r.qr;
-        const int *nu = K->nu[i], *nk = K->nk[i];
+    // RB reactions per pass: their rates are independent, so their table loads, LDS reads and
+    // transcendental chains overlap; the scatter follows in reaction order.
+    constexpr int RB = 4;
+    for (int i0 = w; i0 < K->ii; i0 += NW * RB) {
+        double q[RB];
 #pragma unroll
-        for (int n = 0; n < 6; ++n)
-            if (nk[n] >= 0) Wp[nk[n] * 64] = Wp[nk[n] * 64] + q * (double)nu[n];
+        for (int u = 0; u < RB; ++u) {
+            const int i = i0 + NW * u;
+            q[u] = 0.0;
+            if (i < K->ii) {
+                const Rop r = reaction<64>(K, i, T, lnT, pfac, ctot, Cp, Gp);
+                q[u] = r.qf - r.qr;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            const int i = i0 + NW * u;
+            if (i < K->ii) {
+                const int *nu = K->nu[i], *nk = K->nk[i];
+#pragma unroll
+                for (int n = 0; n < 6; ++n)
+                    if (nk[n] >= 0) Wp[nk[n] * 64] = Wp[nk[n] * 64] + q[u] * (double)nu[n];
+            }
+        }
     }
     __syncthreads();
     double su_w = 0.0;
@@ -167,7 +186,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
 
     long col = 0;
     double rho_si, dt, T0;
-    double y[NS], yraw[NS];
+    double y[NS];
     if (a.layout == 0) {
         const int b = c / a.ncell, i = c % a.ncell;
         col = long(b) * a.S + a.lv + i;
@@ -177,8 +196,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             const int k = w + NW * j;
-            yraw[j] = k < NSP ? a.qf[(3 + k) * a.P + col] : 0.0;
-            y[j] = yraw[j];
+            y[j] = k < NSP ? a.qf[(3 + k) * a.P + col] : 0.0;
         }
         adjust_lane<NSP>(slot, w, lane, y);
     } else {
@@ -188,8 +206,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             const int k = w + NW * j;
-            yraw[j] = k < NSP ? a.b_y[long(k) * a.ncells + c] : 0.0;
-            y[j] = yraw[j];
+            y[j] = k < NSP ? a.b_y[long(k) * a.ncells + c] : 0.0;
         }
     }
     const double rho = rho_si * 1.e-3;
@@ -204,13 +221,10 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
 
     // ---- Heun over dt with the embedded Euler error estimate ----
     double f0[NS], f0T;
-    rhs_lane<NSP>(th, K, sh, w, lane, rho, y, T0, f0, f0T);
-    double y1[NS];
-#pragma unroll
-    for (int j = 0; j < NS; ++j) y1[j] = y[j] + dt * f0[j];
+    rhs_lane<NSP, false>(th, K, sh, w, lane, rho, y, y, 0.0, T0, f0, f0T);
     const double T1 = T0 + dt * f0T;
     double f1[NS], f1T;
-    rhs_lane<NSP>(th, K, sh, w, lane, rho, y1, T1, f1, f1T);
+    rhs_lane<NSP, true>(th, K, sh, w, lane, rho, y, f0, dt, T1, f1, f1T);   // at y + dt f0
     double err_w = 0.0;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
@@ -260,13 +274,14 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
         // (SRC/thermo.f90:128-132): the creation minus destruction rates of CKCDYR are the net
         // production rates, evaluated at the new state.
         double fn[NS], fnT;
-        rhs_lane<NSP>(th, K, sh, w, lane, rho, y, T, fn, fnT);
+        rhs_lane<NSP, false>(th, K, sh, w, lane, rho, y, y, 0.0, T, fn, fnT);
         double s1 = 0.0, s2 = 0.0;
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             const int k = w + NW * j;
             if (k < NSP) {
-                s1 = s1 + rho * th->h298[k] * (y[j] - yraw[j]) / dt;
+                const double y_in = a.layout == 0 ? a.qf[(3 + k) * a.P + col] : a.b_y[long(k) * a.ncells + c];
+                s1 = s1 + rho * th->h298[k] * (y[j] - y_in) / dt;
                 s2 = s2 + th->h298[k] * fn[j] * rho;     // W_k h_k wdot_k = h_k (W_k wdot_k / rho) rho
             }
         }
