@@ -151,7 +151,7 @@ def oracle_coupled_baseline(cells, nblk, ncell, dx, cfl, steps):
     from oracle.pyoracle import Domain, Oracle
     orc = Oracle()
     d = Domain(orc, cells, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2,
-               select_time=1, do_reaction=1)
+               select_time=1)
     t0 = time.perf_counter()
     for _ in range(steps):
         dt = d.calc_dt(cfl)
