@@ -175,7 +175,10 @@ template <int NSP>
 __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
     constexpr int NS = slots<NSP>();
     __shared__ LaneShared<NSP> sh;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index is uniform across the wave; saying so turns every table read indexed by it
+    // (reactions i = w + 4 n, species k = w + 4 j) into a scalar load and every branch on the
+    // reaction kind into a scalar branch
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const DevThermo *__restrict__ th = a.thermo;
     const DevKin *__restrict__ K = a.kin;
     const int cell0 = blockIdx.x * 64;

@@ -118,36 +118,43 @@ hipError_t launch_gemm(const GemmArgs &a, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Features: x = (b(Y_k), T, log(p/1 atm), log dt) z-scored (b: Box-Cox above); pad to 64 columns.
-// in_range: every raw feature inside the training range [lo, hi].
+// Wave helpers (one cell per wave, lanes over species / feature columns).
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ void features_of(const SurrogateNorm *__restrict__ nm, const DevThermo *__restrict__ th,
-                                            int kk, double rho, double T, double dt, const double *y, long stride,
-                                            uint16_t *xrow, int *in_range) {
-    double swy = 0.0;
-    for (int k = 0; k < kk; ++k) swy += y[k * stride] / th->wt[k];
-    const double p = rho * 8.31451 * T * swy * 1.0e3;          // Pa (ideal gas, SI)
-    bool ok = rho > 0.0 && T > 0.0 && dt > 0.0;
-    for (int c = 0; c < MLP_INP; ++c) {
-        double raw = 0.0;
-        if (c < kk) raw = boxcox(y[c * stride]);
-        else if (c == kk) raw = T;
-        else if (c == kk + 1) raw = log(p / 101325.0);
-        else if (c == kk + 2) raw = log(dt);
-        if (c < MLP_IN) {
-            if (!(raw >= nm->lo[c] && raw <= nm->hi[c])) ok = false;
-            xrow[c] = f2bf(float((raw - nm->in_mean[c]) / nm->in_std[c]));
-        } else {
-            xrow[c] = 0;
-        }
-    }
-    if (in_range) *in_range = ok ? 1 : 0;
+__device__ __forceinline__ double wsum64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// One row of features per cell: rows 0 .. npad of all cells (list == null; rows >= n zero-filled)
-// or rows 0 .. *list_n for the cells list[row].
-__global__ void k_features(FeatArgs f) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// Column c of the mass fractions of a cell: the primitive state's Y renormalised by adjust_sum
+// (layout 0, as solve_source sees it) or the batch arrays as given (layout 1).
+template <class Args>
+__device__ __forceinline__ void cell_state(const Args &f, int cell, int lane, int kk, double &y, double &rho,
+                                           double &T, long &col) {
+    y = 0.0;
+    if (f.layout == 0) {
+        col = long(cell / f.ncell) * f.S + f.lv + (cell % f.ncell);
+        if (lane < kk) y = fmax(f.qf[(3 + lane) * f.P + col], 0.0);
+        y = y / wsum64(y);
+        rho = f.qf[col];
+        T = f.qf[2 * f.P + col];
+    } else {
+        col = 0;
+        if (lane < kk) y = f.Y[long(lane) * f.n + cell];
+        rho = f.rho[cell];
+        T = f.T[cell];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Features: x = (b(Y_k), T, log(p/1 atm), log dt) z-scored (b: Box-Cox above), padded to 64
+// columns; in_range = every raw feature inside the training range [lo, hi].  ONE ROW PER WAVE,
+// lane = feature column: rows 0 .. npad of all cells (list == null; rows >= n zero-filled) or rows
+// 0 .. *list_n for the cells list[row].
+// ---------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_features(FeatArgs f) {
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
     int cell = t;
     if (f.list) {
         if (t >= int(*f.list_n)) return;
@@ -155,29 +162,29 @@ __global__ void k_features(FeatArgs f) {
     } else {
         if (t >= f.npad) return;
         if (t >= f.n) {
-            for (int j = 0; j < MLP_INP; ++j) f.X[long(t) * MLP_INP + j] = 0;
-            if (f.in_range) f.in_range[t] = 0;
+            f.X[long(t) * MLP_INP + lane] = 0;
+            if (f.in_range && lane == 0) f.in_range[t] = 0;
             return;
         }
     }
     const int kk = f.kk;
-    double y[MAXK];
-    double rho, T, dt;
-    if (f.layout == 0) {
-        const long col = long(cell / f.ncell) * f.S + f.lv + (cell % f.ncell);
-        double s = 0.0;
-        for (int k = 0; k < kk; ++k) { y[k] = fmax(f.qf[(3 + k) * f.P + col], 0.0); s += y[k]; }
-        for (int k = 0; k < kk; ++k) y[k] /= s;
-        rho = f.qf[col];
-        T = f.qf[2 * f.P + col];
-        dt = f.dt_dev ? *f.dt_dev : f.dt;
-    } else {
-        for (int k = 0; k < kk; ++k) y[k] = f.Y[long(k) * f.n + cell];
-        rho = f.rho[cell];
-        T = f.T[cell];
-        dt = f.dtv[cell];
-    }
-    features_of(f.norm, f.thermo, kk, rho, T, dt, y, 1, f.X + long(t) * MLP_INP, f.in_range ? f.in_range + t : nullptr);
+    const SurrogateNorm *__restrict__ nm = f.norm;
+    double y, rho, T;
+    long col;
+    cell_state(f, cell, lane, kk, y, rho, T, col);
+    const double dt = f.layout == 0 ? (f.dt_dev ? *f.dt_dev : f.dt) : f.dtv[cell];
+    const double swy = wsum64(lane < kk ? y / f.thermo->wt[lane] : 0.0);
+    const double p = rho * 8.31451 * T * swy * 1.0e3;          // Pa (ideal gas, SI)
+    double raw = 0.0;
+    if (lane < kk) raw = boxcox(y);
+    else if (lane == kk) raw = T;
+    else if (lane == kk + 1) raw = log(p / 101325.0);
+    else if (lane == kk + 2) raw = log(dt);
+    const bool feat = lane < kk + 3;
+    const bool ok_l = !feat || (raw >= nm->lo[lane] && raw <= nm->hi[lane]);
+    f.X[long(t) * MLP_INP + lane] = feat ? f2bf(float((raw - nm->in_mean[lane]) / nm->in_std[lane])) : uint16_t(0);
+    const bool ok = __all(ok_l) && rho > 0.0 && T > 0.0 && dt > 0.0;
+    if (f.in_range && lane == 0) f.in_range[t] = ok ? 1 : 0;
 }
 
 // Normalised targets: (b(Y1) - b(Y0), T1 - T0) z-scored, columns >= 54 zero.
@@ -277,18 +284,21 @@ __global__ void k_to_bf16(const float *w, uint16_t *w16, long n) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Inference epilogue: one thread per cell.  Candidate Y1 = Y0 + dY (network), negative entries
-// clipped, element balance restored by a weighted projection onto sum_k a_ek dY_k = 0
-// (a_ek = atoms of element e per unit mass of species k, weights Y1_k), then T1 from conservation of
+// Inference epilogue, ONE CELL PER WAVE (lane = species).  Candidate Y1 = b^-1(b(Y0) + db)
+// (network), then the element balance is restored by the weighted projection minimising
+// sum_k dY_k^2 / w_k subject to sum_k a_ek (Y1_k - Y0_k) = 0 (a_ek = atoms of element e per unit
+// mass of species k, w_k = max(Y1_k, 0) + 1e-12; MM x MM normal equations, elements absent from
+// the mixture left out), negative entries are clipped, and T1 follows from conservation of
 // sum_k Y_k u_k(T) / W_k (Newton from the network's T).  Accepted when the inputs were inside the
-// training range, the element residual after projection and clipping is <= resid_tol and the
-// network temperature agrees with the energy temperature to t_tol; otherwise the cell is queued for
-// the FP64 integrator.
+// training range, the element residual after projection and clipping is <= resid_tol, and the
+// network temperature agrees with the energy temperature to t_tol; otherwise the cell is appended
+// to the fallback list for the FP64 integrator.
 // ---------------------------------------------------------------------------------------------
-template <int NSP>
-__global__ void k_apply(ApplyArgs a) {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;   // network row
-    int t = row;                                               // cell
+template <int NSP, int MM>
+__global__ void __launch_bounds__(256) k_apply(ApplyArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);        // network row
+    int t = row;                                                 // cell
     if (a.list) {
         if (row >= int(*a.list_n)) return;
         t = a.list[row];
@@ -297,123 +307,104 @@ __global__ void k_apply(ApplyArgs a) {
     }
     const DevThermo *__restrict__ th = a.thermo;
     const SurrogateNorm *__restrict__ nm = a.norm;
-    long col = 0;
-    double T0, y0[NSP];
-    if (a.layout == 0) {
-        const int b = t / a.ncell, i = t % a.ncell;
-        col = long(b) * a.S + a.lv + i;
-        T0 = a.qf[2 * a.P + col];
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) { y0[k] = fmax(a.qf[(3 + k) * a.P + col], 0.0); s = s + y0[k]; }
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) y0[k] = y0[k] / s;
-    } else {
-        T0 = a.T[t];
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) y0[k] = a.Y[long(k) * a.n + t];
-    }
+    const bool sp = lane < NSP;
+    double y0, rho, T0;
+    long col;
+    cell_state(a, t, lane, NSP, y0, rho, T0, col);
+    (void)rho;
     const float *out = a.Yh + long(row) * MLP_OUTP;
-    double y1[NSP];
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) y1[k] = boxcox_inv(boxcox(y0[k]) + (double(out[k]) * nm->out_std[k] + nm->out_mean[k]));
+    double y1 = sp ? boxcox_inv(boxcox(y0) + (double(out[lane]) * nm->out_std[lane] + nm->out_mean[lane])) : 0.0;
     const double Tnet = T0 + (double(out[NSP]) * nm->out_std[NSP] + nm->out_mean[NSP]);
     bool ok = a.in_range[row] != 0;
-    // weighted projection onto element conservation (5 x 5 normal equations)
-    constexpr int ME = 8;
-    const int mm = a.mm;
-    double G[ME][ME], r[ME], el0[ME];
-    for (int e = 0; e < mm; ++e) {
-        r[e] = 0.0;
-        el0[e] = 0.0;
-        for (int f = 0; f < mm; ++f) G[e][f] = 0.0;
-    }
+    const double iw = sp ? 1.0 / th->wt[lane] : 0.0;
+    double ae[MM];
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) {
-        const double w = fmax(y1[k], 0.0) + 1.0e-12;
-        for (int e = 0; e < mm; ++e) {
-            const double ae = a.ncf[k * mm + e] / th->wt[k];
-            r[e] += ae * (y1[k] - y0[k]);
-            el0[e] += ae * y0[k];
-            for (int f = 0; f < mm; ++f) G[e][f] += w * ae * (a.ncf[k * mm + f] / th->wt[k]);
-        }
+    for (int e = 0; e < MM; ++e) ae[e] = sp ? a.ncf[lane * MM + e] * iw : 0.0;
+    const double w = sp ? fmax(y1, 0.0) + 1.0e-12 : 0.0;
+    double G[MM][MM], r[MM], el0[MM];
+#pragma unroll
+    for (int e = 0; e < MM; ++e) {
+        r[e] = wsum64(ae[e] * (y1 - y0));
+        el0[e] = wsum64(ae[e] * y0);
+#pragma unroll
+        for (int f = e; f < MM; ++f) G[e][f] = wsum64(w * ae[e] * ae[f]);
+#pragma unroll
+        for (int f = 0; f < e; ++f) G[e][f] = G[f][e];
     }
-    // Gaussian elimination for lambda (elements absent from the mixture are skipped)
-    double lam[ME];
-    int live[ME];
-    for (int e = 0; e < mm; ++e) { live[e] = G[e][e] > 1.0e-300 ? 1 : 0; lam[e] = 0.0; }
-    for (int e = 0; e < mm; ++e) {
-        if (!live[e]) continue;
-        for (int f = e + 1; f < mm; ++f) {
-            if (!live[f]) continue;
+    double lam[MM];
+    bool live[MM];
+#pragma unroll
+    for (int e = 0; e < MM; ++e) { live[e] = G[e][e] > 1.0e-300; lam[e] = 0.0; }
+#pragma unroll
+    for (int e = 0; e < MM; ++e) {
+#pragma unroll
+        for (int f = e + 1; f < MM; ++f) {
+            if (!live[e] || !live[f]) continue;
             const double l = G[f][e] / G[e][e];
-            for (int g = e; g < mm; ++g) G[f][g] -= l * G[e][g];
+#pragma unroll
+            for (int g = e; g < MM; ++g) G[f][g] -= l * G[e][g];
             r[f] -= l * r[e];
         }
     }
-    for (int e = mm - 1; e >= 0; --e) {
+#pragma unroll
+    for (int e = MM - 1; e >= 0; --e) {
         if (!live[e]) continue;
-        double s = r[e];
-        for (int f = e + 1; f < mm; ++f)
-            if (live[f]) s -= G[e][f] * lam[f];
-        lam[e] = s / G[e][e];
-    }
-    double ssum = 0.0;
+        double acc = r[e];
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) {
-        const double w = fmax(y1[k], 0.0) + 1.0e-12;
-        double c = 0.0;
-        for (int e = 0; e < mm; ++e) c += lam[e] * (a.ncf[k * mm + e] / th->wt[k]);
-        y1[k] = fmax(y1[k] - w * c, 0.0);
-        ssum = ssum + y1[k];
+        for (int f = e + 1; f < MM; ++f)
+            if (live[f]) acc -= G[e][f] * lam[f];
+        lam[e] = acc / G[e][e];
     }
+    double c = 0.0;
+#pragma unroll
+    for (int e = 0; e < MM; ++e) c += lam[e] * ae[e];
+    y1 = sp ? fmax(y1 - w * c, 0.0) : 0.0;
+    const double ssum = wsum64(y1);
     double resid = 0.0;
-    for (int e = 0; e < mm; ++e) {
-        if (!(el0[e] > 1.0e-300)) continue;
-        double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < NSP; ++k) s += (a.ncf[k * mm + e] / th->wt[k]) * (y1[k] - y0[k]);
-        resid = fmax(resid, fabs(s) / el0[e]);
+    for (int e = 0; e < MM; ++e) {
+        const double s = wsum64(ae[e] * (y1 - y0));
+        if (el0[e] > 1.0e-300) resid = fmax(resid, fabs(s) / el0[e]);
     }
     if (!(resid <= a.resid_tol) || !(fabs(ssum - 1.0) <= 1.0e-3)) ok = false;
     // energy-consistent temperature
-    double e0 = 0.0, T1 = Tnet;
+    double e0;
     {
         const TPow p = tpow(T0, th->ru);
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) e0 = e0 + y0[k] * (h_mol(nasa(th, k, T0), p) - p.rut) / th->wt[k];
+        e0 = wsum64(sp ? y0 * (h_mol(nasa(th, lane, T0), p) - p.rut) * iw : 0.0);
     }
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) y1[k] = y1[k] / ssum;
+    y1 = y1 / ssum;
+    double T1 = Tnet;
     if (!(T1 > 200.0 && T1 < 6000.0)) ok = false;
     for (int it = 0; it < 20 && ok; ++it) {
         const TPow p = tpow(T1, th->ru);
         double ek = 0.0, cv = 0.0;
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) {
-            const Nasa c = nasa(th, k, T1);
-            ek = ek + y1[k] * (h_mol(c, p) - p.rut) / th->wt[k];
-            cv = cv + y1[k] * (cp_mol(c, p) - th->ru) / th->wt[k];
+        if (sp) {
+            const Nasa cf = nasa(th, lane, T1);
+            ek = y1 * (h_mol(cf, p) - p.rut) * iw;
+            cv = y1 * (cp_mol(cf, p) - th->ru) * iw;
         }
+        ek = wsum64(ek);
+        cv = wsum64(cv);
         const double d = (e0 - ek) / cv;
         T1 = T1 + d;
         if (fabs(d) <= 1.0e-12 * T1) break;
     }
     if (!(fabs(T1 - Tnet) <= a.t_tol * T1)) ok = false;
-    if (a.route) a.route[t] = ok ? 1 : 0;
+    if (lane == 0 && a.route) a.route[t] = ok ? 1 : 0;
     if (!ok) {
-        const unsigned q = atomicAdd(a.nfallback, 1u);
-        a.fallback[q] = t;
+        if (lane == 0) {
+            const unsigned q = atomicAdd(a.nfallback, 1u);
+            a.fallback[q] = t;
+        }
         return;
     }
     if (a.layout == 0) {
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) a.qf[(3 + k) * a.P + col] = y1[k];
-        a.qf[2 * a.P + col] = T1;
+        if (sp) a.qf[(3 + lane) * a.P + col] = y1;
+        if (lane == 0) a.qf[2 * a.P + col] = T1;
     } else {
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) a.Y[long(k) * a.n + t] = y1[k];
-        a.T[t] = T1;
+        if (sp) a.Y[long(lane) * a.n + t] = y1;
+        if (lane == 0) a.T[t] = T1;
     }
 }
 
@@ -424,7 +415,8 @@ static unsigned blocks(long n, int b) { return unsigned((n + b - 1) / b); }
 
 hipError_t mlp_features(const FeatArgs &f, int rows, hipStream_t st) {
     if (rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_features, dim3(blocks(rows, 128)), dim3(128), 0, st, f);
+    if (f.kk + 3 > MLP_INP) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_features, dim3(blocks(rows, 4)), dim3(256), 0, st, f);
     return hipGetLastError();
 }
 
@@ -480,9 +472,21 @@ hipError_t mlp_to_bf16(const float *w, uint16_t *w16, long n, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int NSP>
+static hipError_t apply_mm(const ApplyArgs &a, hipStream_t st) {
+    const dim3 g(blocks(a.n, 4)), b(256);
+    switch (a.mm) {
+        case 4: hipLaunchKernelGGL((k_apply<NSP, 4>), g, b, 0, st, a); break;
+        case 5: hipLaunchKernelGGL((k_apply<NSP, 5>), g, b, 0, st, a); break;
+        case 6: hipLaunchKernelGGL((k_apply<NSP, 6>), g, b, 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t mlp_apply(const ApplyArgs &a, int nsp, hipStream_t st) {
     if (a.n <= 0) return hipSuccess;
-#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_apply<N>), dim3(blocks(a.n, 64)), dim3(64), 0, st, a); return hipGetLastError(); }
+#define FCFD_CASE(N) if (nsp == N) return apply_mm<N>(a, st);
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;
