@@ -80,7 +80,7 @@ struct RhsOut {
 // Rates of progress are scattered into the species sums with LDS atomics; one wave owns the
 // workgroup, so the order of those additions is fixed and the result reproducible.
 template <int NSP>
-__device__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP> &sh,
+__device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP> &sh,
                       double rho, double y_l, double T, bool split) {
     const int lane = lane_id();
     double uk = 0.0, cpk = 0.0, ck = 0.0;
@@ -137,7 +137,7 @@ __device__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict
 
 // Jacobian df/dy into sh.J (rows/cols 0..NSP, NSP = temperature).
 template <int NSP>
-__device__ void jacobian(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP> &sh,
+__device__ __forceinline__ void jacobian(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP> &sh,
                          double rho, double y_l, double T) {
     const int lane = lane_id();
     constexpr int N = NSP + 1;
@@ -194,7 +194,7 @@ __device__ void jacobian(const DevThermo *__restrict__ th, const DevKin *__restr
 // written back over sh.J; rank = elimination step of this lane's row (N: not pivoted yet, -1: lane
 // beyond the system).  Returns false if a pivot is zero.
 template <int NSP>
-__device__ bool lu(Shared<NSP> &sh, double hg, int &rank) {
+__device__ __forceinline__ bool lu(Shared<NSP> &sh, double hg, int &rank) {
     constexpr int N = NSP + 1;
     const int lane = lane_id();
     rank = lane < N ? N : -1;
@@ -238,7 +238,7 @@ __device__ bool lu(Shared<NSP> &sh, double hg, int &rank) {
 
 // Solves M x = b with the factors in sh.J; b and x distributed one component per lane.
 template <int NSP>
-__device__ double solve(const Shared<NSP> &sh, int rank, double b) {
+__device__ __forceinline__ double solve(const Shared<NSP> &sh, int rank, double b) {
     constexpr int N = NSP + 1;
     const int lane = lane_id();
     const double *row = sh.J + (lane < N ? lane : 0) * LDJ;
@@ -263,7 +263,7 @@ __device__ __forceinline__ double wnorm(double v, double w, int n) {
 
 // Mixture e = sum Y u / W and cv = sum Y cv / W at T (wave sums); y_l this lane's Y.
 template <int NSP>
-__device__ void energy(const DevThermo *__restrict__ th, double y_l, double T, double &e, double &cv) {
+__device__ __forceinline__ void energy(const DevThermo *__restrict__ th, double y_l, double T, double &e, double &cv) {
     const int lane = lane_id();
     double eu = 0.0, ec = 0.0;
     if (lane < NSP) {

@@ -7,16 +7,26 @@
 
 namespace fcfd {
 
+// x**n by binary powering (the product order of the CPU restatement); negative n as 1 / x**(-n).
+// Written without recursion so it inlines: a call from a kernel holding hundreds of live registers
+// costs far more than the multiplications.  The common exponents 0..3 are straight-line.
 __device__ __forceinline__ double ipow(double x, int n) {
-    if (n < 0) return 1.0 / ipow(x, -n);
-    double r = 1.0;
-    bool first = true;
-    while (n) {
-        if (n & 1) { r = first ? x : r * x; first = false; }
-        n >>= 1;
-        if (n) x = x * x;
+    const bool neg = n < 0;
+    int m = neg ? -n : n;
+    double r;
+    if (m == 1) r = x;
+    else if (m == 2) r = x * x;
+    else if (m == 0) r = 1.0;
+    else {
+        r = 1.0;
+        bool first = true;
+        while (m) {
+            if (m & 1) { r = first ? x : r * x; first = false; }
+            m >>= 1;
+            if (m) x = x * x;
+        }
     }
-    return r;
+    return neg ? 1.0 / r : r;
 }
 
 // CKSMH (cklib.f:7540-7560): S/R - H/RT of one species.

@@ -55,7 +55,7 @@ __device__ __forceinline__ double sum4(double *slot, int w, int lane, double v) 
 // f = dy/dt at (y + s d, T) for this wave's species slots (f[j] of species w + NW*j) and fT for
 // all waves (SHIFT = false: at y).  rho in g/cm^3.  Slots beyond NSP are ignored.
 template <int NSP, bool SHIFT>
-__device__ void rhs_lane(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, LaneShared<NSP> &sh,
+__device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, LaneShared<NSP> &sh,
                          int w, int lane, double rho, const double (&y)[slots<NSP>()],
                          const double (&d)[slots<NSP>()], double s, double T, double (&f)[slots<NSP>()],
                          double &fT) {
@@ -138,7 +138,7 @@ __device__ void rhs_lane(const DevThermo *__restrict__ th, const DevKin *__restr
 
 // Mixture e = sum Y u / W and cv (erg/g) over all species at T for this cell.
 template <int NSP>
-__device__ void energy_lane(const DevThermo *__restrict__ th, double *slot, int w, int lane,
+__device__ __forceinline__ void energy_lane(const DevThermo *__restrict__ th, double *slot, int w, int lane,
                             const double (&y)[slots<NSP>()], double T, double &e, double &cv) {
     constexpr int NS = slots<NSP>();
     const TPow p = tpow(T, th->ru);
@@ -158,7 +158,7 @@ __device__ void energy_lane(const DevThermo *__restrict__ th, double *slot, int 
 
 // adjust_sum (SRC/thermo.f90:136-149) of this wave's slots: clip at 0, divide by the cell's sum.
 template <int NSP>
-__device__ void adjust_lane(double *slot, int w, int lane, double (&y)[slots<NSP>()]) {
+__device__ __forceinline__ void adjust_lane(double *slot, int w, int lane, double (&y)[slots<NSP>()]) {
     constexpr int NS = slots<NSP>();
     double s = 0.0;
 #pragma unroll
