@@ -265,7 +265,7 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
     a.dt = dt; a.dt_dev = dt_dev; a.stats = c->d_stats;
     if (want_hrr) { a.hrr = c->d_hrr; a.hrr_dot = c->d_hrr_dot; }
     a.layout = 0;
-    a.route = c->sur ? c->d_accepted : nullptr;
+    a.route = c->d_accepted;
     const int n = c->nblk * c->ncell;
     // explicit screen, one cell per lane: quiescent cells are final, the rest form the stiff list
     hipError_t e = timed(c, 6, [&]() {
@@ -409,6 +409,8 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
         HIP_TRY(hipMalloc(&c->d_hrr, sizeof(double) * nint));
         HIP_TRY(hipMalloc(&c->d_hrr_dot, sizeof(double) * nint));
         HIP_TRY(hipMalloc(&c->d_stats, sizeof(ChemCellStats) * nint));
+        HIP_TRY(hipMalloc(&c->d_accepted, sizeof(int) * nint));
+        HIP_TRY(hipMemset(c->d_accepted, 0, sizeof(int) * nint));
         HIP_TRY(hipMalloc(&c->d_list, sizeof(int) * nint));
         HIP_TRY(hipMalloc(&c->d_list_n, sizeof(unsigned)));
         HIP_TRY(hipMemset(c->d_hrr, 0, sizeof(double) * nint));
@@ -622,10 +624,6 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s) {
     if (s) {
         const int n = c->nblk * c->ncell;
         HIP_TRY(s->reserve(int(round_up(n, 64)), false));
-        if (!c->d_accepted) {
-            HIP_TRY(hipMalloc(&c->d_accepted, sizeof(int) * size_t(n)));
-            HIP_TRY(hipMemset(c->d_accepted, 0, sizeof(int) * size_t(n)));
-        }
     }
     drop_graphs(c);
     c->sur = s;
@@ -633,7 +631,7 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s) {
 }
 
 int fcfd_source_accepted(fcfd_ctx *c, int *per_cell, int *n_fallback) {
-    if (!c || !c->d_accepted) { set_error("fcfd_source_accepted: no surrogate solve_source on this context"); return FCFD_E_ARG; }
+    if (!c || !c->d_accepted) { set_error("fcfd_source_accepted: the context has no chemistry"); return FCFD_E_ARG; }
     const size_t n = size_t(c->nblk) * c->ncell;
     std::vector<int> h(n);
     HIP_TRY(hipMemcpyAsync(h.data(), c->d_accepted, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));

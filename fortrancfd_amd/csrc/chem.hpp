@@ -56,7 +56,7 @@ struct ChemArgs {
     // explicit screen output: cells that need the implicit integrator (list + length)
     int *stiff;
     unsigned *stiff_n;
-    int *route;                  // optional per-cell route code: 2 = accepted by the explicit screen
+    int *route;                  // optional per-cell route: 2 accepted by the explicit screen, 0 stiff
     int layout;                  // 0: domain, 1: batch
 };
 

@@ -253,6 +253,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
             if (lane == leader) base = atomicAdd(a.stiff_n, unsigned(__popcll(bal)));
             base = __shfl(base, leader, 64);
             a.stiff[base + rank] = cell;
+            if (a.route) a.route[cell] = 0;
         }
     }
     // ---- accepted cells: state update and energy-consistent temperature ----

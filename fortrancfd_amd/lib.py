@@ -239,7 +239,8 @@ class Solver:
         self._sur = sur
 
     def source_accepted(self):
-        """(per-cell accepted flags, number of FP64 fallback cells) of the last surrogate call."""
+        """(per-cell route of the last solve_source: 2 explicit screen, 1 surrogate, 0 implicit
+        FP64 integrator; number of implicit-integrator cells)."""
         acc = np.zeros(self.nblk * self.ncell, np.int32)
         nf = C.c_int()
         _check(lib().fcfd_source_accepted(self.handle, acc.ctypes.data_as(C.POINTER(C.c_int)),

@@ -5,7 +5,9 @@
 
 Training pairs come from the batched-chemistry generator (workloads.chemistry_batch: T ~ U(1000,
 2600) K, p ~ log-U(1, 50) atm, dt ~ log-U(1e-8, 1e-6) s, mixing-line compositions) solved by this
-package's FP64 integrator on the GPU.  The network is trained on the GPU by the library's own
+package's FP64 integrator on the GPU, optionally joined by the stiff-cell (state, result) pairs of
+coupled reacting runs with other hot-spot seeds than the benchmark's (ignition kernels of pristine
+fresh mixture, which the mixing line never produces).  The network is trained on the GPU by the library's own
 forward / backward / Adam kernels (fcfd_sur_train_step): Adam, learning rate 1e-3 on a cosine
 schedule, batch 8192.  Species enter and leave the network through the Box-Cox transform
 b(Y) = 10 (Y^0.1 - 1); inputs are z-scored with the training set's mean and standard deviation;
@@ -69,6 +71,47 @@ def solve_pairs(mech, rho, dt, T, Y, chunk=131072, log=print):
     return T1, Y1
 
 
+def coupled_pairs(mech, seeds, steps, every=2, nblk=16, ncell=4096, dx=25e-6, cfl=0.5, log=print):
+    """(state, FP64 result) pairs of the cells the explicit screen sends to the implicit integrator
+    in coupled reacting runs: the 65536-cell CH4/air domain with three 32-cell hot spots of the
+    given seeds, stepped with the FP64 chemistry; every `every`-th step the stiff cells' states
+    before and after solve_source are recorded.  Returns host arrays (rho, dt, T0, Y0, T1, Y1)."""
+    from .lib import Solver
+    out = {k: [] for k in ("rho", "dt", "T0", "Y0", "T1", "Y1")}
+    t0 = time.time()
+    for seed in seeds:
+        s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2,
+                   select_time=1, do_reaction=1)
+        s.init_cells(workloads.coupled_cells(mech, nblk, ncell, seed=seed))
+        lv = s.lv
+        inner = (slice(None), slice(lv, lv + ncell))
+        n0 = sum(len(v) for v in out["T0"])
+        for step in range(steps):
+            dt = s.calc_dt(cfl)
+            s.solve_fluid(dt)
+            rec = step % every == 0
+            if rec:
+                q0 = s.download()[1][inner].reshape(-1, mech.nequ)
+            s.solve_source(dt, diagnostics=False)
+            if rec:
+                route, _ = s.source_accepted()
+                stiff = route == 0
+                if stiff.any():
+                    q1 = s.download()[1][inner].reshape(-1, mech.nequ)
+                    y0 = np.maximum(q0[stiff, 3:], 0.0)
+                    out["rho"].append(q0[stiff, 0])
+                    out["dt"].append(np.full(stiff.sum(), dt))
+                    out["T0"].append(q0[stiff, 2])
+                    out["Y0"].append(y0 / y0.sum(axis=1, keepdims=True))
+                    out["T1"].append(q1[stiff, 2])
+                    out["Y1"].append(q1[stiff, 3:])
+            s.refresh()
+        n1 = sum(len(v) for v in out["T0"])
+        log(f"  coupled run seed {seed}: {steps} steps, {n1 - n0} stiff-cell pairs "
+            f"({time.time() - t0:.1f} s)")
+    return tuple(np.concatenate(out[k]) for k in ("rho", "dt", "T0", "Y0", "T1", "Y1"))
+
+
 def fit_norm(mech, feats, dY, dTv):
     kk = mech.kk
     nm = {k: np.zeros(64) for k in Surrogate.NORM_FIELDS}
@@ -113,14 +156,21 @@ def load_surrogate(mech, path=DEFAULT_WEIGHTS, resid_tol=1e-5, t_tol=2e-3):
     return s
 
 
-def train(mech, n_pairs, epochs, batch=8192, lr=1e-3, seed=workloads.SEED + 4, log=print):
-    """Generate n_pairs training pairs, fit the scaling and train for `epochs` passes.
-    Returns (Surrogate, params, norm, meta)."""
+def train(mech, n_pairs, epochs, batch=8192, lr=1e-3, seed=workloads.SEED + 4, log=print,
+          coupled_seeds=(), coupled_steps=5000, coupled_every=2):
+    """Generate n_pairs batched-chemistry training pairs (plus, with coupled_seeds, the stiff-cell
+    pairs of coupled runs with those hot-spot seeds), fit the scaling and train for `epochs`
+    passes.  Returns (Surrogate, params, norm, meta)."""
     torch = _torch()
     t0 = time.time()
     rho, dt, T0, Y0 = workloads.chemistry_batch(mech, n_pairs, seed=seed)
     log(f"generated {n_pairs} states ({time.time() - t0:.1f} s); solving with the FP64 integrator")
     T1, Y1 = solve_pairs(mech, rho, dt, T0, Y0, log=log)
+    n_batch = n_pairs
+    if coupled_seeds:
+        c = coupled_pairs(mech, coupled_seeds, coupled_steps, coupled_every, log=log)
+        rho, dt, T0, Y0, T1, Y1 = (np.concatenate([a, b]) for a, b in zip((rho, dt, T0, Y0, T1, Y1), c))
+        n_pairs = T0.size
     t_solve = time.time() - t0
     feats = raw_features(mech, rho, dt, T0, Y0)
     norm = fit_norm(mech, feats, boxcox(Y1) - boxcox(Y0), T1 - T0)
@@ -152,7 +202,9 @@ def train(mech, n_pairs, epochs, batch=8192, lr=1e-3, seed=workloads.SEED + 4, l
         log(f"  epoch {ep + 1}/{epochs}: loss {loss:.4e} (lr {lr_t:.2e}, {time.time() - t1:.1f} s)")
     torch.cuda.synchronize()
     t_train = time.time() - t1
-    meta = {"pairs": n_pairs, "epochs": epochs, "batch": batch, "lr": lr, "seed": seed,
+    meta = {"pairs": n_pairs, "batched_pairs": n_batch, "coupled_pairs": n_pairs - n_batch,
+            "coupled_seeds": list(coupled_seeds), "coupled_steps": coupled_steps,
+            "epochs": epochs, "batch": batch, "lr": lr, "seed": seed,
             "steps": total, "final_loss": float(hist[-1]), "loss_history": [float(v) for v in hist],
             "solve_s": t_solve, "train_s": t_train,
             "train_samples_per_s": total * batch / t_train}
@@ -224,6 +276,10 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--out", default=DEFAULT_WEIGHTS)
     t.add_argument("--eval-cells", type=int, default=65536)
+    t.add_argument("--coupled-seeds", type=int, nargs="*", default=[],
+                   help="hot-spot seeds of coupled runs whose stiff cells join the training set")
+    t.add_argument("--coupled-steps", type=int, default=5000)
+    t.add_argument("--coupled-every", type=int, default=2)
     e = sub.add_parser("eval")
     e.add_argument("--cells", type=int, default=65536)
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -231,7 +287,9 @@ def main(argv=None):
     log = lambda m: print(m, flush=True)
     mech = Mechanism()
     if a.cmd == "train":
-        s, params, norm, meta = train(mech, a.pairs, a.epochs, a.batch, a.lr, log=log)
+        s, params, norm, meta = train(mech, a.pairs, a.epochs, a.batch, a.lr, log=log,
+                                      coupled_seeds=a.coupled_seeds, coupled_steps=a.coupled_steps,
+                                      coupled_every=a.coupled_every)
         s.set_thresholds()
         if a.eval_cells:
             meta["eval"] = evaluate(mech, s, a.eval_cells, log=log)

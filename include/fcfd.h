@@ -199,8 +199,8 @@ int fcfd_sur_infer_batch(fcfd_surrogate *s, const fcfd_opts *o, int n, const dou
 /* solve_source of a context through the surrogate (NULL: the FP64 integrator on every cell).
  * hrr / hrr_dot of fcfd_solve_source are not available in surrogate mode. */
 int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s);
-/* Per interior cell of the last surrogate solve_source: 2 explicit screen, 1 surrogate, 0 FP64
- * implicit integrator; *n_fallback = number of cells with 0. */
+/* Per interior cell of the last solve_source: 2 explicit screen, 1 surrogate, 0 FP64 implicit
+ * integrator; *n_fallback = number of cells with 0. */
 int fcfd_source_accepted(fcfd_ctx *c, int *per_cell, int *n_fallback);
 
 /* Plain BF16 GEMM C (FP32) = A B on device arrays with element (m, k) of A at A[m*sam + k*sak]
