@@ -35,13 +35,6 @@ struct LaneShared {
 template <int NSP>
 constexpr int slots() { return (NSP + NW - 1) / NW; }
 
-__device__ __forceinline__ Nasa nasa_lane(const DevThermo *__restrict__ m, int k, double T) {
-    const double *lo = m->alo[k], *hi = m->ahi[k];
-    const bool up = T > m->tmid[k];
-    return Nasa{up ? hi[0] : lo[0], up ? hi[1] : lo[1], up ? hi[2] : lo[2], up ? hi[3] : lo[3],
-                up ? hi[4] : lo[4], up ? hi[5] : lo[5], up ? hi[6] : lo[6]};
-}
-
 // Sum over the four waves of one value per lane, in wave order; `slot` is a [NW][64] LDS area that
 // no wave uses between the surrounding barriers.
 __device__ __forceinline__ double sum4(double *slot, int w, int lane, double v) {
@@ -73,7 +66,7 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
             const double ck = rho * yj / th->wt[k];
             sh.C[k * 64 + lane] = ck;
             ctot_w = ctot_w + ck;
-            const Nasa a = nasa_lane(th, k, T);
+            const Nasa a = nasa(th, k, T);
             uk[j] = h_mol(a, p) - p.rut;
             cvm_w = cvm_w + yj * (cp_mol(a, p) - th->ru) / th->wt[k];
             const double t2 = T * T, t3 = T * t2, t4 = t2 * t2;     // CKSMH, as smh_species
@@ -147,7 +140,7 @@ __device__ __forceinline__ void energy_lane(const DevThermo *__restrict__ th, do
     for (int j = 0; j < NS; ++j) {
         const int k = w + NW * j;
         if (k < NSP) {
-            const Nasa a = nasa_lane(th, k, T);
+            const Nasa a = nasa(th, k, T);
             eu = eu + y[j] * (h_mol(a, p) - p.rut) / th->wt[k];
             ec = ec + y[j] * (cp_mol(a, p) - th->ru) / th->wt[k];
         }

@@ -34,9 +34,14 @@ struct Nasa {
     double a0, a1, a2, a3, a4, a5, a6;
 };
 
+// Both coefficient sets are read (with a wave-uniform k these are scalar loads) and the range is
+// selected per lane; selecting the pointer instead would turn every coefficient read into a
+// per-lane memory access.
 __device__ __forceinline__ Nasa nasa(const DevThermo *__restrict__ m, int k, double T) {
-    const double *a = (T > m->tmid[k]) ? m->ahi[k] : m->alo[k];
-    return Nasa{a[0], a[1], a[2], a[3], a[4], a[5], a[6]};
+    const double *lo = m->alo[k], *hi = m->ahi[k];
+    const bool up = T > m->tmid[k];
+    return Nasa{up ? hi[0] : lo[0], up ? hi[1] : lo[1], up ? hi[2] : lo[2], up ? hi[3] : lo[3],
+                up ? hi[4] : lo[4], up ? hi[5] : lo[5], up ? hi[6] : lo[6]};
 }
 
 // CKHML (cklib.f:3551-3555): molar enthalpy, erg/mol.

@@ -17,30 +17,39 @@ struct Side {
     double rho, u, T, sy, p, a, E;
 };
 
-template <int NSP>
+// y[k * stride] holds the reconstructed Y_k (a register array with stride 1 or an LDS column).
+template <int NSP, int STRIDE = 1>
 __device__ __forceinline__ Side side_state(const DevThermo *__restrict__ m, double rho, double u,
-                                           double T, const double (&y)[NSP]) {
+                                           double T, const double *y) {
     Side s;
     s.rho = rho; s.u = u; s.T = T;
     // adjust_sum(Y) denominator (Y itself is renormalised lazily where QcL/R needs it).
     double sy = 0.0, syw = 0.0;
 #pragma unroll
     for (int k = 0; k < NSP; ++k) {
-        sy = sy + fmax(y[k], 0.0);
-        syw = syw + y[k] / m->wt[k];          // CKYTX on the raw reconstructed Y
+        sy = sy + fmax(y[k * STRIDE], 0.0);
+        syw = syw + y[k * STRIDE] / m->wt[k];          // CKYTX on the raw reconstructed Y
     }
     s.sy = sy;
-    double x[NSP];
+    // mole fractions, clipped and renormalised (SRC/flux.f90:250-253), formed on the fly in the
+    // mixture sums instead of being held in an array: each sum keeps its species order, and the
+    // per-species quotient is the same expression every time it is formed
     double sx = 0.0;
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) {
-        x[k] = fmax(y[k] / (syw * m->wt[k]), 0.0);
-        sx = sx + x[k];
-    }
+    for (int k = 0; k < NSP; ++k) sx = sx + fmax(y[k * STRIDE] / (syw * m->wt[k]), 0.0);
+    const TPow tp = tpow(T, m->ru);
+    double wtm = 0.0;
+    MixSums ms{0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < NSP; ++k) x[k] = x[k] / sx;
-    const double wtm = mix_w<NSP>(m, x);
-    const MixSums ms = mix_sums<NSP>(m, x, T, true, true, true);
+    for (int k = 0; k < NSP; ++k) {
+        const double xk = fmax(y[k * STRIDE] / (syw * m->wt[k]), 0.0) / sx;
+        wtm = wtm + xk * m->wt[k];                                   // mix_w
+        const Nasa a = nasa(m, k, T);                                // mix_sums
+        ms.ubml = ms.ubml + xk * (h_mol(a, tp) - tp.rut);
+        const double c = cp_mol(a, tp);
+        ms.cpbml = ms.cpbml + xk * c;
+        ms.cvbml = ms.cvbml + xk * (c - m->ru);
+    }
     s.p = (rho * 1.e-3) * m->ru * T / wtm * 0.1;                 // pressure(), CKPX
     const double e = ms.ubml * 1.e-4 / wtm;                      // specific_energy()
     s.E = rho * (e + 0.5 * (u * u));
@@ -50,28 +59,55 @@ __device__ __forceinline__ Side side_state(const DevThermo *__restrict__ m, doub
 }
 
 // ---------------------------------------------------------------------------------------------
-// calc_flux_advection: one thread per interface.
+// calc_flux_advection: TWO LANES PER INTERFACE.  Lanes 2j and 2j+1 of a wave share interface j
+// of the wave's range: lane parity s = 0 builds the left state, s = 1 the right state (each
+// reconstructs the stencil, keeps its side's values and runs qfToQc / pressure / sonic speed on
+// them); the two side records and, per species, the two mass fractions are exchanged between the
+// pair (side records by a lane shuffle, mass fractions through LDS), both lanes evaluate the same
+// Riemann branch, and the flux components
+// are written alternately by the two lanes.  Every output is the same expression of the same
+// operands as in a one-lane evaluation, so the rounding is unchanged.
 // ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double pair_other(double v) { return __shfl_xor(v, 1, 64); }
+
+__device__ __forceinline__ Side pair_other(const Side &me) {
+    Side o;
+    o.rho = pair_other(me.rho); o.u = pair_other(me.u); o.T = pair_other(me.T);
+    o.sy = pair_other(me.sy); o.p = pair_other(me.p); o.a = pair_other(me.a); o.E = pair_other(me.E);
+    return o;
+}
+
 template <int NSP, int RK>
-__global__ void __launch_bounds__(64) k_flux(FluxArgs a) {
-    const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
+    const long g2 = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long g = g2 >> 1;                               // interface (pairs never straddle a wave)
+    const int s = int(threadIdx.x & 1);                   // 0: left state, 1: right state
     const int nif = a.ncell + 1;
-    if (g >= long(a.nblk) * nif) return;
+    if (g >= long(a.nblk) * nif) return;                  // both lanes of a pair leave together
     const int b = int(g / nif), j = int(g % nif);
     const long c0 = long(b) * a.S + a.lv + j - 1;      // left cell of the interface
     const DevThermo *__restrict__ m = a.thermo;
     const double *__restrict__ q = a.qf;
 
-    double rl, rr, ul, ur, tl, tr;
-    reconstruct<RK>(q + 0 * a.P, c0, a.muscl, rl, rr);
-    reconstruct<RK>(q + 1 * a.P, c0, a.muscl, ul, ur);
-    reconstruct<RK>(q + 2 * a.P, c0, a.muscl, tl, tr);
-    double yl[NSP], yr[NSP];
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) reconstruct<RK>(q + (3 + k) * a.P, c0, a.muscl, yl[k], yr[k]);
-
-    const Side L = side_state<NSP>(m, rl, ul, tl, yl);
-    const Side R = side_state<NSP>(m, rr, ur, tr, yr);
+    double l, r;
+    reconstruct<RK>(q + 0 * a.P, c0, a.muscl, l, r);
+    const double rho_s = s ? r : l;
+    reconstruct<RK>(q + 1 * a.P, c0, a.muscl, l, r);
+    const double u_s = s ? r : l;
+    reconstruct<RK>(q + 2 * a.P, c0, a.muscl, l, r);
+    const double T_s = s ? r : l;
+    // reconstructed mass fractions of this lane's side, species-major in LDS (column = lane)
+    __shared__ double ys[NSP * 64];
+    double *y = ys + threadIdx.x;
+    for (int k = 0; k < NSP; ++k) {
+        reconstruct<RK>(q + (3 + k) * a.P, c0, a.muscl, l, r);
+        y[k * 64] = s ? r : l;
+    }
+    const double *yo_col = ys + (threadIdx.x ^ 1);     // the pair partner's side
+    const Side me = side_state<NSP, 64>(m, rho_s, u_s, T_s, y);
+    const Side ot = pair_other(me);
+    const Side L = s ? ot : me;
+    const Side R = s ? me : ot;
 
     // physical fluxes of mass, momentum, energy (SRC/flux.f90:270-280)
     const double fl0 = L.rho * L.u, fr0 = R.rho * R.u;
@@ -81,20 +117,25 @@ __global__ void __launch_bounds__(64) k_flux(FluxArgs a) {
     const long PF = a.PF;
     // Fplus accumulates (SRC/flux.f90:358): on top of the transport flux when there is one
     const bool acc = a.accumulate != 0;
-    auto put = [acc](double &d, double v) { d = acc ? d + v : v; };
+    // component v is written by lane parity v & 1
+    auto put = [acc, s](double &d, int v, double val) {
+        if ((v & 1) == s) d = acc ? d + val : val;
+    };
 
     if (a.riemann == 0) {                                         // LxF (SRC/flux.f90:291-294)
         const double lL = fmax(fmax(fabs(L.u - L.a), fabs(L.u)), fabs(L.u + L.a));
         const double lR = fmax(fmax(fabs(R.u - R.a), fabs(R.u)), fabs(R.u + R.a));
         const double al = fmax(lL, lR);
-        put(F[0], 0.5 * (fl0 + fr0 - al * (R.rho - L.rho)));
-        put(F[PF], 0.5 * (fl1 + fr1 - al * (R.rho * R.u - L.rho * L.u)));
-        put(F[2 * PF], 0.5 * (fl2 + fr2 - al * (R.E - L.E)));
+        put(F[0], 0, 0.5 * (fl0 + fr0 - al * (R.rho - L.rho)));
+        put(F[PF], 1, 0.5 * (fl1 + fr1 - al * (R.rho * R.u - L.rho * L.u)));
+        put(F[2 * PF], 2, 0.5 * (fl2 + fr2 - al * (R.E - L.E)));
 #pragma unroll
         for (int k = 0; k < NSP; ++k) {
-            const double QL = L.rho * (fmax(yl[k], 0.0) / L.sy), QR = R.rho * (fmax(yr[k], 0.0) / R.sy);
-            const double fL = L.rho * yl[k] * L.u, fR = R.rho * yr[k] * R.u;
-            put(F[(3 + k) * PF], 0.5 * (fL + fR - al * (QR - QL)));
+            const double ym = y[k * 64], yo = yo_col[k * 64];
+            const double yl = s ? yo : ym, yr = s ? ym : yo;
+            const double QL = L.rho * (fmax(yl, 0.0) / L.sy), QR = R.rho * (fmax(yr, 0.0) / R.sy);
+            const double fL = L.rho * yl * L.u, fR = R.rho * yr * R.u;
+            put(F[(3 + k) * PF], 3 + k, 0.5 * (fL + fR - al * (QR - QL)));
         }
     } else {
         const double rt = sqrt(R.rho / L.rho);
@@ -106,13 +147,15 @@ __global__ void __launch_bounds__(64) k_flux(FluxArgs a) {
             auto hll = [&](double fL, double fR, double QL, double QR) {
                 return br == 0 ? fL : (br == 1 ? (sR * fL - sL * fR + sL * sR * (QR - QL)) / (sR - sL) : fR);
             };
-            put(F[0], hll(fl0, fr0, L.rho, R.rho));
-            put(F[PF], hll(fl1, fr1, L.rho * L.u, R.rho * R.u));
-            put(F[2 * PF], hll(fl2, fr2, L.E, R.E));
+            put(F[0], 0, hll(fl0, fr0, L.rho, R.rho));
+            put(F[PF], 1, hll(fl1, fr1, L.rho * L.u, R.rho * R.u));
+            put(F[2 * PF], 2, hll(fl2, fr2, L.E, R.E));
 #pragma unroll
             for (int k = 0; k < NSP; ++k) {
-                const double QL = L.rho * (fmax(yl[k], 0.0) / L.sy), QR = R.rho * (fmax(yr[k], 0.0) / R.sy);
-                put(F[(3 + k) * PF], hll(L.rho * yl[k] * L.u, R.rho * yr[k] * R.u, QL, QR));
+                const double ym = y[k * 64], yo = yo_col[k * 64];
+                const double yl = s ? yo : ym, yr = s ? ym : yo;
+                const double QL = L.rho * (fmax(yl, 0.0) / L.sy), QR = R.rho * (fmax(yr, 0.0) / R.sy);
+                put(F[(3 + k) * PF], 3 + k, hll(L.rho * yl * L.u, R.rho * yr * R.u, QL, QR));
             }
         } else {                                                  // HLLC (SRC/flux.f90:321-354)
             const double sM = (L.p - R.p + R.rho * R.u * (sR - R.u) - L.rho * L.u * (sL - L.u)) /
@@ -123,29 +166,32 @@ __global__ void __launch_bounds__(64) k_flux(FluxArgs a) {
             const Side S = left ? L : R;
             const double f0 = left ? fl0 : fr0, f1 = left ? fl1 : fr1, f2 = left ? fl2 : fr2;
             if (br == 0 || br == 3) {                             // supersonic: upwind flux
-                put(F[0], f0);
-                put(F[PF], f1);
-                put(F[2 * PF], f2);
+                put(F[0], 0, f0);
+                put(F[PF], 1, f1);
+                put(F[2 * PF], 2, f2);
 #pragma unroll
-                for (int k = 0; k < NSP; ++k) put(F[(3 + k) * PF], S.rho * (left ? yl[k] : yr[k]) * S.u);
+                for (int k = 0; k < NSP; ++k) {
+                    const double yk = (left == (s == 0)) ? y[k * 64] : yo_col[k * 64];
+                    put(F[(3 + k) * PF], 3 + k, S.rho * yk * S.u);
+                }
             } else {                                              // star state of the upwind side
                 const double sK = left ? sL : sR;
                 const double c = S.rho * (sK - S.u) / (sK - sM);
                 const double v2 = S.E / S.rho + (sM - S.u) * (sM + S.p / (S.rho * (sK - S.u)));
-                put(F[0], f0 + sK * (c * 1.0 - S.rho));
-                put(F[PF], f1 + sK * (c * sM - S.rho * S.u));
-                put(F[2 * PF], f2 + sK * (c * v2 - S.E));
+                put(F[0], 0, f0 + sK * (c * 1.0 - S.rho));
+                put(F[PF], 1, f1 + sK * (c * sM - S.rho * S.u));
+                put(F[2 * PF], 2, f2 + sK * (c * v2 - S.E));
 #pragma unroll
                 for (int k = 0; k < NSP; ++k) {
-                    const double yk = left ? yl[k] : yr[k];
+                    const double yk = (left == (s == 0)) ? y[k * 64] : yo_col[k * 64];
                     const double Qk = S.rho * (fmax(yk, 0.0) / S.sy);
                     const double fk = S.rho * yk * S.u;
-                    put(F[(3 + k) * PF], fk + sK * (c * (Qk / S.rho) - Qk));
+                    put(F[(3 + k) * PF], 3 + k, fk + sK * (c * (Qk / S.rho) - Qk));
                 }
             }
         }
     }
-    if (a.godunov) {                                              // SRC/flux.f90:360-367
+    if (a.godunov && s == 0) {                                    // SRC/flux.f90:360-367
         const double rt = sqrt(R.rho / L.rho);
         double *G = a.godunov + g;
         G[0] = sqrt(R.rho * L.rho);
@@ -195,7 +241,7 @@ __device__ __forceinline__ void ghost_source(const StageArgs &a, int b, int i, i
 // interpolation_block / QcToqf on every column (:201-215).  mode 0: Q = Qin - dt/dx*dF;
 // mode 1: Q = h1*Q0 + h2*(Qin - dt/dx*dF); mode 2: no update (ghost fill + QcToqf only).
 template <int NSP>
-__global__ void __launch_bounds__(64) k_stage_end(StageArgs a) {
+__global__ void __launch_bounds__(64, 2) k_stage_end(StageArgs a) {
     const long c = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= long(a.nblk) * a.S) return;
     const int b = int(c / a.S), i = int(c % a.S) - a.lv;
@@ -453,7 +499,7 @@ static int recon_kind(int flag_acu, int flag_weno) {
 
 template <int NSP>
 static hipError_t launch_flux_nsp(const FluxArgs &a, int rk, hipStream_t st) {
-    const long n = long(a.nblk) * (a.ncell + 1);
+    const long n = 2 * long(a.nblk) * (a.ncell + 1);            // two lanes per interface
     const dim3 grid(unsigned((n + 63) / 64)), blk(64);
     switch (rk) {
         case R_FIRST: hipLaunchKernelGGL((k_flux<NSP, R_FIRST>), grid, blk, 0, st, a); break;

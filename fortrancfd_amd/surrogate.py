@@ -75,7 +75,8 @@ def coupled_pairs(mech, seeds, steps, every=2, nblk=16, ncell=4096, dx=25e-6, cf
     """(state, FP64 result) pairs of the cells the explicit screen sends to the implicit integrator
     in coupled reacting runs: the 65536-cell CH4/air domain with three 32-cell hot spots of the
     given seeds, stepped with the FP64 chemistry; every `every`-th step the stiff cells' states
-    before and after solve_source are recorded.  Returns host arrays (rho, dt, T0, Y0, T1, Y1)."""
+    before and after solve_source are recorded (the steps in between replay the captured step).
+    Returns host arrays (rho, dt, T0, Y0, T1, Y1)."""
     from .lib import Solver
     out = {k: [] for k in ("rho", "dt", "T0", "Y0", "T1", "Y1")}
     t0 = time.time()
@@ -86,26 +87,27 @@ def coupled_pairs(mech, seeds, steps, every=2, nblk=16, ncell=4096, dx=25e-6, cf
         lv = s.lv
         inner = (slice(None), slice(lv, lv + ncell))
         n0 = sum(len(v) for v in out["T0"])
-        for step in range(steps):
+        t = 0.0
+        for step in range(0, steps, every):
+            if every > 1:
+                t = s.advance(cfl, every - 1, t)         # device-resident steps in between
             dt = s.calc_dt(cfl)
             s.solve_fluid(dt)
-            rec = step % every == 0
-            if rec:
-                q0 = s.download()[1][inner].reshape(-1, mech.nequ)
+            q0 = s.download()[1][inner].reshape(-1, mech.nequ)
             s.solve_source(dt, diagnostics=False)
-            if rec:
-                route, _ = s.source_accepted()
-                stiff = route == 0
-                if stiff.any():
-                    q1 = s.download()[1][inner].reshape(-1, mech.nequ)
-                    y0 = np.maximum(q0[stiff, 3:], 0.0)
-                    out["rho"].append(q0[stiff, 0])
-                    out["dt"].append(np.full(stiff.sum(), dt))
-                    out["T0"].append(q0[stiff, 2])
-                    out["Y0"].append(y0 / y0.sum(axis=1, keepdims=True))
-                    out["T1"].append(q1[stiff, 2])
-                    out["Y1"].append(q1[stiff, 3:])
+            route, _ = s.source_accepted()
+            stiff = route == 0
+            if stiff.any():
+                q1 = s.download()[1][inner].reshape(-1, mech.nequ)
+                y0 = np.maximum(q0[stiff, 3:], 0.0)
+                out["rho"].append(q0[stiff, 0])
+                out["dt"].append(np.full(stiff.sum(), dt))
+                out["T0"].append(q0[stiff, 2])
+                out["Y0"].append(y0 / y0.sum(axis=1, keepdims=True))
+                out["T1"].append(q1[stiff, 2])
+                out["Y1"].append(q1[stiff, 3:])
             s.refresh()
+            t += dt
         n1 = sum(len(v) for v in out["T0"])
         log(f"  coupled run seed {seed}: {steps} steps, {n1 - n0} stiff-cell pairs "
             f"({time.time() - t0:.1f} s)")
