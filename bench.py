@@ -251,6 +251,13 @@ def bench_inert(args, mech, nblk, ncell, dx, nhot, names):
     s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1)
     s.init_cells(cells)
     cfl = 0.5
+    if args.calib_copy_mb:
+        # a device-to-device copy of known size: the reference point that scales the hardware
+        # byte counters (FETCH_SIZE / WRITE_SIZE) of a profiled run to bytes
+        src = torch.empty(args.calib_copy_mb << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     t = s.advance(cfl, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -343,6 +350,8 @@ def main():
     ap.add_argument("--cells", type=int, default=1 << 20)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="inert workloads: one device copy of this many MiB before the steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every block of the domain lives on one device and this "

@@ -287,7 +287,7 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
             ApplyArgs ap{};
             ap.layout = 0; ap.qf = c->qf; ap.P = c->P; ap.ncell = c->ncell; ap.lv = c->lv; ap.S = c->S;
             ap.route = c->d_accepted;
-            return s->accept(ap, c->d_list, c->d_list_n, n, c->stream);
+            return s->accept(ap, a, c->d_list, c->d_list_n, n, c->stream);
         });
         if (e != hipSuccess) return e;
         e = timed(c, 3, [&]() { return s->fallback(a, chem_workgroups(c->mech), c->stream); });

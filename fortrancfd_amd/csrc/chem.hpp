@@ -66,6 +66,13 @@ hipError_t launch_chem(const ChemArgs &a, int nsp, int n_workgroups, hipStream_t
 // appended to a.stiff (a.stiff_n must be zero before the launch).
 hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st);
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
+// FP64 verification of surrogate answers for the cells cand_list[0 .. *ncand): accepted when the
+// trapezoidal-rule residual Y1 - Y0 - dt/2 (f(Y0) + f(Y1)) of the candidate (Y1, T1) = cand[i],
+// weighted by the integrator's tolerances, has an RMS <= tol (1: the integrator's own bar); accepted
+// answers are written to the state, the others appended to the fallback list.
+hipError_t launch_chem_verify(const ChemArgs &a, int nsp, const int *cand_list, const unsigned *ncand,
+                              const double *cand, int rows_max, int *fallback, unsigned *nfallback, double tol,
+                              hipStream_t st);
 // Screen, then SDIRK on the cells the screen rejected; list / list_n are scratch for ncells cells.
 hipError_t launch_chem_full(ChemArgs a, int nsp, int n_workgroups, int *list, unsigned *list_n,
                             hipStream_t st);

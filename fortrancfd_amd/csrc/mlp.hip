@@ -289,10 +289,11 @@ __global__ void k_to_bf16(const float *w, uint16_t *w16, long n) {
 // sum_k dY_k^2 / w_k subject to sum_k a_ek (Y1_k - Y0_k) = 0 (a_ek = atoms of element e per unit
 // mass of species k, w_k = max(Y1_k, 0) + 1e-12; MM x MM normal equations, elements absent from
 // the mixture left out), negative entries are clipped, and T1 follows from conservation of
-// sum_k Y_k u_k(T) / W_k (Newton from the network's T).  Accepted when the inputs were inside the
-// training range, the element residual after projection and clipping is <= resid_tol, and the
-// network temperature agrees with the energy temperature to t_tol; otherwise the cell is appended
-// to the fallback list for the FP64 integrator.
+// sum_k Y_k u_k(T) / W_k (Newton from the network's T).  A candidate when the inputs were inside
+// the training range, the element residual after projection and clipping is <= resid_tol, and the
+// network temperature agrees with the energy temperature to t_tol: it goes to a candidate buffer
+// and list for the FP64 verification (chem_lane.hip); otherwise the cell is appended to the
+// fallback list for the FP64 integrator.
 // ---------------------------------------------------------------------------------------------
 template <int NSP, int MM>
 __global__ void __launch_bounds__(256) k_apply(ApplyArgs a) {
@@ -391,7 +392,7 @@ __global__ void __launch_bounds__(256) k_apply(ApplyArgs a) {
         if (fabs(d) <= 1.0e-12 * T1) break;
     }
     if (!(fabs(T1 - Tnet) <= a.t_tol * T1)) ok = false;
-    if (lane == 0 && a.route) a.route[t] = ok ? 1 : 0;
+    if (lane == 0 && a.route) a.route[t] = 0;
     if (!ok) {
         if (lane == 0) {
             const unsigned q = atomicAdd(a.nfallback, 1u);
@@ -399,12 +400,14 @@ __global__ void __launch_bounds__(256) k_apply(ApplyArgs a) {
         }
         return;
     }
-    if (a.layout == 0) {
-        if (sp) a.qf[(3 + lane) * a.P + col] = y1;
-        if (lane == 0) a.qf[2 * a.P + col] = T1;
-    } else {
-        if (sp) a.Y[long(lane) * a.n + t] = y1;
-        if (lane == 0) a.T[t] = T1;
+    // candidate: verified against the FP64 right-hand side before it may enter the state
+    unsigned q = 0;
+    if (lane == 0) q = atomicAdd(a.ncand, 1u);
+    q = __shfl(q, 0, 64);
+    if (sp) a.cand[long(q) * (NSP + 1) + lane] = y1;
+    if (lane == 0) {
+        a.cand[long(q) * (NSP + 1) + NSP] = T1;
+        a.cand_list[q] = t;
     }
 }
 

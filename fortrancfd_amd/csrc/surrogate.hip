@@ -54,6 +54,8 @@ hipError_t Surrogate::reserve(int rows, bool training) {
     if ((e = grow(in_range, capIR, r)) != hipSuccess) return e;
     if ((e = grow(fb_list, capFB, r)) != hipSuccess) return e;
     if ((e = grow(stiff, capST, r)) != hipSuccess) return e;
+    if ((e = grow(cand_list, capCL, r)) != hipSuccess) return e;
+    if ((e = grow(cand, capCD, r * size_t(mech->host.kk + 1))) != hipSuccess) return e;
     const size_t hr = std::min<size_t>(r, size_t(INFER_CHUNK));
     for (int i = 0; i < 2; ++i)
         if ((e = grow(Hping[i], capH[i], hr * MLP_HID)) != hipSuccess) return e;
@@ -173,7 +175,8 @@ hipError_t Surrogate::network(FeatArgs fa, const int *list, const unsigned *list
     return forward(X, rows, Yh, false, st, list_n);
 }
 
-hipError_t Surrogate::accept(ApplyArgs ap, const int *list, const unsigned *list_n, int rows_max, hipStream_t st) {
+hipError_t Surrogate::accept(ApplyArgs ap, const ChemArgs &ca, const int *list, const unsigned *list_n, int rows_max,
+                             hipStream_t st) {
     ap.thermo = mech->d_thermo;
     ap.norm = d_norm;
     ap.ncf = d_ncf;
@@ -187,9 +190,14 @@ hipError_t Surrogate::accept(ApplyArgs ap, const int *list, const unsigned *list
     ap.resid_tol = resid_tol;
     ap.t_tol = t_tol;
     ap.n = rows_max;
+    ap.ncand = d_ncand;
+    ap.cand_list = cand_list;
+    ap.cand = cand;
     hipError_t e = launch_zero_u32(d_nfb, st);
     if (e != hipSuccess) return e;
-    return mlp_apply(ap, mech->host.kk, st);
+    if ((e = launch_zero_u32(d_ncand, st)) != hipSuccess) return e;
+    if ((e = mlp_apply(ap, mech->host.kk, st)) != hipSuccess) return e;
+    return launch_chem_verify(ca, mech->host.kk, cand_list, d_ncand, cand, rows_max, fb_list, d_nfb, verify_tol, st);
 }
 
 hipError_t Surrogate::fallback(ChemArgs ca, int chem_wg, hipStream_t st) {
@@ -207,7 +215,7 @@ Surrogate::~Surrogate() {
     for (int l = 0; l < 4; ++l) { f(H[l]); f(Pre[l]); }
     for (int i = 0; i < 2; ++i) { f(Hping[i]); f(dZ[i]); }
     f(X); f(Yh); f(Yt); f(dY); f(gpart); f(cs_part); f(loss_part); f(d_loss); f(in_range); f(fb_list);
-    f(stiff); f(d_stiff_n); f(d_nfb); f(d_norm); f(d_ncf);
+    f(stiff); f(d_stiff_n); f(d_nfb); f(d_norm); f(d_ncf); f(cand_list); f(cand); f(d_ncand);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -257,6 +265,7 @@ int fcfd_sur_create(const fcfd_mech *m, fcfd_surrogate **out) {
     HIP_TRY(hipMalloc(&s->d_loss, 4));
     HIP_TRY(hipMalloc(&s->d_nfb, sizeof(unsigned)));
     HIP_TRY(hipMalloc(&s->d_stiff_n, sizeof(unsigned)));
+    HIP_TRY(hipMalloc(&s->d_ncand, sizeof(unsigned)));
     HIP_TRY(hipMalloc(&s->d_norm, sizeof(SurrogateNorm)));
     SurrogateNorm nm{};
     for (int c = 0; c < MLP_INP; ++c) { nm.in_std[c] = 1.0; nm.lo[c] = -1e300; nm.hi[c] = 1e300; }
@@ -338,10 +347,11 @@ int fcfd_sur_set_norm(fcfd_surrogate *s, const double *norm) {
     return FCFD_OK;
 }
 
-int fcfd_sur_set_thresholds(fcfd_surrogate *s, double resid_tol, double t_tol) {
-    if (!s || !(resid_tol > 0.0) || !(t_tol > 0.0)) { set_error("fcfd_sur_set_thresholds: invalid argument"); return FCFD_E_ARG; }
+int fcfd_sur_set_thresholds(fcfd_surrogate *s, double resid_tol, double t_tol, double verify_tol) {
+    if (!s || !(resid_tol > 0.0) || !(t_tol > 0.0) || !(verify_tol > 0.0)) { set_error("fcfd_sur_set_thresholds: invalid argument"); return FCFD_E_ARG; }
     s->resid_tol = resid_tol;
     s->t_tol = t_tol;
+    s->verify_tol = verify_tol;
     return FCFD_OK;
 }
 
@@ -420,7 +430,7 @@ int fcfd_sur_infer_batch(fcfd_surrogate *s, const fcfd_opts *o, int n, const dou
     HIP_TRY(s->network(fa, s->stiff, s->d_stiff_n, n, st));
     ApplyArgs ap{};
     ap.layout = 1; ap.rho = rho; ap.T = T; ap.Y = Y; ap.route = accepted;
-    HIP_TRY(s->accept(ap, s->stiff, s->d_stiff_n, n, st));
+    HIP_TRY(s->accept(ap, ca, s->stiff, s->d_stiff_n, n, st));
     HIP_TRY(s->fallback(ca, chem_workgroups(s->mech), st));
     if (n_fallback) {
         unsigned nf = 0;

@@ -51,6 +51,9 @@ struct ApplyArgs {
     int *route;                 // [n] or null: 1 surrogate accepted, 0 sent to the FP64 integrator
     unsigned *nfallback;        // fallback-list length (zeroed before the launch)
     int *fallback;              // fallback list (cell indices)
+    unsigned *ncand;            // candidate-list length (zeroed before the launch)
+    int *cand_list;             // candidate cells
+    double *cand;               // [candidate][KK + 1]: Y1, T1
     double resid_tol, t_tol;
 };
 

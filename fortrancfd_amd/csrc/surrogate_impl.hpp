@@ -26,15 +26,18 @@ struct Surrogate {
     uint16_t *H[4] = {}, *Hping[2] = {}, *dY = nullptr, *dZ[2] = {};
     float *Pre[4] = {};
     float *gpart = nullptr, *cs_part = nullptr, *loss_part = nullptr, *d_loss = nullptr;
-    int *in_range = nullptr, *fb_list = nullptr, *stiff = nullptr;
-    unsigned *d_nfb = nullptr, *d_stiff_n = nullptr;
+    int *in_range = nullptr, *fb_list = nullptr, *stiff = nullptr, *cand_list = nullptr;
+    unsigned *d_nfb = nullptr, *d_stiff_n = nullptr, *d_ncand = nullptr;
+    double *cand = nullptr;                // [rows][KK + 1] surrogate candidates awaiting verification
     size_t capX = 0, capYh = 0, capYt = 0, capdY = 0, capIR = 0, capFB = 0, capCS = 0, capLP = 0, capST = 0;
+    size_t capCL = 0, capCD = 0;
     size_t capH[2] = {}, capHl[4] = {}, capPre[4] = {}, capdZ[2] = {};
     SurrogateNorm h_norm{};
     SurrogateNorm *d_norm = nullptr;
     double *d_ncf = nullptr;
     double resid_tol = 1.0e-5;           // element-conservation residual bound
     double t_tol = 2.0e-3;               // network T vs energy-consistent T (relative)
+    double verify_tol = 1.0;             // FP64 trapezoidal residual, RMS in integrator tolerances
 
     hipError_t reserve(int rows, bool training);
     hipError_t gemm(const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, int M, int N, int K,
@@ -46,9 +49,11 @@ struct Surrogate {
     hipError_t backward(int rows, hipStream_t st);
     hipError_t adam(float lr, hipStream_t st);
     // Surrogate branch of the chemistry source for the cells list[0 .. *list_n) that the explicit
-    // screen rejected: features + network (rows <= rows_max), acceptance, FP64 SDIRK for the rest.
+    // screen rejected: features + network (rows <= rows_max), acceptance test, FP64 verification of
+    // the candidates, FP64 SDIRK for the rest.
     hipError_t network(FeatArgs fa, const int *list, const unsigned *list_n, int rows_max, hipStream_t st);
-    hipError_t accept(ApplyArgs ap, const int *list, const unsigned *list_n, int rows_max, hipStream_t st);
+    hipError_t accept(ApplyArgs ap, const ChemArgs &ca, const int *list, const unsigned *list_n, int rows_max,
+                      hipStream_t st);
     hipError_t fallback(ChemArgs ca, int chem_wg, hipStream_t st);
     ~Surrogate();
 };

@@ -322,8 +322,9 @@ class Surrogate:
         assert arr.shape == (6, 64)
         _check(lib().fcfd_sur_set_norm(self.handle, _p(arr)))
 
-    def set_thresholds(self, resid_tol=1e-5, t_tol=2e-3):
-        _check(lib().fcfd_sur_set_thresholds(self.handle, C.c_double(resid_tol), C.c_double(t_tol)))
+    def set_thresholds(self, resid_tol=1e-5, t_tol=2e-3, verify_tol=1.0):
+        _check(lib().fcfd_sur_set_thresholds(self.handle, C.c_double(resid_tol), C.c_double(t_tol),
+                                             C.c_double(verify_tol)))
 
     def features(self, rho, dt, T, Y, X, in_range=None, stream=None):
         n = int(T.numel())

@@ -148,12 +148,12 @@ def load(path=DEFAULT_WEIGHTS):
     return z["params"], norm, meta
 
 
-def load_surrogate(mech, path=DEFAULT_WEIGHTS, resid_tol=1e-5, t_tol=2e-3):
+def load_surrogate(mech, path=DEFAULT_WEIGHTS, resid_tol=1e-5, t_tol=2e-3, verify_tol=1.0):
     params, norm, meta = load(path)
     s = Surrogate(mech)
     s.set_params(params)
     s.set_norm(norm)
-    s.set_thresholds(resid_tol, t_tol)
+    s.set_thresholds(resid_tol, t_tol, verify_tol)
     s.meta = meta
     return s
 

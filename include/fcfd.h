@@ -152,12 +152,15 @@ int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double 
  * SRC/source.f90:58 where its answer is accepted) ----------------------------------------------
  * A 56 -> 4 x 256 GELU -> 54 multilayer perceptron with BF16 weights and activations and FP32
  * accumulation.  Inputs per cell: b(Y_k) (KK = 53), T, log(p / 1 atm), log dt, z-scored, with the
- * Box-Cox transform b(Y) = 10 (Y^0.1 - 1); outputs (b(Y1) - b(Y0), T1 - T0) z-scored.  A cell's answer is accepted when every raw
- * input lies in the training range, the element-conservation residual of the final state (after
- * projection onto element conservation and clipping of negative mass fractions) is at most
- * resid_tol relative, and the network temperature agrees with the energy-consistent temperature
- * to t_tol relative; every other cell is integrated by the FP64 integrator.  The final T of an
- * accepted cell comes from conservation of the specific internal energy, as in the FP64 path.
+ * Box-Cox transform b(Y) = 10 (Y^0.1 - 1); outputs (b(Y1) - b(Y0), T1 - T0) z-scored.  Only cells
+ * the explicit screen cannot finish reach the network.  A network answer becomes a candidate when
+ * every raw input lies in the training range, the element-conservation residual of the final state
+ * (after projection onto element conservation and clipping of negative mass fractions) is at most
+ * resid_tol relative, and the network temperature agrees with the energy-consistent temperature to
+ * t_tol relative; a candidate is accepted when its FP64 trapezoidal-rule residual
+ * Y1 - Y0 - dt/2 (f(Y0) + f(Y1)), weighted like the integrator's error norm, has an RMS of at most
+ * verify_tol.  Every other cell is integrated by the FP64 integrator.  The final T of an accepted
+ * cell comes from conservation of the specific internal energy, as in the FP64 path.
  *
  * Parameters (fcfd_sur_nparams() floats) are ordered W0 (64 x 256, row-major, k-major), b0 (256),
  * W1 (256 x 256), b1, W2, b2, W3, b3, W4 (256 x 64), b4 (64); input rows 56..63 of W0 and output
@@ -173,7 +176,9 @@ int fcfd_sur_set_params(fcfd_surrogate *s, const float *params);   /* host; rese
 int fcfd_sur_get_params(fcfd_surrogate *s, float *params);
 int fcfd_sur_get_grads(fcfd_surrogate *s, float *grads);           /* of the last train step */
 int fcfd_sur_set_norm(fcfd_surrogate *s, const double *norm);
-int fcfd_sur_set_thresholds(fcfd_surrogate *s, double resid_tol, double t_tol);
+/* Acceptance bounds: element residual (default 1e-5), network-vs-energy T (2e-3, relative) and the
+ * FP64 verification residual (1: the integrator's own tolerance). */
+int fcfd_sur_set_thresholds(fcfd_surrogate *s, double resid_tol, double t_tol, double verify_tol);
 /* Features of n cells (device rho[n] kg/m^3, dt[n], T[n], Y[KK][n]) into X (rows padded to a
  * multiple of 64 with zeros); in_range[n] (may be NULL) = 1 where every input is in range. */
 int fcfd_sur_features(fcfd_surrogate *s, int n, const double *rho, const double *dt,

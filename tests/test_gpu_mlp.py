@@ -189,7 +189,7 @@ def test_accepted_cells_conserve_elements_and_energy(torch, mech, orc):
     norm = {"in_mean": np.zeros(64), "in_std": np.ones(64), "lo": np.full(64, -1e300),
             "hi": np.full(64, 1e300), "out_mean": out_mean, "out_std": np.ones(64)}
     s.set_norm(norm)
-    s.set_thresholds(1e-5, 1e-1)
+    s.set_thresholds(1e-5, 1e-1, 1e30)          # verification off: every candidate accepted
     n = 512
     rho, dt, T, Y = config3_cells(mech, torch, n, 22)
     T0, Y0 = T.cpu().numpy(), Y.cpu().numpy().T.copy()
@@ -250,3 +250,28 @@ def test_features_and_targets_match_host(torch, mech):
     reft = (out - norm["out_mean"][:54]) / norm["out_std"][:54]
     assert np.all(tg[n:] == 0) and np.all(tg[:, 54:] == 0)
     np.testing.assert_allclose(tg[:n, :54], reft, rtol=1e-5, atol=1e-5)
+
+
+def test_unverified_candidates_take_the_fp64_path_bitwise(torch, mech):
+    """With the FP64 verification at the integrator's tolerance, candidates that are not valid
+    steps (the shifted identity network above) are rejected and every such cell is integrated by
+    the FP64 path: the result equals the plain integrator bit for bit."""
+    from fortrancfd_amd.lib import Surrogate
+    s = Surrogate(mech)
+    lay = M.unpack(M.init_params(2))
+    lay[4][0][:] = 0
+    s.set_params(M.pack(lay))
+    out_mean = np.zeros(64)
+    out_mean[mech.names.index("O2")] = 1e-4
+    out_mean[mech.names.index("N2")] = -1e-4
+    s.set_norm({"in_mean": np.zeros(64), "in_std": np.ones(64), "lo": np.full(64, -1e300),
+                "hi": np.full(64, 1e300), "out_mean": out_mean, "out_std": np.ones(64)})
+    s.set_thresholds(1e-5, 1e-1, 1.0)
+    rho, dt, T, Y = config3_cells(mech, torch, 512, 22)
+    T2, Y2 = T.clone(), Y.clone()
+    acc = torch.zeros(512, dtype=torch.int32, device="cuda")
+    print("infer_batch identity network with verification, 512 cells", flush=True)
+    s.infer_batch(rho, dt, T, Y, accepted=acc)
+    assert not (acc.cpu().numpy() == 1).any()
+    mech.chem_batch_device(rho, dt, T2, Y2)
+    assert torch.equal(T, T2) and torch.equal(Y, Y2)
