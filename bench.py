@@ -119,8 +119,36 @@ def fluid_bytes(nequ, ntot, nblk, ncell):
     return 8.0 * nequ * (ntot + nblk * (ncell + 1))
 
 
-def roofline_for(kern, mech, ntot, nblk, ncell, screen_cells):
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE / WRITE_SIZE summary of this
+    workload (profiles/<workload>_hbm_traffic.json, two rocprofv3 --pmc runs scaled by a copy of
+    known size), or None."""
+    path = os.path.join(REPO, "profiles", TRAFFIC_PROFILES.get(workload, ""))
+    if not os.path.isfile(path):
+        return None, None
+    kernels = json.load(open(path))["kernels"]
+    short = kernel.split(" ")[0] + "<"
+    for name, rec in kernels.items():
+        if short in name:
+            return rec["read_bytes"] + rec["write_bytes"], os.path.relpath(path, REPO)
+    return None, None
+
+
+TRAFFIC_PROFILES = {"inert1m": "fluid_inert1m_hbm_traffic.json",
+                    "coupled65536": "coupled65536_hbm_traffic.json"}
+
+
+def roofline_for(kern, mech, ntot, nblk, ncell, screen_cells, workload=""):
     """Roofline of the kernel with the largest total time in the timed pass."""
+    roof = _roofline_for(kern, mech, ntot, nblk, ncell, screen_cells)
+    traffic, src = measured_traffic(workload, roof["kernel"])
+    if traffic is not None:
+        roof["traffic"] = traffic
+        roof["traffic_source"] = src
+    return roof
+
+
+def _roofline_for(kern, mech, ntot, nblk, ncell, screen_cells):
     name, rec = max(kern.items(), key=lambda kv: kv[1]["total_ms"])
     avg_s = rec["avg_ms"] / 1e3
     if name == "k_flux":
@@ -206,6 +234,13 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
     route, nfb = s_sur.source_accepted()
     s_fp, el_fp, t_fp = run(None)
     _, qf_fp = s_fp.download()
+    route_fp, _ = s_fp.source_accepted()
+    st_fp = s_fp.chem_stats()[route_fp == 0]
+    fp64_stiff = {"cells": int(st_fp.shape[0])}
+    if st_fp.shape[0]:
+        fp64_stiff.update({"steps_mean": float(st_fp[:, 0].mean()), "steps_max": int(st_fp[:, 0].max()),
+                           "rejects_mean": float(st_fp[:, 1].mean()), "rejects_max": int(st_fp[:, 1].max()),
+                           "newton_mean": float(st_fp[:, 2].mean()), "newton_max": int(st_fp[:, 2].max())})
     err_sur = errors(qf_sur, qf_fp)
     t_err, y_err = err_sur["T_rel_max"], err_sur["major_species_abs_max"]
     s_u, el_u, t_u = run(sur_u)
@@ -221,7 +256,7 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
     kern = kernel_table(s_sur)
     s_sur.set_timing(False)
     screen_cells = ntot
-    roof = roofline_for(kern, mech, ntot, nblk, ncell, screen_cells)
+    roof = roofline_for(kern, mech, ntot, nblk, ncell, screen_cells, args.workload)
 
     cpu = None
     if not args.no_cpu:
@@ -239,7 +274,8 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
         "value": value,
         "unit": "cell-steps/s",
         "ms_per_step": el_sur / args.steps * 1e3,
-        "fp64": {"value": fp64_value, "ms_per_step": el_fp / args.steps * 1e3, "sim_time_s": t_fp},
+        "fp64": {"value": fp64_value, "ms_per_step": el_fp / args.steps * 1e3, "sim_time_s": t_fp,
+                 "implicit_cells_last_step": fp64_stiff},
         "speedup_vs_fp64": value / fp64_value,
         "error_vs_fp64": dict(err_sur, major_species=list(MAJOR), after_steps=args.warmup + args.steps),
         "chemistry_routes_last_step": {"explicit_screen": int((route == 2).sum()),
@@ -292,7 +328,8 @@ def bench_inert(args, mech, nblk, ncell, dx, nhot, names):
     s.advance(cfl, min(args.steps, 200), t)
     kern = kernel_table(s)
     s.set_timing(False)
-    roof = roofline_for({k: v for k, v in kern.items() if k == "k_flux"}, mech, ntot, nblk, ncell, 0)
+    roof = roofline_for({k: v for k, v in kern.items() if k == "k_flux"}, mech, ntot, nblk, ncell, 0,
+                        args.workload)
     cpu = None
     if not args.no_cpu:
         secs = reference_cpu_baseline(cells, nblk, ncell, dx, args.cpu_steps, cfl)
