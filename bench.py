@@ -135,7 +135,8 @@ def measured_traffic(workload, kernel):
 
 
 TRAFFIC_PROFILES = {"inert1m": "fluid_inert1m_hbm_traffic.json",
-                    "coupled65536": "coupled65536_hbm_traffic.json"}
+                    "inert65536": "fluid_inert65536_hbm_traffic.json",
+                    "coupled65536": "fluid_inert65536_hbm_traffic.json"}
 
 
 def roofline_for(kern, mech, ntot, nblk, ncell, screen_cells, workload=""):

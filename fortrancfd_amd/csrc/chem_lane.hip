@@ -14,6 +14,8 @@
 // partial sums are combined in a fixed order.  Species-wise work (thermo, state update, norms) is
 // split over the waves by k = wave mod 4; cell-wise scalars (T, sums) are combined through LDS.
 // Reaction tables are read with a wave-uniform reaction index (scalar loads).
+#include <algorithm>
+
 #include "chem.hpp"
 #include "chem_device.hpp"
 #include "fcfd_device.hpp"
@@ -324,13 +326,23 @@ struct VerifyArgs {
 };
 
 template <int NSP>
+__device__ __forceinline__ void verify_block(const ChemArgs &a, const VerifyArgs &v, LaneShared<NSP> &sh, int w,
+                                             int lane, int item0, int nitems);
+
+template <int NSP>
 __global__ void __launch_bounds__(256) k_chem_verify(ChemArgs a, VerifyArgs v) {
-    constexpr int NS = slots<NSP>();
     __shared__ LaneShared<NSP> sh;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int item0 = blockIdx.x * 64;
     const int nitems = int(*v.ncand);
-    if (item0 >= nitems) return;                                 // uniform per workgroup
+    // a small grid strides over the candidate list (its length is only known on the device)
+    for (int item0 = blockIdx.x * 64; item0 < nitems; item0 += gridDim.x * 64)
+        verify_block<NSP>(a, v, sh, w, lane, item0, nitems);
+}
+
+template <int NSP>
+__device__ __forceinline__ void verify_block(const ChemArgs &a, const VerifyArgs &v, LaneShared<NSP> &sh, int w,
+                                             int lane, int item0, int nitems) {
+    constexpr int NS = slots<NSP>();
     const DevThermo *__restrict__ th = a.thermo;
     const DevKin *__restrict__ K = a.kin;
     const int item = item0 + lane;
@@ -427,7 +439,8 @@ __global__ void __launch_bounds__(256) k_chem_verify(ChemArgs a, VerifyArgs v) {
 hipError_t launch_chem_verify(const ChemArgs &a, int nsp, const int *cand_list, const unsigned *ncand,
                               const double *cand, int rows_max, int *fallback, unsigned *nfallback, double tol,
                               hipStream_t st) {
-    const unsigned nb = unsigned((rows_max + 63) / 64);
+    // at most one workgroup per CU (each holds 163 KB of LDS); the kernel strides over the list
+    const unsigned nb = unsigned(std::min((rows_max + 63) / 64, 256));
     if (nb == 0) return hipSuccess;
     VerifyArgs v{cand_list, ncand, cand, fallback, nfallback, tol};
 #define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_verify<N>), dim3(nb), dim3(256), 0, st, a, v); return hipGetLastError(); }
