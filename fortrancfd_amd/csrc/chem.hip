@@ -13,6 +13,8 @@
 // lanes run over reactions (rates, Jacobian scatter) and species (thermo, production rates); the
 // 54x54 Newton matrix lives one row per lane in VGPRs and is factored by pivoted elimination
 // across lanes; vectors of the state are distributed one component per lane.
+#include <algorithm>
+
 #include "chem.hpp"
 #include "chem_device.hpp"
 #include "fcfd_device.hpp"
@@ -47,83 +49,137 @@ __device__ __forceinline__ double bcast(double v, int src) {
     return __hiloint2double(hi, lo);
 }
 
-// Lane holding the largest value; ties go to the lower lane (the CPU restatement's order).
-__device__ __forceinline__ int wave_argmax(double v) {
-    int bi = lane_id();
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(v, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov > v || (ov == v && oi < bi)) { v = ov; bi = oi; }
-    }
-    return __builtin_amdgcn_readfirstlane(bi);
+// One DPP move of a double (both halves with the same control); rows outside ROWMASK and lanes
+// without a source keep their own value.
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+    return __hiloint2double(hi, lo);
 }
 
-template <int NSP>
+// Largest value over the wave: row_shr 1/2/4/8 within each row of 16 lanes, then row_bcast 15 / 31
+// across rows; lane 63 ends with the maximum.  fmax returns one of its operands, so the result is
+// bitwise one of the inputs (NaN inputs are passed over).
+__device__ __forceinline__ double wave_max(double v) {
+    v = fmax(v, dpp_f64<0x111>(v));
+    v = fmax(v, dpp_f64<0x112>(v));
+    v = fmax(v, dpp_f64<0x114>(v));
+    v = fmax(v, dpp_f64<0x118>(v));
+    v = fmax(v, dpp_f64<0x142, 0xA>(v));
+    v = fmax(v, dpp_f64<0x143, 0xC>(v));
+    return bcast(v, 63);
+}
+
+// row[j] -= l * prow[j] for j in [j0, n): the pivot row is never the row updated, so the loads of a
+// group of eight do not wait for the stores of the previous group.
+__device__ __forceinline__ void eliminate(double *__restrict__ row, const double *__restrict__ prow, double l,
+                                          int j0, int n) {
+    int j = j0;
+    for (; j + 8 <= n; j += 8) {
+        double r[8], p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            r[u] = row[j + u];
+            p[u] = prow[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) row[j + u] = r[u] - l * p[u];
+    }
+    for (; j < n; ++j) row[j] = row[j] - l * prow[j];
+}
+
+// LDS of one cell's workgroup of NW waves.
+template <int NSP, int NW>
 struct Shared {
-    double J[(NSP + 1) * LDJ];   // Jacobian, overwritten by the LU factors of M
-    double c[64], smh[64], u[64], prow[64], wdot[64], cdot[64], ddot[64];
+    double J[NW][(NSP + 1) * LDJ];   // per-wave Jacobian partials; J[0] ends as the Jacobian and is
+                                     // then overwritten by the LU factors of M
+    double W[NW][64];                // per-wave production-rate partials
+    double Cd[NW][64], Dd[NW][64];   // per-wave creation / destruction partials (heat release)
+    double c[64], smh[64], u[64];
     int perm[64];
     int cell;
 };
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 struct RhsOut {
     double f;      // this lane's component of dy/dt
     double cvm;    // mixture cv, erg/(g K)
     double uk;     // this lane's species internal energy, erg/mol (0 on other lanes)
     double cvk;    // this lane's species cv, erg/(mol K)
+    double cd, dd; // with split: this lane's creation / destruction rate (CKCDYR)
 };
 
-// f = dy/dt at (y_l, T); y_l is this lane's component (lane < NSP: Y_lane).  Leaves the
-// concentrations in sh.c, S/R-H/RT in sh.smh, u_k in sh.u and the production rates in sh.wdot, or,
-// with split, CKCDYR's creation / destruction rates in sh.cdot / sh.ddot (cklib.f:1497-1517).
-// Rates of progress are scattered into the species sums with LDS atomics; one wave owns the
-// workgroup, so the order of those additions is fixed and the result reproducible.
-template <int NSP>
-__device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP> &sh,
-                      double rho, double y_l, double T, bool split) {
+// f = dy/dt at (y_l, T); y_l is this lane's component (lane < NSP: Y_lane), the same in every wave
+// of the workgroup.  Leaves the concentrations in sh.c, S/R-H/RT in sh.smh and u_k in sh.u; with
+// split, returns CKCDYR's creation / destruction rates (cklib.f:1497-1517) instead of using them.
+// Wave w evaluates reactions w*64 + lane, (w + NW)*64 + lane, ... and scatters their rates of
+// progress into its own partial sums with LDS atomics (one wave's atomics land in a fixed order);
+// the partials are then added in wave order, so the result is reproducible.
+template <int NSP, int NW>
+__device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP, NW> &sh,
+                      double rho, double y_l, double T, bool split, int wave) {
     const int lane = lane_id();
     double uk = 0.0, cpk = 0.0, ck = 0.0;
     if (lane < NSP) {
         ck = rho * y_l / th->wt[lane];
-        sh.c[lane] = ck;
         const Nasa a = nasa(th, lane, T);
         const TPow p = tpow(T, th->ru);
         uk = h_mol(a, p) - p.rut;
         cpk = cp_mol(a, p);
-        sh.u[lane] = uk;
-        sh.smh[lane] = smh_species(a, T);
+        if (wave == 0) {
+            sh.c[lane] = ck;
+            sh.u[lane] = uk;
+            sh.smh[lane] = smh_species(a, T);
+        }
     }
-    sh.wdot[lane] = 0.0;
-    sh.cdot[lane] = 0.0;
-    sh.ddot[lane] = 0.0;
+    sh.W[wave][lane] = 0.0;
+    sh.Cd[wave][lane] = 0.0;
+    sh.Dd[wave][lane] = 0.0;
     const double ctot = wsum(ck);
     __syncthreads();
+    double *wdot_w = sh.W[wave], *cdot_w = sh.Cd[wave], *ddot_w = sh.Dd[wave];
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
-    for (int i = lane; i < K->ii; i += 64) {
+    for (int i = wave * 64 + lane; i < K->ii; i += 64 * NW) {
         const Rop r = reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
         const int *nu = K->nu[i], *nk = K->nk[i];
         if (!split) {
             const double q = r.qf - r.qr;
             for (int n = 0; n < 6; ++n)
-                if (nk[n] >= 0) atomicAdd(&sh.wdot[nk[n]], q * (double)nu[n]);
+                if (nk[n] >= 0) atomicAdd(&wdot_w[nk[n]], q * (double)nu[n]);
         } else {
             for (int n = 0; n < 3; ++n) {
                 if (nk[n] >= 0) {
                     const double a = (double)abs(nu[n]);
-                    atomicAdd(&sh.cdot[nk[n]], r.qr * a);
-                    atomicAdd(&sh.ddot[nk[n]], r.qf * a);
+                    atomicAdd(&cdot_w[nk[n]], r.qr * a);
+                    atomicAdd(&ddot_w[nk[n]], r.qf * a);
                 }
                 if (nk[n + 3] >= 0) {
                     const double a = (double)nu[n + 3];
-                    atomicAdd(&sh.cdot[nk[n + 3]], r.qf * a);
-                    atomicAdd(&sh.ddot[nk[n + 3]], r.qr * a);
+                    atomicAdd(&cdot_w[nk[n + 3]], r.qf * a);
+                    atomicAdd(&ddot_w[nk[n + 3]], r.qr * a);
                 }
             }
         }
     }
     __syncthreads();
-    const double wdot = lane < NSP ? sh.wdot[lane] : 0.0;
+    double wdot = 0.0, cd = 0.0, dd = 0.0;
+    if (lane < NSP) {
+        wdot = sh.W[0][lane];
+        if (split) {
+            cd = sh.Cd[0][lane];
+            dd = sh.Dd[0][lane];
+        }
+#pragma unroll
+        for (int w = 1; w < NW; ++w) {
+            wdot = wdot + sh.W[w][lane];
+            if (split) {
+                cd = cd + sh.Cd[w][lane];
+                dd = dd + sh.Dd[w][lane];
+            }
+        }
+    }
     const double cvk = cpk - th->ru;
     const double cvm = wsum(lane < NSP ? y_l * cvk / th->wt[lane] : 0.0);
     const double su = wsum(lane < NSP ? uk * wdot : 0.0);
@@ -131,48 +187,105 @@ __device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const De
     o.cvm = cvm;
     o.uk = uk;
     o.cvk = cvk;
+    o.cd = cd;
+    o.dd = dd;
     o.f = lane < NSP ? wdot * th->wt[lane] / rho : (lane == NSP ? -su / (rho * cvm) : 0.0);
+    if (NW > 1) __syncthreads();   // every wave has read the partials before the next evaluation clears them
     return o;
 }
 
-// Jacobian df/dy into sh.J (rows/cols 0..NSP, NSP = temperature).
-template <int NSP>
-__device__ __forceinline__ void jacobian(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP> &sh,
-                         double rho, double y_l, double T) {
+// The integrator evaluates f at six places (explicit pre-step, Jacobian, Newton, heat release);
+// one out-of-line copy serves them all (inlined at each, the rate code is some 80 kB of the
+// kernel).  The shared block goes over as an LDS-address-space pointer so the callee still addresses
+// it with LDS instructions.
+#define FCFD_LDS __attribute__((address_space(3)))
+
+#define FCFD_GLB __attribute__((address_space(1)))
+
+template <int NSP, int NW>
+__device__ __attribute__((noinline)) RhsOut rhs_call(FCFD_GLB const DevThermo *th, FCFD_GLB const DevKin *K,
+                                                     FCFD_LDS Shared<NSP, NW> *shl, double rho, double y_l, double T,
+                                                     bool split, int wave) {
+    Shared<NSP, NW> &sh = *(Shared<NSP, NW> *)shl;
+    return rhs<NSP, NW>((const DevThermo *)th, (const DevKin *)K, sh, rho, y_l, T, split, wave);
+}
+
+__device__ __forceinline__ FCFD_GLB const DevThermo *glb(const DevThermo *p) { return (FCFD_GLB const DevThermo *)p; }
+__device__ __forceinline__ FCFD_GLB const DevKin *glb(const DevKin *p) { return (FCFD_GLB const DevKin *)p; }
+
+template <int NSP, int NW>
+__device__ __forceinline__ FCFD_LDS Shared<NSP, NW> *lds(Shared<NSP, NW> &sh) {
+    return (FCFD_LDS Shared<NSP, NW> *)&sh;
+}
+
+// Jacobian df/dy into sh.J (rows/cols 0..NSP, NSP = temperature); out of line like rhs_call.
+// Returns f at (y, T), which the first Newton iteration of the step's first stage reuses.
+// Wave w scatters its reactions into its own partial sh.J[w]; the partials are added in wave order.
+template <int NSP, int NW>
+__device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *th1, FCFD_GLB const DevKin *K1,
+                                                   FCFD_LDS Shared<NSP, NW> *shl, double rho, double y_l, double T,
+                                                   int wave) {
+    const DevThermo *__restrict__ th = (const DevThermo *)th1;
+    const DevKin *__restrict__ K = (const DevKin *)K1;
+    Shared<NSP, NW> &sh = *(Shared<NSP, NW> *)shl;
     const int lane = lane_id();
     constexpr int N = NSP + 1;
-    const RhsOut f0 = rhs<NSP>(th, K, sh, rho, y_l, T, false);
-    for (int e = lane; e < N * LDJ; e += 64) sh.J[e] = 0.0;
+    const RhsOut f0 = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, y_l, T, false, wave);
+    for (int e = lane; e < N * LDJ; e += 64) sh.J[wave][e] = 0.0;
+    double *Jw = sh.J[wave];
+    double *J = sh.J[0];
     const double ctot = wsum(lane < NSP ? sh.c[lane] : 0.0);
     __syncthreads();
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
-    for (int i = lane; i < K->ii; i += 64) {
-        const int *nu = K->nu[i], *nk = K->nk[i];
-        if (nu[0] == 0) continue;
-        const Rop r = reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
-        double dq[6];
-        for (int s = 0; s < 6; ++s) {
-            dq[s] = 0.0;
-            if (nk[s] < 0) continue;
-            const int lo = s < 3 ? 0 : 3;
-            const int a = abs(nu[s]);
-            double d = (double)a * ipow(sh.c[nk[s]], a - 1);
-            for (int t = lo; t < lo + 3; ++t)
-                if (t != s && nk[t] >= 0) d = d * ipow(sh.c[nk[t]], abs(nu[t]));
-            dq[s] = s < 3 ? r.kf * d : -(r.kr * d);
+    // d(rate of progress)/d(concentration) of two reactions per lane per pass (rows i and i + 64):
+    // their chains are independent and overlap; the scatter follows in row order
+    constexpr int RB = 2;
+    for (int i0 = wave * 64 + lane; i0 < K->ii; i0 += 64 * NW * RB) {
+        double dq[RB][6];
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            const int i = min(i0 + 64 * NW * u, K->ii - 1);
+            const int *nu = K->nu[i], *nk = K->nk[i];
+            const Rop r = reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
+            for (int s = 0; s < 6; ++s) {
+                dq[u][s] = 0.0;
+                if (nk[s] < 0) continue;
+                const int lo = s < 3 ? 0 : 3;
+                const int a = abs(nu[s]);
+                double d = (double)a * ipow(sh.c[nk[s]], a - 1);
+                for (int t = lo; t < lo + 3; ++t)
+                    if (t != s && nk[t] >= 0) d = d * ipow(sh.c[nk[t]], abs(nu[t]));
+                dq[u][s] = s < 3 ? r.kf * d : -(r.kr * d);
+            }
         }
-        for (int s = 0; s < 6; ++s) {
-            if (nk[s] < 0) continue;
-            for (int t = 0; t < 6; ++t)
-                if (nk[t] >= 0) atomicAdd(&sh.J[nk[t] * LDJ + nk[s]], (double)nu[t] * dq[s]);
+#pragma unroll
+        for (int u = 0; u < RB; ++u) {
+            const int i = i0 + 64 * NW * u;
+            if (i >= K->ii) continue;
+            const int *nu = K->nu[i], *nk = K->nk[i];
+            if (nu[0] == 0) continue;
+            for (int s = 0; s < 6; ++s) {
+                if (nk[s] < 0) continue;
+                for (int t = 0; t < 6; ++t)
+                    if (nk[t] >= 0) atomicAdd(&Jw[nk[t] * LDJ + nk[s]], (double)nu[t] * dq[u][s]);
+            }
         }
     }
     __syncthreads();
+    if (NW > 1) {
+        for (int e = threadIdx.x; e < N * LDJ; e += 64 * NW) {
+            double v = sh.J[0][e];
+#pragma unroll
+            for (int w = 1; w < NW; ++w) v = v + sh.J[w][e];
+            J[e] = v;
+        }
+        __syncthreads();
+    }
     // temperature row from the concentration Jacobian, before rows are rescaled to Y
     double trow = 0.0;
-    if (lane < NSP) {
+    if (wave == 0 && lane < NSP) {
         double s = 0.0;
-        for (int j = 0; j < NSP; ++j) s = s + sh.u[j] * sh.J[j * LDJ + lane];
+        for (int j = 0; j < NSP; ++j) s = s + sh.u[j] * J[j * LDJ + lane];
         s = s * rho / th->wt[lane];
         const double fT = bcast(f0.f, NSP);
         trow = -s / (rho * f0.cvm) - fT * (f0.cvk / th->wt[lane]) / f0.cvm;
@@ -180,78 +293,91 @@ __device__ __forceinline__ void jacobian(const DevThermo *__restrict__ th, const
     __syncthreads();
     if (lane < NSP) {
         const double wr = th->wt[lane];
-        for (int k = 0; k < NSP; ++k) sh.J[lane * LDJ + k] = sh.J[lane * LDJ + k] * wr / th->wt[k];
-        sh.J[NSP * LDJ + lane] = trow;
+        for (int k = wave; k < NSP; k += NW) J[lane * LDJ + k] = J[lane * LDJ + k] * wr / th->wt[k];
+        if (wave == 0) J[NSP * LDJ + lane] = trow;
     }
     // temperature column by a forward difference
     const double dT = T * 1.0e-6;
-    const RhsOut f1 = rhs<NSP>(th, K, sh, rho, y_l, T + dT, false);
-    if (lane < N) sh.J[lane * LDJ + NSP] = (f1.f - f0.f) / dT;
+    const RhsOut f1 = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, y_l, T + dT, false, wave);
+    if (wave == 0 && lane < N) J[lane * LDJ + NSP] = (f1.f - f0.f) / dT;
     __syncthreads();
+    return f0;
 }
 
-// LU with partial pivoting of M = I - hg*J: rows are factored one per lane in VGPRs and the factors
-// written back over sh.J; rank = elimination step of this lane's row (N: not pivoted yet, -1: lane
-// beyond the system).  Returns false if a pivot is zero.
-template <int NSP>
-__device__ __forceinline__ bool lu(Shared<NSP> &sh, double hg, int &rank) {
+// LU with partial pivoting of M = I - hg*J, in place in sh.J[0]: lane r owns row r (LDS row stride
+// odd in 64-bit words, so the lanes of one column hit distinct banks).  Rows are not swapped:
+// sh.perm[k] is the pivot row of step k; ties go to the lower row.  Every wave finds the same pivot
+// and multipliers; the update of the trailing columns is split into NW contiguous column ranges,
+// one per wave.  A step's multipliers are stored into column k at the start of the next step, once
+// every wave has read that column.  Returns false if a pivot is zero.  The loops stay rolled
+// (unrolled, the 55-row factorisation is some 66 kB of straight-line code).
+template <int NSP, int NW>
+__device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
     constexpr int N = NSP + 1;
     const int lane = lane_id();
-    rank = lane < N ? N : -1;
-    double m[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        m[j] = 0.0;
-        if (lane < N) m[j] = -hg * sh.J[lane * LDJ + j];
-    }
-#pragma unroll
-    for (int j = 0; j < N; ++j)
-        if (lane == j) m[j] = m[j] + 1.0;
-    bool ok = true;
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const int p = wave_argmax(rank == N ? fabs(m[k]) : -1.0);
-        if (lane == p) {
-            rank = k;
-#pragma unroll
-            for (int j = k; j < N; ++j) sh.prow[j] = m[j];
+    int rank = lane < N ? N : -1;   // elimination step of this lane's row (N: not pivoted yet)
+    double *row = sh.J[0] + (lane < N ? lane : 0) * LDJ;
+    if (lane < N) {
+        for (int j = wave; j < N; j += NW) {
+            double m = -hg * row[j];
+            if (j == lane) m = m + 1.0;
+            row[j] = m;
         }
-        if (lane == 0) sh.perm[k] = p;
-        __syncthreads();
-        const double piv = sh.prow[k];
+    }
+    __syncthreads();
+    bool ok = true;
+    int kl = -1;          // column of this lane's multiplier still to be stored (-1: none)
+    double l = 0.0;
+    for (int k = 0; k < N; ++k) {
+        if (wave == 0 && kl >= 0) row[kl] = l;
+        kl = -1;
+        // pivot: the unpivoted row with the largest |entry| in column k, the lowest such row on ties
+        const bool open = rank == N;
+        const double v = open ? fabs(row[k]) : -1.0;
+        const double vmax = wave_max(v);
+        unsigned long long hit = __ballot(open && v == vmax);
+        if (hit == 0ull) {                      // column k of every open row is NaN
+            hit = __ballot(open);
+            ok = false;
+        }
+        const int p = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
+        if (lane == p) rank = k;
+        if (wave == 0 && lane == 0) sh.perm[k] = p;
+        const double *prow = sh.J[0] + p * LDJ;
+        const double piv = prow[k];
         if (!(piv != 0.0)) ok = false;
         if (rank == N) {
-            const double l = m[k] / piv;
-            m[k] = l;
-#pragma unroll
-            for (int j = k + 1; j < N; ++j) m[j] = m[j] - l * sh.prow[j];
+            l = row[k] / piv;
+            kl = k;
+            const int len = N - (k + 1);
+            eliminate(row, prow, l, k + 1 + len * wave / NW, k + 1 + len * (wave + 1) / NW);
         }
         __syncthreads();
     }
-    if (lane < N) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) sh.J[lane * LDJ + j] = m[j];
-    }
+    if (wave == 0 && kl >= 0) row[kl] = l;
     __syncthreads();
     return ok;
 }
 
-// Solves M x = b with the factors in sh.J; b and x distributed one component per lane.
-template <int NSP>
-__device__ __forceinline__ double solve(const Shared<NSP> &sh, int rank, double b) {
+// Solves M x = b with the factors in sh.J[0]; b and x distributed one component per lane (every
+// wave solves the same system).  Lane r carries the right-hand side of the row pivoted at step r
+// (row sh.perm[r]), so each step's broadcast comes from the lane the loop counter names.
+template <int NSP, int NW>
+__device__ __forceinline__ double solve(const Shared<NSP, NW> &sh, double b) {
     constexpr int N = NSP + 1;
     const int lane = lane_id();
-    const double *row = sh.J + (lane < N ? lane : 0) * LDJ;
+    const int pr = lane < N ? sh.perm[lane] : 0;
+    const double *row = sh.J[0] + pr * LDJ;
+    double bq = __shfl(b, pr, 64);
     for (int k = 0; k < N; ++k) {
-        const double bp = bcast(b, sh.perm[k]);
-        if (rank > k) b = b - row[k] * bp;
+        const double bp = bcast(bq, k);
+        if (lane > k && lane < N) bq = bq - row[k] * bp;
     }
     double x = 0.0;
     for (int k = N - 1; k >= 0; --k) {
-        const int p = sh.perm[k];
-        const double xk = bcast(b / row[k], p);
+        const double xk = bcast(bq / row[k], k);
         if (lane == k) x = xk;
-        if (rank >= 0 && rank < k) b = b - row[k] * xk;
+        if (lane < k) bq = bq - row[k] * xk;
     }
     return x;
 }
@@ -281,19 +407,26 @@ __device__ __forceinline__ double adjusted(double y_l, int lane, int nsp) {
     return v / wsum(v);
 }
 
-template <int NSP>
-__global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
-    __shared__ Shared<NSP> sh;
+// NW waves per cell.  NW = 1: one cell per wave, for long work lists (throughput).  NW = 4: the four
+// waves of a workgroup split every reaction loop and the LU updates of one cell and run the rest of
+// the integrator redundantly on identical values, so they take identical decisions and meet at the
+// same barriers; for short lists, where the launch lasts as long as its slowest cell.  A launch
+// runs only if its mode matches the list length (a.coop_max), read on the device.
+template <int NSP, int NW>
+__global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
+    __shared__ Shared<NSP, NW> sh;
     constexpr int N = NSP + 1;
     const int lane = lane_id();
+    const int wave = NW > 1 ? wave_id() : 0;
     const DevThermo *__restrict__ th = a.thermo;
     const DevKin *__restrict__ K = a.kin;
+    const int nitems = a.list_n ? int(*a.list_n) : a.ncells;
+    if ((NW > 1) != (nitems <= a.coop_max)) return;
     for (;;) {
-        if (lane == 0) sh.cell = int(atomicAdd(a.counter, 1u));
+        if (threadIdx.x == 0) sh.cell = int(atomicAdd(a.counter, 1u));
         __syncthreads();
         const int item = sh.cell;
         __syncthreads();
-        const int nitems = a.list_n ? int(*a.list_n) : a.ncells;
         if (item >= nitems) break;
         const int cell = a.list ? a.list[item] : item;
 
@@ -321,15 +454,14 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
 
         double t = 0.0, h = dt;
         int steps = 0, rejects = 0, newton = 0;
-        int rank = -1;
         // ---- explicit pre-step: Heun over the whole dt, accepted when the embedded Euler error
         //      estimate is within tolerance (quiescent cells never build a Jacobian) ----
         if (!a.skip_explicit) {
             const double w = lane < NSP ? 1.0 / (a.atol + a.rtol * fabs(y))
                                         : (lane == NSP ? 1.0 / (a.rtol * fabs(y)) : 0.0);
-            const RhsOut f0 = rhs<NSP>(th, K, sh, rho, y, bcast(y, NSP), false);
+            const RhsOut f0 = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, y, bcast(y, NSP), false, wave);
             const double y1 = lane < N ? y + dt * f0.f : 0.0;
-            const RhsOut f1 = rhs<NSP>(th, K, sh, rho, y1, bcast(y1, NSP), false);
+            const RhsOut f1 = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, y1, bcast(y1, NSP), false, wave);
             const double err = wnorm(lane < N ? (dt * 0.5) * (f1.f - f0.f) : 0.0, w, N);
             if (err <= 1.0) {
                 if (lane < N) y = y + (dt * 0.5) * (f0.f + f1.f);
@@ -343,12 +475,11 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
             const double Tn = bcast(y, NSP);
             const double w = lane < NSP ? 1.0 / (a.atol + a.rtol * fabs(y))
                                         : (lane == NSP ? 1.0 / (a.rtol * fabs(y)) : 0.0);
-            bool have_j = false;
             for (;;) {
-                if (!have_j) jacobian<NSP>(th, K, sh, rho, y, Tn);   // lu() overwrites it
-                have_j = false;
+                // J at (y, Tn) (lu() overwrites it) and f there, the first Newton evaluation of stage 0
+                const RhsOut fy = jacobian<NSP, NW>(glb(th), glb(K), lds(sh), rho, y, Tn, wave);
                 const double hg = h * CH_G;
-                bool ok = lu<NSP>(sh, hg, rank);
+                bool ok = lu<NSP, NW>(sh, hg, wave);
                 double k0 = 0.0, k1 = 0.0, k2 = 0.0, z = 0.0;
                 for (int st = 0; st < 3 && ok; ++st) {
                     double base = y;
@@ -359,9 +490,11 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
                     bool conv = false;
                     for (int it = 0; it < CH_MAXIT; ++it) {
                         const double Tz = bcast(z, NSP);
-                        const RhsOut f = rhs<NSP>(th, K, sh, rho, z, Tz, false);
+                        RhsOut f;
+                        if (st == 0 && it == 0) f = fy;      // z = y, Tz = Tn
+                        else f = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, z, Tz, false, wave);
                         double r = lane < N ? -(z - base - hg * f.f) : 0.0;
-                        r = solve<NSP>(sh, rank, r);
+                        r = solve<NSP, NW>(sh, r);
                         if (lane < N) z = z + r;
                         ++newton;
                         const double nrm = wnorm(lane < N ? r : 0.0, w, N);
@@ -380,7 +513,7 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
                     continue;
                 }
                 double ev = lane < N ? h * (CH_E1 * k0 + CH_E2 * k1 + CH_E3 * k2) : 0.0;
-                ev = solve<NSP>(sh, rank, ev);
+                ev = solve<NSP, NW>(sh, ev);
                 const double err = wnorm(lane < N ? ev : 0.0, w, N);
                 const double fac = 0.9 * pow(fmax(err, 1.0e-10), -1.0 / 3.0);
                 if (err <= 1.0) {
@@ -408,29 +541,29 @@ __global__ void __launch_bounds__(64) k_chem(ChemArgs a) {
             if (fabs(d) <= 1.0e-12 * T) break;
         }
 
-        // ---- store (SRC/source.f90:60-73) ----
+        // ---- store (SRC/source.f90:60-73); wave 0 writes ----
         if (a.layout == 0) {
-            if (lane < NSP) a.qf[(3 + lane) * a.P + col] = ya;
-            if (lane == 0) a.qf[2 * a.P + col] = T;
+            if (wave == 0 && lane < NSP) a.qf[(3 + lane) * a.P + col] = ya;
+            if (wave == 0 && lane == 0) a.qf[2 * a.P + col] = T;
             if (a.hrr) {
                 // heat_release_rate (SRC/thermo.f90:98-101)
                 const double term = lane < NSP ? rho * th->h298[lane] * (ya - y_raw) / dt : 0.0;
                 const double s1 = wsum(term);
                 // heat_release_rate_omegadot via CKCDYR at the new state (SRC/thermo.f90:128-132)
-                rhs<NSP>(th, K, sh, rho, ya, T, true);
-                const double cd = lane < NSP ? sh.cdot[lane] : 0.0;
-                const double dd = lane < NSP ? sh.ddot[lane] : 0.0;
+                const RhsOut fc = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, ya, T, true, wave);
+                const double cd = lane < NSP ? fc.cd : 0.0;
+                const double dd = lane < NSP ? fc.dd : 0.0;
                 const double s2 = wsum(lane < NSP ? th->wt[lane] * th->h298[lane] * (cd - dd) : 0.0);
-                if (lane == 0) {
+                if (wave == 0 && lane == 0) {
                     a.hrr[cell] = -s1 * 1.e-1;
                     a.hrr_dot[cell] = -s2 * 1.e-1;
                 }
             }
         } else {
-            if (lane < NSP) a.b_y[long(lane) * a.ncells + cell] = y;
-            if (lane == 0) a.b_T[cell] = T;
+            if (wave == 0 && lane < NSP) a.b_y[long(lane) * a.ncells + cell] = y;
+            if (wave == 0 && lane == 0) a.b_T[cell] = T;
         }
-        if (a.stats && lane == 0) a.stats[cell] = ChemCellStats{steps, rejects, newton, failed};
+        if (a.stats && wave == 0 && lane == 0) a.stats[cell] = ChemCellStats{steps, rejects, newton, failed};
     }
 }
 
@@ -459,13 +592,23 @@ hipError_t launch_chem_full(ChemArgs a, int nsp, int n_workgroups, int *list, un
     return launch_chem(a, nsp, n_workgroups, st);
 }
 
-hipError_t launch_chem(const ChemArgs &a, int nsp, int n_workgroups, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, a.counter);
+// n_workgroups one-wave workgroups (four per CU) for long lists; a quarter as many four-wave
+// workgroups (one per CU) for lists of at most 2 * n_workgroups cells.  Both kernels are launched;
+// the one whose mode does not match the list length on the device returns at once.
+hipError_t launch_chem(const ChemArgs &a0, int nsp, int n_workgroups, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, a0.counter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int nwg = n_workgroups < a.ncells ? n_workgroups : a.ncells;
-    if (nwg <= 0) return hipSuccess;
-#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem<N>), dim3(nwg), dim3(64), 0, st, a); return hipGetLastError(); }
+    ChemArgs a = a0;
+    a.coop_max = 2 * n_workgroups;
+    const int nwg1 = std::min(n_workgroups, a.ncells);
+    const int nwg4 = std::min(std::max(n_workgroups / 4, 1), a.ncells);
+    if (nwg1 <= 0) return hipSuccess;
+#define FCFD_CASE(N) if (nsp == N) { \
+        hipLaunchKernelGGL((k_chem<N, 4>), dim3(nwg4), dim3(256), 0, st, a); \
+        if ((e = hipGetLastError()) != hipSuccess) return e; \
+        hipLaunchKernelGGL((k_chem<N, 1>), dim3(nwg1), dim3(64), 0, st, a); \
+        return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;

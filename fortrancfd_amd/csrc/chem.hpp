@@ -58,9 +58,11 @@ struct ChemArgs {
     unsigned *stiff_n;
     int *route;                  // optional per-cell route: 2 accepted by the explicit screen, 0 stiff
     int layout;                  // 0: domain, 1: batch
+    int coop_max;                // set by launch_chem: lists up to this length run four waves per cell
 };
 
-// Wave-per-cell SDIRK integrator over all cells or over a work list (persistent grid).
+// SDIRK integrator over all cells or over a work list (persistent grid): one cell per wave for long
+// lists, one cell per four-wave workgroup for short ones.
 hipError_t launch_chem(const ChemArgs &a, int nsp, int n_workgroups, hipStream_t st);
 // Lane-per-cell explicit screen over cells 0 .. ncells: accepted cells are final, the others are
 // appended to a.stiff (a.stiff_n must be zero before the launch).
