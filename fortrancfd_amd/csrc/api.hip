@@ -91,6 +91,7 @@ struct fcfd_ctx {
     fcfd_surrogate *sur = nullptr;
     int *d_accepted = nullptr;
     int *d_list = nullptr;               // cells the explicit screen sends to the implicit integrator
+    int *d_order = nullptr;              // that list (or the surrogate's fallback list) reordered, slowest first
     unsigned *d_list_n = nullptr;
     double *acc = nullptr;               // RK4 stage accumulator
     int graph_next[3] = {0, 0, 0};       // state buffer after the captured step from buffer i
@@ -262,7 +263,7 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
     a.thermo = c->thermo; a.kin = static_cast<const DevKin *>(c->mech->d_kin); a.counter = c->d_counter;
     a.ncells = c->nblk * c->ncell; a.rtol = c->o.rtol; a.atol = c->o.atol;
     a.qf = c->qf; a.P = c->P; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S;
-    a.dt = dt; a.dt_dev = dt_dev; a.stats = c->d_stats;
+    a.dt = dt; a.dt_dev = dt_dev; a.stats = c->d_stats; a.order = c->d_order;
     if (want_hrr) { a.hrr = c->d_hrr; a.hrr_dot = c->d_hrr_dot; }
     a.layout = 0;
     a.route = c->d_accepted;
@@ -412,6 +413,7 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
         HIP_TRY(hipMalloc(&c->d_accepted, sizeof(int) * nint));
         HIP_TRY(hipMemset(c->d_accepted, 0, sizeof(int) * nint));
         HIP_TRY(hipMalloc(&c->d_list, sizeof(int) * nint));
+        HIP_TRY(hipMalloc(&c->d_order, sizeof(int) * nint));
         HIP_TRY(hipMalloc(&c->d_list_n, sizeof(unsigned)));
         HIP_TRY(hipMemset(c->d_hrr, 0, sizeof(double) * nint));
         HIP_TRY(hipMemset(c->d_hrr_dot, 0, sizeof(double) * nint));
@@ -429,7 +431,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->qf); (void)hipFree(c->F); (void)hipFree(c->god); (void)hipFree(c->cells);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
-    (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_list_n);
+    (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_order); (void)hipFree(c->d_list_n);
     (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

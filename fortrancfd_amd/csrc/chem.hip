@@ -37,12 +37,6 @@ constexpr int LDJ = 65;   // LDS row stride of the Jacobian (odd in 64-bit words
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-__device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 __device__ __forceinline__ double bcast(double v, int src) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
@@ -56,6 +50,27 @@ __device__ __forceinline__ double dpp_f64(double v) {
     const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, ROWMASK, 0xF, false);
     const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, ROWMASK, 0xF, false);
     return __hiloint2double(hi, lo);
+}
+
+// As dpp_f64, but lanes without a source and rows outside ROWMASK read 0.
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double dpp_f64_or0(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the wave, the same value in every lane: prefix sums within rows of 16 lanes (row_shr
+// 1/2/4/8), then row totals carried across rows (row_bcast 15 / 31), lane 63 broadcast.  The order
+// of the additions is fixed, so equal inputs give bitwise equal sums in every wave.
+__device__ __forceinline__ double wsum(double v) {
+    v = v + dpp_f64_or0<0x111>(v);
+    v = v + dpp_f64_or0<0x112>(v);
+    v = v + dpp_f64_or0<0x114>(v);
+    v = v + dpp_f64_or0<0x118>(v);
+    v = v + dpp_f64_or0<0x142, 0xA>(v);
+    v = v + dpp_f64_or0<0x143, 0xC>(v);
+    return bcast(v, 63);
 }
 
 // Largest value over the wave: row_shr 1/2/4/8 within each row of 16 lanes, then row_bcast 15 / 31
@@ -359,6 +374,12 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
     return ok;
 }
 
+// dst[0 .. n) = src[0 .. n), by the whole workgroup.
+__device__ __forceinline__ void copy_j(double *dst, const double *src, int n) {
+    for (int e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
+    __syncthreads();
+}
+
 // Solves M x = b with the factors in sh.J[0]; b and x distributed one component per lane (every
 // wave solves the same system).  Lane r carries the right-hand side of the row pivoted at step r
 // (row sh.perm[r]), so each step's broadcast comes from the lane the loop counter names.
@@ -475,9 +496,19 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
             const double Tn = bcast(y, NSP);
             const double w = lane < NSP ? 1.0 / (a.atol + a.rtol * fabs(y))
                                         : (lane == NSP ? 1.0 / (a.rtol * fabs(y)) : 0.0);
+            // J at (y, Tn) and f there, the first Newton evaluation of stage 0.  lu() overwrites J;
+            // with several waves the spare partial sh.J[1] keeps a copy, so a rejected attempt (same
+            // y) restores J instead of evaluating it again.
+            RhsOut fy;
+            bool have_j = false;
             for (;;) {
-                // J at (y, Tn) (lu() overwrites it) and f there, the first Newton evaluation of stage 0
-                const RhsOut fy = jacobian<NSP, NW>(glb(th), glb(K), lds(sh), rho, y, Tn, wave);
+                if (!have_j) {
+                    fy = jacobian<NSP, NW>(glb(th), glb(K), lds(sh), rho, y, Tn, wave);
+                    if (NW > 1) copy_j(sh.J[1], sh.J[0], N * LDJ);
+                } else {
+                    copy_j(sh.J[0], sh.J[NW > 1 ? 1 : 0], N * LDJ);
+                }
+                have_j = NW > 1;
                 const double hg = h * CH_G;
                 bool ok = lu<NSP, NW>(sh, hg, wave);
                 double k0 = 0.0, k1 = 0.0, k2 = 0.0, z = 0.0;
@@ -571,6 +602,47 @@ __global__ void k_zero_counter(unsigned int *c) {
     if (threadIdx.x == 0) *c = 0u;
 }
 
+// Stable partition of a work list by the cells' attempt counts (steps + rejects) of the previous
+// integration: first those with at least CH_HEAVY, then the rest.  A launch lasts as long as its
+// slowest cell, and the cells that were slow in the last step are, near a flame, the slow ones in
+// this step; started first, they do not wait behind a queue of cheap cells.  One workgroup.
+constexpr int CH_HEAVY = 4;
+
+__global__ void __launch_bounds__(1024) k_heavy_first(const int *list, const unsigned *list_n,
+                                                      const ChemCellStats *stats, int *out) {
+    __shared__ unsigned wcount[16], woff[16], base, total;
+    const int n = int(*list_n);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) base = 0;
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int i0 = 0; i0 < n; i0 += 1024) {
+            const int i = i0 + t;
+            int cell = 0;
+            bool take = false;
+            if (i < n) {
+                cell = list[i];
+                const ChemCellStats st = stats[cell];
+                take = (st.steps + st.rejects >= CH_HEAVY) == (pass == 0);
+            }
+            const unsigned long long b = __ballot(take);
+            const unsigned rank = unsigned(__popcll(b & ((1ull << lane) - 1ull)));
+            if (lane == 0) wcount[w] = unsigned(__popcll(b));
+            __syncthreads();
+            if (t == 0) {
+                unsigned s = 0;
+                for (int q = 0; q < 16; ++q) { woff[q] = s; s += wcount[q]; }
+                total = s;
+            }
+            __syncthreads();
+            if (take) out[base + woff[w] + rank] = cell;
+            __syncthreads();
+            if (t == 0) base += total;
+            __syncthreads();
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st) {
@@ -601,6 +673,11 @@ hipError_t launch_chem(const ChemArgs &a0, int nsp, int n_workgroups, hipStream_
     if (e != hipSuccess) return e;
     ChemArgs a = a0;
     a.coop_max = 2 * n_workgroups;
+    if (a.order && a.list && a.list_n && a.stats && a.layout == 0) {
+        hipLaunchKernelGGL(k_heavy_first, dim3(1), dim3(1024), 0, st, a.list, a.list_n, a.stats, a.order);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        a.list = a.order;
+    }
     const int nwg1 = std::min(n_workgroups, a.ncells);
     const int nwg4 = std::min(std::max(n_workgroups / 4, 1), a.ncells);
     if (nwg1 <= 0) return hipSuccess;

@@ -59,6 +59,9 @@ struct ChemArgs {
     int *route;                  // optional per-cell route: 2 accepted by the explicit screen, 0 stiff
     int layout;                  // 0: domain, 1: batch
     int coop_max;                // set by launch_chem: lists up to this length run four waves per cell
+    // optional (domain layout with stats): buffer of the list's capacity that receives the list
+    // reordered with the cells whose previous integration took the most attempts first
+    int *order;
 };
 
 // SDIRK integrator over all cells or over a work list (persistent grid): one cell per wave for long
