@@ -86,8 +86,8 @@ __device__ __forceinline__ double wave_max(double v) {
     return bcast(v, 63);
 }
 
-// row[j] -= l * prow[j] for j in [j0, n): the pivot row is never the row updated, so the loads of a
-// group of eight do not wait for the stores of the previous group.
+// row[j] = fma(-l, prow[j], row[j]) for j in [j0, n): the pivot row is never the row updated, so
+// the loads of a group of eight do not wait for the stores of the previous group.
 __device__ __forceinline__ void eliminate(double *__restrict__ row, const double *__restrict__ prow, double l,
                                           int j0, int n) {
     int j = j0;
@@ -99,9 +99,9 @@ __device__ __forceinline__ void eliminate(double *__restrict__ row, const double
             p[u] = prow[j + u];
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) row[j + u] = r[u] - l * p[u];
+        for (int u = 0; u < 8; ++u) row[j + u] = fma(-l, p[u], r[u]);
     }
-    for (; j < n; ++j) row[j] = row[j] - l * prow[j];
+    for (; j < n; ++j) row[j] = fma(-l, prow[j], row[j]);
 }
 
 // LDS of one cell's workgroup of NW waves.
@@ -112,6 +112,7 @@ struct Shared {
     double W[NW][64];                // per-wave production-rate partials
     double Cd[NW][64], Dd[NW][64];   // per-wave creation / destruction partials (heat release)
     double c[64], smh[64], u[64];
+    double rdiag[64];                // 1 / pivot of each elimination step
     int perm[64];
     int cell;
 };
@@ -357,9 +358,12 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
         }
         const int p = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
         if (lane == p) rank = k;
-        if (wave == 0 && lane == 0) sh.perm[k] = p;
         const double *prow = sh.J[0] + p * LDJ;
         const double piv = prow[k];
+        if (wave == 0 && lane == 0) {
+            sh.perm[k] = p;
+            sh.rdiag[k] = 1.0 / piv;
+        }
         if (!(piv != 0.0)) ok = false;
         if (rank == N) {
             l = row[k] / piv;
@@ -382,7 +386,9 @@ __device__ __forceinline__ void copy_j(double *dst, const double *src, int n) {
 
 // Solves M x = b with the factors in sh.J[0]; b and x distributed one component per lane (every
 // wave solves the same system).  Lane r carries the right-hand side of the row pivoted at step r
-// (row sh.perm[r]), so each step's broadcast comes from the lane the loop counter names.
+// (row sh.perm[r]), so each step's broadcast comes from the lane the loop counter names.  Each step
+// of the two substitutions is one broadcast and one fused multiply-add (the back substitution
+// multiplies by the stored reciprocal pivot), the whole of the chain from one step to the next.
 template <int NSP, int NW>
 __device__ __forceinline__ double solve(const Shared<NSP, NW> &sh, double b) {
     constexpr int N = NSP + 1;
@@ -392,13 +398,13 @@ __device__ __forceinline__ double solve(const Shared<NSP, NW> &sh, double b) {
     double bq = __shfl(b, pr, 64);
     for (int k = 0; k < N; ++k) {
         const double bp = bcast(bq, k);
-        if (lane > k && lane < N) bq = bq - row[k] * bp;
+        if (lane > k && lane < N) bq = fma(-row[k], bp, bq);
     }
     double x = 0.0;
     for (int k = N - 1; k >= 0; --k) {
-        const double xk = bcast(bq / row[k], k);
+        const double xk = bcast(bq, k) * sh.rdiag[k];
         if (lane == k) x = xk;
-        if (lane < k) bq = bq - row[k] * xk;
+        if (lane < k) bq = fma(-row[k], xk, bq);
     }
     return x;
 }

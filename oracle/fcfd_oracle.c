@@ -1176,8 +1176,9 @@ void fo_chem_jac(const fo_mech *m, double rho, const double *y, double *J) {
 }
 
 /* LU of A (n x n, row-major) with partial pivoting, rows not swapped: rank[r] is the elimination
- * step at which row r was the pivot, perm[k] the pivot row of step k; ties go to the lower row. */
-static int ch_lu(double *A, int n, int *perm, int *rank) {
+ * step at which row r was the pivot, perm[k] the pivot row of step k (ties go to the lower row),
+ * rdiag[k] = 1 / pivot of step k.  The trailing update is one fused multiply-add per entry. */
+static int ch_lu(double *A, int n, int *perm, int *rank, double *rdiag) {
     for (int r = 0; r < n; ++r) rank[r] = n;
     for (int k = 0; k < n; ++k) {
         int p = -1;
@@ -1188,29 +1189,31 @@ static int ch_lu(double *A, int n, int *perm, int *rank) {
         rank[p] = k;
         perm[k] = p;
         const double piv = A[p * n + k];
+        rdiag[k] = 1.0 / piv;
         for (int r = 0; r < n; ++r) {
             if (rank[r] != n) continue;
             const double l = A[r * n + k] / piv;
             A[r * n + k] = l;
-            for (int j = k + 1; j < n; ++j) A[r * n + j] = A[r * n + j] - l * A[p * n + j];
+            for (int j = k + 1; j < n; ++j) A[r * n + j] = fma(-l, A[p * n + j], A[r * n + j]);
         }
     }
     return 0;
 }
 
-/* Solves A x = b in place (b -> x, x in natural order). */
-static void ch_solve(const double *A, int n, const int *perm, const int *rank, double *b) {
+/* Solves A x = b in place (b -> x, x in natural order): fused multiply-adds, and the back
+ * substitution multiplies by the stored reciprocal pivots. */
+static void ch_solve(const double *A, int n, const int *perm, const int *rank, const double *rdiag, double *b) {
     double x[65];
     for (int k = 0; k < n; ++k) {
         const double bp = b[perm[k]];
         for (int r = 0; r < n; ++r)
-            if (rank[r] > k) b[r] = b[r] - A[r * n + k] * bp;
+            if (rank[r] > k) b[r] = fma(-A[r * n + k], bp, b[r]);
     }
     for (int k = n - 1; k >= 0; --k) {
         const int p = perm[k];
-        x[k] = b[p] / A[p * n + k];
+        x[k] = b[p] * rdiag[k];
         for (int r = 0; r < n; ++r)
-            if (rank[r] < k) b[r] = b[r] - A[r * n + k] * x[k];
+            if (rank[r] < k) b[r] = fma(-A[r * n + k], x[k], b[r]);
     }
     memcpy(b, x, sizeof(double) * n);
 }
@@ -1242,6 +1245,7 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
     double y[65], w[65], f[65], z[65], base[65], k[3][65], r[65], e[65], ya[64];
     static double J[65 * 65], M[65 * 65];
     int perm[65], rank[65];
+    double rdiag[65];
     fo_chem_stats s = {0, 0, 0, 0, 0};
     memcpy(y, yT, sizeof(double) * n);
     double e0, cv0;
@@ -1273,7 +1277,7 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
             const double hg = h * g;
             for (int i = 0; i < n * n; ++i) M[i] = -hg * J[i];
             for (int i = 0; i < n; ++i) M[i * n + i] = M[i * n + i] + 1.0;
-            int ok = ch_lu(M, n, perm, rank) == 0;
+            int ok = ch_lu(M, n, perm, rank, rdiag) == 0;
             /* stage s: z = y + h sum_{j<s} a_sj k_j + h g f(z); predictor z = base + h g k_{s-1} */
             for (int st = 0; st < 3 && ok; ++st) {
                 for (int i = 0; i < n; ++i) {
@@ -1288,7 +1292,7 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
                 for (int it = 0; it < CH_MAXIT; ++it) {
                     fo_chem_rhs(m, rho, z, f);
                     for (int i = 0; i < n; ++i) r[i] = -(z[i] - base[i] - hg * f[i]);
-                    ch_solve(M, n, perm, rank, r);
+                    ch_solve(M, n, perm, rank, rdiag, r);
                     for (int i = 0; i < n; ++i) z[i] = z[i] + r[i];
                     s.newton++;
                     const double nrm = ch_norm(r, w, n);
@@ -1306,7 +1310,7 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
                 continue;
             }
             for (int i = 0; i < n; ++i) e[i] = h * (CH_E1 * k[0][i] + CH_E2 * k[1][i] + CH_E3 * k[2][i]);
-            ch_solve(M, n, perm, rank, e);
+            ch_solve(M, n, perm, rank, rdiag, e);
             const double err = ch_norm(e, w, n);
             const double fac = 0.9 * pow(fmax(err, 1.0e-10), -1.0 / 3.0);
             if (err <= 1.0) {
