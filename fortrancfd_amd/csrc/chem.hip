@@ -33,7 +33,7 @@ constexpr double CH_E3 = 0.435866521508459;
 constexpr double CH_KAPPA = 0.03;
 constexpr int CH_MAXIT = 7;
 constexpr int CH_MAXSTEPS = 5000;
-constexpr int LDJ = 65;   // LDS row stride of the Jacobian (odd in 64-bit words)
+constexpr int LDJ = 65;   // LDS row stride of the Jacobian (odd in 64-bit words, > NSP + 8)
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -132,10 +132,16 @@ struct RhsOut {
 // split, returns CKCDYR's creation / destruction rates (cklib.f:1497-1517) instead of using them.
 // Wave w evaluates reactions w*64 + lane, (w + NW)*64 + lane, ... and scatters their rates of
 // progress into its own partial sums with LDS atomics (one wave's atomics land in a fixed order);
-// the partials are then added in wave order, so the result is reproducible.
+// the partials are then added in wave order, so the result is reproducible.  In three pieces so
+// that the Jacobian can share the reaction loop: rhs_begin (species terms, ends with a barrier),
+// rhs_scatter per reaction, rhs_end (partial sums and f).
+struct RhsPre {
+    double uk, cpk, ctot;
+};
+
 template <int NSP, int NW>
-__device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP, NW> &sh,
-                      double rho, double y_l, double T, bool split, int wave) {
+__device__ __forceinline__ RhsPre rhs_begin(const DevThermo *__restrict__ th, Shared<NSP, NW> &sh, double rho,
+                                            double y_l, double T, int wave) {
     const int lane = lane_id();
     double uk = 0.0, cpk = 0.0, ck = 0.0;
     if (lane < NSP) {
@@ -155,30 +161,39 @@ __device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const De
     sh.Dd[wave][lane] = 0.0;
     const double ctot = wsum(ck);
     __syncthreads();
-    double *wdot_w = sh.W[wave], *cdot_w = sh.Cd[wave], *ddot_w = sh.Dd[wave];
-    const double alogt = log(T), pfac = th->pa / (th->ru * T);
-    for (int i = wave * 64 + lane; i < K->ii; i += 64 * NW) {
-        const Rop r = reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
-        const int *nu = K->nu[i], *nk = K->nk[i];
-        if (!split) {
-            const double q = r.qf - r.qr;
-            for (int n = 0; n < 6; ++n)
-                if (nk[n] >= 0) atomicAdd(&wdot_w[nk[n]], q * (double)nu[n]);
-        } else {
-            for (int n = 0; n < 3; ++n) {
-                if (nk[n] >= 0) {
-                    const double a = (double)abs(nu[n]);
-                    atomicAdd(&cdot_w[nk[n]], r.qr * a);
-                    atomicAdd(&ddot_w[nk[n]], r.qf * a);
-                }
-                if (nk[n + 3] >= 0) {
-                    const double a = (double)nu[n + 3];
-                    atomicAdd(&cdot_w[nk[n + 3]], r.qf * a);
-                    atomicAdd(&ddot_w[nk[n + 3]], r.qr * a);
-                }
+    return RhsPre{uk, cpk, ctot};
+}
+
+template <int NSP, int NW>
+__device__ __forceinline__ void rhs_scatter(const DevKin *__restrict__ K, Shared<NSP, NW> &sh, int i, const Rop &r,
+                                            bool split, int wave) {
+    const int *nu = K->nu[i], *nk = K->nk[i];
+    if (!split) {
+        double *wdot_w = sh.W[wave];
+        const double q = r.qf - r.qr;
+        for (int n = 0; n < 6; ++n)
+            if (nk[n] >= 0) atomicAdd(&wdot_w[nk[n]], q * (double)nu[n]);
+    } else {
+        double *cdot_w = sh.Cd[wave], *ddot_w = sh.Dd[wave];
+        for (int n = 0; n < 3; ++n) {
+            if (nk[n] >= 0) {
+                const double a = (double)abs(nu[n]);
+                atomicAdd(&cdot_w[nk[n]], r.qr * a);
+                atomicAdd(&ddot_w[nk[n]], r.qf * a);
+            }
+            if (nk[n + 3] >= 0) {
+                const double a = (double)nu[n + 3];
+                atomicAdd(&cdot_w[nk[n + 3]], r.qf * a);
+                atomicAdd(&ddot_w[nk[n + 3]], r.qr * a);
             }
         }
     }
+}
+
+template <int NSP, int NW>
+__device__ __forceinline__ RhsOut rhs_end(const DevThermo *__restrict__ th, Shared<NSP, NW> &sh, double rho,
+                                          double y_l, const RhsPre &pre, bool split) {
+    const int lane = lane_id();
     __syncthreads();
     double wdot = 0.0, cd = 0.0, dd = 0.0;
     if (lane < NSP) {
@@ -196,18 +211,30 @@ __device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const De
             }
         }
     }
-    const double cvk = cpk - th->ru;
+    const double cvk = pre.cpk - th->ru;
     const double cvm = wsum(lane < NSP ? y_l * cvk / th->wt[lane] : 0.0);
-    const double su = wsum(lane < NSP ? uk * wdot : 0.0);
+    const double su = wsum(lane < NSP ? pre.uk * wdot : 0.0);
     RhsOut o;
     o.cvm = cvm;
-    o.uk = uk;
+    o.uk = pre.uk;
     o.cvk = cvk;
     o.cd = cd;
     o.dd = dd;
     o.f = lane < NSP ? wdot * th->wt[lane] / rho : (lane == NSP ? -su / (rho * cvm) : 0.0);
     if (NW > 1) __syncthreads();   // every wave has read the partials before the next evaluation clears them
     return o;
+}
+
+template <int NSP, int NW>
+__device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP, NW> &sh,
+                      double rho, double y_l, double T, bool split, int wave) {
+    const RhsPre pre = rhs_begin<NSP, NW>(th, sh, rho, y_l, T, wave);
+    const double alogt = log(T), pfac = th->pa / (th->ru * T);
+    for (int i = wave * 64 + lane_id(); i < K->ii; i += 64 * NW) {
+        const Rop r = reaction<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
+        rhs_scatter<NSP, NW>(K, sh, i, r, split, wave);
+    }
+    return rhs_end<NSP, NW>(th, sh, rho, y_l, pre, split);
 }
 
 // The integrator evaluates f at six places (explicit pre-step, Jacobian, Newton, heat release);
@@ -246,23 +273,27 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
     Shared<NSP, NW> &sh = *(Shared<NSP, NW> *)shl;
     const int lane = lane_id();
     constexpr int N = NSP + 1;
-    const RhsOut f0 = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, y_l, T, false, wave);
+    // f at (y, T) and the concentration Jacobian from ONE pass over the reactions: each rate of
+    // progress goes to the production rates and its derivatives to the wave's partial sh.J[wave]
+    // (cleared after rhs_begin's barrier, when no wave still reads the previous LU factors)
+    const RhsPre pre = rhs_begin<NSP, NW>(th, sh, rho, y_l, T, wave);
     for (int e = lane; e < N * LDJ; e += 64) sh.J[wave][e] = 0.0;
     double *Jw = sh.J[wave];
     double *J = sh.J[0];
-    const double ctot = wsum(lane < NSP ? sh.c[lane] : 0.0);
-    __syncthreads();
+    const double ctot = pre.ctot;
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
-    // d(rate of progress)/d(concentration) of two reactions per lane per pass (rows i and i + 64):
-    // their chains are independent and overlap; the scatter follows in row order
+    // two reactions per lane per pass (rows i and i + 64 NW): their chains are independent and
+    // overlap; the scatters follow in row order
     constexpr int RB = 2;
     for (int i0 = wave * 64 + lane; i0 < K->ii; i0 += 64 * NW * RB) {
         double dq[RB][6];
+        Rop rr[RB];
 #pragma unroll
         for (int u = 0; u < RB; ++u) {
             const int i = min(i0 + 64 * NW * u, K->ii - 1);
             const int *nu = K->nu[i], *nk = K->nk[i];
             const Rop r = reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
+            rr[u] = r;
             for (int s = 0; s < 6; ++s) {
                 dq[u][s] = 0.0;
                 if (nk[s] < 0) continue;
@@ -278,6 +309,7 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
         for (int u = 0; u < RB; ++u) {
             const int i = i0 + 64 * NW * u;
             if (i >= K->ii) continue;
+            rhs_scatter<NSP, NW>(K, sh, i, rr[u], false, wave);
             const int *nu = K->nu[i], *nk = K->nk[i];
             if (nu[0] == 0) continue;
             for (int s = 0; s < 6; ++s) {
@@ -287,7 +319,7 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
             }
         }
     }
-    __syncthreads();
+    const RhsOut f0 = rhs_end<NSP, NW>(th, sh, rho, y_l, pre, false);   // its barrier also completes J
     if (NW > 1) {
         for (int e = threadIdx.x; e < N * LDJ; e += 64 * NW) {
             double v = sh.J[0][e];
@@ -344,12 +376,16 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
     bool ok = true;
     int kl = -1;          // column of this lane's multiplier still to be stored (-1: none)
     double l = 0.0;
+    // this row's entry in the current pivot column, carried in a register from step to step: every
+    // wave applies the step's update to column k + 1 itself (the same fused multiply-add the wave
+    // owning that column stores), so the pivot search never waits for LDS
+    double rk = lane < N ? row[0] : 0.0;
     for (int k = 0; k < N; ++k) {
         if (wave == 0 && kl >= 0) row[kl] = l;
         kl = -1;
         // pivot: the unpivoted row with the largest |entry| in column k, the lowest such row on ties
         const bool open = rank == N;
-        const double v = open ? fabs(row[k]) : -1.0;
+        const double v = open ? fabs(rk) : -1.0;
         const double vmax = wave_max(v);
         unsigned long long hit = __ballot(open && v == vmax);
         if (hit == 0ull) {                      // column k of every open row is NaN
@@ -359,17 +395,22 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
         const int p = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
         if (lane == p) rank = k;
         const double *prow = sh.J[0] + p * LDJ;
-        const double piv = prow[k];
+        const double piv = bcast(rk, p);
+        const int k1 = k + 1 < N ? k + 1 : k;
+        const double next_r = row[k1], next_p = prow[k1];
         if (wave == 0 && lane == 0) {
             sh.perm[k] = p;
             sh.rdiag[k] = 1.0 / piv;
         }
         if (!(piv != 0.0)) ok = false;
         if (rank == N) {
-            l = row[k] / piv;
+            l = rk / piv;
             kl = k;
             const int len = N - (k + 1);
             eliminate(row, prow, l, k + 1 + len * wave / NW, k + 1 + len * (wave + 1) / NW);
+            rk = fma(-l, next_p, next_r);
+        } else {
+            rk = next_r;
         }
         __syncthreads();
     }
@@ -392,19 +433,44 @@ __device__ __forceinline__ void copy_j(double *dst, const double *src, int n) {
 template <int NSP, int NW>
 __device__ __forceinline__ double solve(const Shared<NSP, NW> &sh, double b) {
     constexpr int N = NSP + 1;
+    static_assert(N + 7 <= LDJ, "the eight-step loads read up to column N + 6 of a row");
     const int lane = lane_id();
     const int pr = lane < N ? sh.perm[lane] : 0;
     const double *row = sh.J[0] + pr * LDJ;
     double bq = __shfl(b, pr, 64);
-    for (int k = 0; k < N; ++k) {
-        const double bp = bcast(bq, k);
-        if (lane > k && lane < N) bq = fma(-row[k], bp, bq);
+    // the factors of eight steps are loaded together, ahead of the chain that consumes them (row
+    // storage is LDJ > N + 7 wide, so the reads past column N - 1 stay inside the row)
+    for (int k0 = 0; k0 < N; k0 += 8) {
+        double rv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rv[u] = row[k0 + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = k0 + u;
+            if (k < N) {
+                const double bp = bcast(bq, k);
+                if (lane > k && lane < N) bq = fma(-rv[u], bp, bq);
+            }
+        }
     }
     double x = 0.0;
-    for (int k = N - 1; k >= 0; --k) {
-        const double xk = bcast(bq, k) * sh.rdiag[k];
-        if (lane == k) x = xk;
-        if (lane < k) bq = fma(-row[k], xk, bq);
+    for (int k1 = N - 1; k1 >= 0; k1 -= 8) {
+        double rv[8], dv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = k1 - u >= 0 ? k1 - u : 0;
+            rv[u] = row[k];
+            dv[u] = sh.rdiag[k];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = k1 - u;
+            if (k >= 0) {
+                const double xk = bcast(bq, k) * dv[u];
+                if (lane == k) x = xk;
+                if (lane < k) bq = fma(-rv[u], xk, bq);
+            }
+        }
     }
     return x;
 }
