@@ -231,7 +231,9 @@ __device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const De
     const RhsPre pre = rhs_begin<NSP, NW>(th, sh, rho, y_l, T, wave);
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
     for (int i = wave * 64 + lane_id(); i < K->ii; i += 64 * NW) {
-        const Rop r = reaction<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
+        // rows of a pass from K->nplain0 on take the branch-free evaluation (a uniform test)
+        const Rop r = i - lane_id() >= K->nplain0 ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
+                                                  : reaction<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
         rhs_scatter<NSP, NW>(K, sh, i, r, split, wave);
     }
     return rhs_end<NSP, NW>(th, sh, rho, y_l, pre, split);
@@ -292,7 +294,9 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
         for (int u = 0; u < RB; ++u) {
             const int i = min(i0 + 64 * NW * u, K->ii - 1);
             const int *nu = K->nu[i], *nk = K->nk[i];
-            const Rop r = reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
+            const Rop r = i0 - lane + 64 * NW * u >= K->nplain0
+                              ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
+                              : reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
             rr[u] = r;
             for (int s = 0; s < 6; ++s) {
                 dq[u][s] = 0.0;

@@ -10,6 +10,7 @@ namespace fcfd {
 // CKWYR's summation order (slot-major, then reaction).
 struct DevKin {
     int ii, kk, nthb, nfal, ncsr;
+    int nplain0;                        // rows nplain0 .. ii-1 suit reaction_plain (chem_device.hpp)
     double small, exparg;
     double A[MAXR], B[MAXR], E[MAXR];   // PAR(1..3): A, temperature exponent, E/R
     int rev[MAXR];                      // NSPEC > 0: reverse rate from equilibrium

@@ -114,4 +114,49 @@ __device__ __forceinline__ Rop reaction(const DevKin *__restrict__ K, int i, dou
     return r;
 }
 
+// x**m for 0 <= m <= 3 without branches, with ipow's product order (x**3 = x * (x * x)).
+__device__ __forceinline__ double ipow3(double x, int m) {
+    const double x2 = x * x;
+    const double x3 = x * x2;
+    return m == 1 ? x : (m == 2 ? x2 : (m == 3 ? x3 : 1.0));
+}
+
+// reaction() for a row without third body or fall-off whose coefficients and coefficient sum are at
+// most 3 in magnitude (rows K->nplain0 .. ii - 1), written without branches: the conditional terms
+// become selects of values computed anyway, and every select reproduces reaction()'s arithmetic (a
+// factor of exactly 1.0 where reaction() skips a multiplication).  Straight-line code lets the
+// compiler interleave the rate chains of several rows.
+template <int STRIDE>
+__device__ __forceinline__ Rop reaction_plain(const DevKin *__restrict__ K, int i, double T, double alogt, double pfac,
+                                              const double *__restrict__ c, const double *__restrict__ smh) {
+    const double kf = K->A[i] * exp(K->B[i] * alogt - K->E[i] / T);
+    const int *nu = K->nu[i], *nk = K->nk[i];
+    int k[6], a[6];
+    double s = 0.0;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+        k[n] = nk[n];
+        a[n] = nu[n];
+        const double t = (double)a[n] * smh[(k[n] >= 0 ? k[n] : 0) * STRIDE];
+        s = k[n] >= 0 ? s + t : s;
+    }
+    const double ex = exp(fmin(s, K->exparg));
+    double eqk = s != 0.0 ? ex : 1.0;
+    const int ns = K->nus[i];
+    const double pp = ipow3(pfac, ns < 0 ? -ns : ns);
+    eqk = eqk * (ns < 0 ? 1.0 / pp : pp);
+    const double kr = K->rev[i] ? kf / fmax(eqk, K->small) : 0.0;
+    double cc[6];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) cc[n] = ipow3(c[(k[n] >= 0 ? k[n] : 0) * STRIDE], a[n] < 0 ? -a[n] : a[n]);
+    double qf = kf * cc[0];
+    double qr = kr * cc[3];
+    qf = qf * (k[1] >= 0 ? cc[1] : 1.0);
+    qf = qf * (k[1] >= 0 && k[2] >= 0 ? cc[2] : 1.0);
+    qr = qr * (k[4] >= 0 ? cc[4] : 1.0);
+    qr = qr * (k[4] >= 0 && k[5] >= 0 ? cc[5] : 1.0);
+    const bool live = a[0] != 0;
+    return Rop{kf, kr, live ? qf : 0.0, live ? qr : 0.0};
+}
+
 }  // namespace fcfd

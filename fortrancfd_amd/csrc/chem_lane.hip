@@ -93,13 +93,23 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
     constexpr int RB = 4;
     for (int i0 = w; i0 < K->ii; i0 += NW * RB) {
         double q[RB];
+        if (i0 >= K->nplain0) {
+            // branch-free rows: the RB rate chains are straight-line code and interleave
 #pragma unroll
-        for (int u = 0; u < RB; ++u) {
-            const int i = i0 + NW * u;
-            q[u] = 0.0;
-            if (i < K->ii) {
-                const Rop r = reaction<64>(K, i, T, lnT, pfac, ctot, Cp, Gp);
-                q[u] = r.qf - r.qr;
+            for (int u = 0; u < RB; ++u) {
+                const int i = i0 + NW * u;
+                const Rop r = reaction_plain<64>(K, i < K->ii ? i : K->ii - 1, T, lnT, pfac, Cp, Gp);
+                q[u] = i < K->ii ? r.qf - r.qr : 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int i = i0 + NW * u;
+                q[u] = 0.0;
+                if (i < K->ii) {
+                    const Rop r = reaction<64>(K, i, T, lnT, pfac, ctot, Cp, Gp);
+                    q[u] = r.qf - r.qr;
+                }
             }
         }
 #pragma unroll

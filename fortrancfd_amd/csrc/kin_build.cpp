@@ -8,8 +8,10 @@
 namespace fcfd {
 
 // Device rows are the mechanism's reactions re-ordered by kind -- fall-off first, then plain
-// third-body, then the rest -- so a wave, which evaluates 64 consecutive rows at once, runs the
-// fall-off and third-body code paths in as few row chunks as possible.  Every per-reaction result
+// third-body, then the remaining reactions whose coefficients exceed 3 in magnitude (or whose
+// product coefficients are negative), then the rest -- so a wave, which evaluates 64 consecutive
+// rows at once, runs the fall-off and third-body code paths in as few row chunks as possible, and
+// the last group (from row nplain0) can take the branch-free evaluation.  Every per-reaction result
 // is independent of the row order; only the order of the species-sum additions changes.
 void build_dev_kin(const Mech &m, DevKin &k) {
     std::memset(&k, 0, sizeof(DevKin));
@@ -22,11 +24,22 @@ void build_dev_kin(const Mech &m, DevKin &k) {
     std::vector<int> tb_of(m.ii, -1), fo_of(m.ii, -1), order;
     for (int t = 0; t < k.nthb; ++t) tb_of[m.itb[t] - 1] = t;
     for (int f = 0; f < k.nfal; ++f) fo_of[m.ifl[f] - 1] = f;
-    for (int pass = 0; pass < 3; ++pass)
+    auto small = [&](int i) {
+        int sum = 0;
+        for (int n = 0; n < 6; ++n) {
+            const int v = m.nu[size_t(i) * 6 + n];
+            if (v > 3 || v < -3 || (n >= 3 && v < 0)) return false;
+            sum += v;
+        }
+        return sum >= -3 && sum <= 3;
+    };
+    for (int pass = 0; pass < 4; ++pass) {
+        if (pass == 3) k.nplain0 = int(order.size());
         for (int i = 0; i < m.ii; ++i) {
-            const int kind = fo_of[i] >= 0 ? 0 : (tb_of[i] >= 0 ? 1 : 2);
+            const int kind = fo_of[i] >= 0 ? 0 : (tb_of[i] >= 0 ? 1 : (small(i) ? 3 : 2));
             if (kind == pass) order.push_back(i);
         }
+    }
     const int np1 = m.npar + 1;
     for (int r = 0; r < m.ii; ++r) {
         const int i = order[r];
