@@ -483,6 +483,9 @@ def main():
            "ms_per_step": line.pop("ms_per_step", None), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
     out.update(line)
+    prop = torch.cuda.get_device_properties(0)
+    out["device"] = {"name": prop.name, "compute_units": prop.multi_processor_count,
+                     "arch": getattr(prop, "gcnArchName", "")}
     print(json.dumps(out), flush=True)
 
 

@@ -119,6 +119,15 @@ struct Shared {
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
+// Rows are evaluated in blocks of 64, one row per lane.  With one wave per cell, wave 0 takes every
+// block.  With NW > 1 waves, wave 0 takes block 0 alone -- rows 0..63, where the fall-off and
+// third-body rows sit and every path of reaction() is taken -- and waves 1..NW-1 take the other
+// blocks in turn, so the heavy block runs beside the plain ones instead of before one of them.
+template <int NW>
+__device__ __forceinline__ int first_block(int wave) { return wave; }
+template <int NW>
+__device__ __forceinline__ int block_step(int wave) { return NW == 1 ? 1 : (wave == 0 ? (1 << 20) : NW - 1); }
+
 struct RhsOut {
     double f;      // this lane's component of dy/dt
     double cvm;    // mixture cv, erg/(g K)
@@ -130,7 +139,7 @@ struct RhsOut {
 // f = dy/dt at (y_l, T); y_l is this lane's component (lane < NSP: Y_lane), the same in every wave
 // of the workgroup.  Leaves the concentrations in sh.c, S/R-H/RT in sh.smh and u_k in sh.u; with
 // split, returns CKCDYR's creation / destruction rates (cklib.f:1497-1517) instead of using them.
-// Wave w evaluates reactions w*64 + lane, (w + NW)*64 + lane, ... and scatters their rates of
+// Each wave evaluates its blocks of rows (first_block / block_step) and scatters their rates of
 // progress into its own partial sums with LDS atomics (one wave's atomics land in a fixed order);
 // the partials are then added in wave order, so the result is reproducible.  In three pieces so
 // that the Jacobian can share the reaction loop: rhs_begin (species terms, ends with a barrier),
@@ -230,10 +239,12 @@ __device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const De
                       double rho, double y_l, double T, bool split, int wave) {
     const RhsPre pre = rhs_begin<NSP, NW>(th, sh, rho, y_l, T, wave);
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
-    for (int i = wave * 64 + lane_id(); i < K->ii; i += 64 * NW) {
-        // rows of a pass from K->nplain0 on take the branch-free evaluation (a uniform test)
-        const Rop r = i - lane_id() >= K->nplain0 ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
-                                                  : reaction<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
+    for (int b = first_block<NW>(wave); b * 64 < K->ii; b += block_step<NW>(wave)) {
+        const int i = b * 64 + lane_id();
+        if (i >= K->ii) continue;
+        // blocks from K->nplain0 on take the branch-free evaluation (a uniform test)
+        const Rop r = b * 64 >= K->nplain0 ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
+                                           : reaction<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
         rhs_scatter<NSP, NW>(K, sh, i, r, split, wave);
     }
     return rhs_end<NSP, NW>(th, sh, rho, y_l, pre, split);
@@ -284,17 +295,19 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
     double *J = sh.J[0];
     const double ctot = pre.ctot;
     const double alogt = log(T), pfac = th->pa / (th->ru * T);
-    // two reactions per lane per pass (rows i and i + 64 NW): their chains are independent and
-    // overlap; the scatters follow in row order
+    // two of the wave's blocks per pass: their rate chains are independent and overlap; the
+    // scatters follow in block order
     constexpr int RB = 2;
-    for (int i0 = wave * 64 + lane; i0 < K->ii; i0 += 64 * NW * RB) {
+    const int bs = block_step<NW>(wave);
+    for (int b0 = first_block<NW>(wave); b0 * 64 < K->ii; b0 += bs * RB) {
         double dq[RB][6];
         Rop rr[RB];
 #pragma unroll
         for (int u = 0; u < RB; ++u) {
-            const int i = min(i0 + 64 * NW * u, K->ii - 1);
+            const int bu = b0 + bs * u;
+            const int i = min(bu * 64 + lane, K->ii - 1);
             const int *nu = K->nu[i], *nk = K->nk[i];
-            const Rop r = i0 - lane + 64 * NW * u >= K->nplain0
+            const Rop r = bu * 64 >= K->nplain0
                               ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
                               : reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
             rr[u] = r;
@@ -311,7 +324,9 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
         }
 #pragma unroll
         for (int u = 0; u < RB; ++u) {
-            const int i = i0 + 64 * NW * u;
+            const int bu = b0 + bs * u;
+            if (bu * 64 >= K->ii) continue;
+            const int i = bu * 64 + lane;
             if (i >= K->ii) continue;
             rhs_scatter<NSP, NW>(K, sh, i, rr[u], false, wave);
             const int *nu = K->nu[i], *nk = K->nk[i];
@@ -402,10 +417,7 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
         const double piv = bcast(rk, p);
         const int k1 = k + 1 < N ? k + 1 : k;
         const double next_r = row[k1], next_p = prow[k1];
-        if (wave == 0 && lane == 0) {
-            sh.perm[k] = p;
-            sh.rdiag[k] = 1.0 / piv;
-        }
+        if (wave == 0 && lane == 0) sh.perm[k] = p;
         if (!(piv != 0.0)) ok = false;
         if (rank == N) {
             l = rk / piv;
@@ -419,6 +431,9 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
         __syncthreads();
     }
     if (wave == 0 && kl >= 0) row[kl] = l;
+    // reciprocal pivots, all at once: the pivot of step k is entry k of row perm[k], which no later
+    // step changes
+    if (wave == 0 && lane < N) sh.rdiag[lane] = 1.0 / sh.J[0][sh.perm[lane] * LDJ + lane];
     __syncthreads();
     return ok;
 }
