@@ -689,6 +689,75 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
     }
 }
 
+// FP64 verification of surrogate candidates, ONE CANDIDATE PER WAVE with the integrator's
+// right-hand side: f at both ends of the step and the trapezoidal residual Y1 - Y0 - dt/2 (f0 + f1),
+// weighted like the integrator's error norm (1 / (atol + rtol max(|Y0|, |Y1|)), 1 / (rtol T0)).  The
+// candidate list is short (the stiff cells), so a grid of one-wave workgroups strides over it and
+// every candidate is in flight at once.
+struct VerifyArgs {
+    const int *cand_list;
+    const unsigned *ncand;
+    const double *cand;
+    int *fallback;
+    unsigned *nfallback;
+    double tol;
+};
+
+template <int NSP>
+__global__ void __launch_bounds__(64) k_chem_verify(ChemArgs a, VerifyArgs v) {
+    __shared__ Shared<NSP, 1> sh;
+    constexpr int N = NSP + 1;
+    const int lane = lane_id();
+    const DevThermo *__restrict__ th = a.thermo;
+    const DevKin *__restrict__ K = a.kin;
+    const int n = int(*v.ncand);
+    for (int q = blockIdx.x; q < n; q += gridDim.x) {
+        const int cell = v.cand_list[q];
+        long col = 0;
+        double rho_si, dt, T0, y0 = 0.0;
+        if (a.layout == 0) {
+            const int b = cell / a.ncell, i = cell % a.ncell;
+            col = long(b) * a.S + a.lv + i;
+            rho_si = a.qf[col];
+            T0 = a.qf[2 * a.P + col];
+            dt = a.dt_dev ? *a.dt_dev : a.dt;
+            if (lane < NSP) y0 = a.qf[(3 + lane) * a.P + col];
+            y0 = adjusted(y0, lane, NSP);                       // adjust_sum, as the integrator's load
+        } else {
+            rho_si = a.b_rho[cell];
+            dt = a.b_dt[cell];
+            T0 = a.b_T[cell];
+            if (lane < NSP) y0 = a.b_y[long(lane) * a.ncells + cell];
+        }
+        const double rho = rho_si * 1.e-3;
+        const double y1 = lane < N ? v.cand[long(q) * N + lane] : 0.0;   // lane NSP: T1
+        const double T1 = bcast(y1, NSP);
+        const double f0 = rhs_call<NSP, 1>(glb(th), glb(K), lds(sh), rho, y0, T0, false, 0).f;
+        const double f1 = rhs_call<NSP, 1>(glb(th), glb(K), lds(sh), rho, y1, T1, false, 0).f;
+        double r = 0.0;
+        if (lane < NSP)
+            r = (y1 - y0 - (dt * 0.5) * (f0 + f1)) * (1.0 / (a.atol + a.rtol * fmax(fabs(y0), fabs(y1))));
+        else if (lane == NSP)
+            r = (T1 - T0 - (dt * 0.5) * (f0 + f1)) * (1.0 / (a.rtol * fabs(T0)));
+        const double err = sqrt(wsum(r * r) / double(N));
+        if (err <= v.tol) {
+            if (a.layout == 0) {
+                if (lane < NSP) a.qf[(3 + lane) * a.P + col] = y1;
+                if (lane == 0) a.qf[2 * a.P + col] = T1;
+            } else {
+                if (lane < NSP) a.b_y[long(lane) * a.ncells + cell] = y1;
+                if (lane == 0) a.b_T[cell] = T1;
+            }
+            if (lane == 0) {
+                if (a.route) a.route[cell] = 1;
+                if (a.stats) a.stats[cell] = ChemCellStats{1, 0, 0, 0};
+            }
+        } else if (lane == 0) {
+            v.fallback[atomicAdd(v.nfallback, 1u)] = cell;
+        }
+    }
+}
+
 __global__ void k_zero_counter(unsigned int *c) {
     if (threadIdx.x == 0) *c = 0u;
 }
@@ -739,6 +808,19 @@ __global__ void __launch_bounds__(1024) k_heavy_first(const int *list, const uns
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, p);
     return hipGetLastError();
+}
+
+hipError_t launch_chem_verify(const ChemArgs &a, int nsp, const int *cand_list, const unsigned *ncand,
+                              const double *cand, int rows_max, int *fallback, unsigned *nfallback, double tol,
+                              hipStream_t st) {
+    // one-wave workgroups, at most four per CU (the register budget's residency)
+    const unsigned nb = unsigned(std::min(rows_max, 1024));
+    if (nb == 0) return hipSuccess;
+    VerifyArgs v{cand_list, ncand, cand, fallback, nfallback, tol};
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_verify<N>), dim3(nb), dim3(64), 0, st, a, v); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_chem_full(ChemArgs a, int nsp, int n_workgroups, int *list, unsigned *list_n, hipStream_t st) {

@@ -75,7 +75,8 @@ hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
 // FP64 verification of surrogate answers for the cells cand_list[0 .. *ncand): accepted when the
 // trapezoidal-rule residual Y1 - Y0 - dt/2 (f(Y0) + f(Y1)) of the candidate (Y1, T1) = cand[i],
 // weighted by the integrator's tolerances, has an RMS <= tol (1: the integrator's own bar); accepted
-// answers are written to the state, the others appended to the fallback list.
+// answers are written to the state, the others appended to the fallback list.  One candidate per
+// wave (chem.hip).
 hipError_t launch_chem_verify(const ChemArgs &a, int nsp, const int *cand_list, const unsigned *ncand,
                               const double *cand, int rows_max, int *fallback, unsigned *nfallback, double tol,
                               hipStream_t st);

@@ -325,139 +325,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
     }
 }
 
-// Verification of surrogate candidates, one candidate per lane (same mapping as the screen).
-struct VerifyArgs {
-    const int *cand_list;
-    const unsigned *ncand;
-    const double *cand;
-    int *fallback;
-    unsigned *nfallback;
-    double tol;
-};
-
-template <int NSP>
-__device__ __forceinline__ void verify_block(const ChemArgs &a, const VerifyArgs &v, LaneShared<NSP> &sh, int w,
-                                             int lane, int item0, int nitems);
-
-template <int NSP>
-__global__ void __launch_bounds__(256) k_chem_verify(ChemArgs a, VerifyArgs v) {
-    __shared__ LaneShared<NSP> sh;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int nitems = int(*v.ncand);
-    // a small grid strides over the candidate list (its length is only known on the device)
-    for (int item0 = blockIdx.x * 64; item0 < nitems; item0 += gridDim.x * 64)
-        verify_block<NSP>(a, v, sh, w, lane, item0, nitems);
-}
-
-template <int NSP>
-__device__ __forceinline__ void verify_block(const ChemArgs &a, const VerifyArgs &v, LaneShared<NSP> &sh, int w,
-                                             int lane, int item0, int nitems) {
-    constexpr int NS = slots<NSP>();
-    const DevThermo *__restrict__ th = a.thermo;
-    const DevKin *__restrict__ K = a.kin;
-    const int item = item0 + lane;
-    const bool live = item < nitems;
-    const int it = live ? item : item0;
-    const int cell = v.cand_list[it];
-    double *slot = sh.W[0];
-    long col = 0;
-    double rho_si, dt, T0;
-    double y[NS], y1[NS];
-    if (a.layout == 0) {
-        const int b = cell / a.ncell, i = cell % a.ncell;
-        col = long(b) * a.S + a.lv + i;
-        rho_si = a.qf[col];
-        T0 = a.qf[2 * a.P + col];
-        dt = a.dt_dev ? *a.dt_dev : a.dt;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int k = w + NW * j;
-            y[j] = k < NSP ? a.qf[(3 + k) * a.P + col] : 0.0;
-        }
-        adjust_lane<NSP>(slot, w, lane, y);
-    } else {
-        rho_si = a.b_rho[cell];
-        dt = a.b_dt[cell];
-        T0 = a.b_T[cell];
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int k = w + NW * j;
-            y[j] = k < NSP ? a.b_y[long(k) * a.ncells + cell] : 0.0;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-        const int k = w + NW * j;
-        y1[j] = k < NSP ? v.cand[long(it) * (NSP + 1) + k] : 0.0;
-    }
-    const double T1 = v.cand[long(it) * (NSP + 1) + NSP];
-    const double rho = rho_si * 1.e-3;
-    double f0[NS], f0T, f1[NS], f1T;
-    rhs_lane<NSP, false>(th, K, sh, w, lane, rho, y, y, 0.0, T0, f0, f0T);
-    rhs_lane<NSP, false>(th, K, sh, w, lane, rho, y1, y1, 0.0, T1, f1, f1T);
-    double err_w = 0.0;
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-        if (w + NW * j < NSP) {
-            const double wt = 1.0 / (a.atol + a.rtol * fmax(fabs(y[j]), fabs(y1[j])));
-            const double r = (y1[j] - y[j] - (dt * 0.5) * (f0[j] + f1[j])) * wt;
-            err_w = err_w + r * r;
-        }
-    }
-    if (w == 0) {
-        const double r = (T1 - T0 - (dt * 0.5) * (f0T + f1T)) * (1.0 / (a.rtol * fabs(T0)));
-        err_w = err_w + r * r;
-    }
-    const double err = sqrt(sum4(slot, w, lane, err_w) / double(NSP + 1));
-    const bool ok = live && err <= v.tol;
-    if (!live) return;
-    if (!ok) {
-        if (w == 0) {
-            const unsigned long long bal = __ballot(1);
-            const int leader = __ffsll((long long)bal) - 1;
-            const int rank = __popcll(bal & ((1ull << lane) - 1ull));
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(v.nfallback, unsigned(__popcll(bal)));
-            base = __shfl(base, leader, 64);
-            v.fallback[base + rank] = cell;
-        }
-        return;
-    }
-    if (a.layout == 0) {
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int k = w + NW * j;
-            if (k < NSP) a.qf[(3 + k) * a.P + col] = y1[j];
-        }
-        if (w == 0) a.qf[2 * a.P + col] = T1;
-    } else {
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            const int k = w + NW * j;
-            if (k < NSP) a.b_y[long(k) * a.ncells + cell] = y1[j];
-        }
-        if (w == 0) a.b_T[cell] = T1;
-    }
-    if (w == 0) {
-        if (a.route) a.route[cell] = 1;
-        if (a.stats) a.stats[cell] = ChemCellStats{1, 0, 0, 0};
-    }
-}
-
 }  // namespace
-
-hipError_t launch_chem_verify(const ChemArgs &a, int nsp, const int *cand_list, const unsigned *ncand,
-                              const double *cand, int rows_max, int *fallback, unsigned *nfallback, double tol,
-                              hipStream_t st) {
-    // at most one workgroup per CU (each holds 163 KB of LDS); the kernel strides over the list
-    const unsigned nb = unsigned(std::min((rows_max + 63) / 64, 256));
-    if (nb == 0) return hipSuccess;
-    VerifyArgs v{cand_list, ncand, cand, fallback, nfallback, tol};
-#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_verify<N>), dim3(nb), dim3(256), 0, st, a, v); return hipGetLastError(); }
-    FCFD_FOR_EACH_NSP(FCFD_CASE)
-#undef FCFD_CASE
-    return hipErrorInvalidValue;
-}
 
 hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st) {
     const unsigned nb = unsigned((a.ncells + 63) / 64);
