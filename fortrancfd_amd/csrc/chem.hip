@@ -838,14 +838,16 @@ hipError_t launch_chem_full(ChemArgs a, int nsp, int n_workgroups, int *list, un
 }
 
 // n_workgroups one-wave workgroups (four per CU) for long lists; a quarter as many four-wave
-// workgroups (one per CU) for lists of at most 2 * n_workgroups cells.  Both kernels are launched;
-// the one whose mode does not match the list length on the device returns at once.
+// workgroups (one per CU) for lists of at most 8 * n_workgroups cells (32 per CU).  Up to there the
+// launch is set by its slowest cells, which run about twice as fast with four waves, more than the
+// four-wave mode's lower throughput costs.  Both kernels are launched; the one whose mode does not
+// match the list length on the device returns at once.
 hipError_t launch_chem(const ChemArgs &a0, int nsp, int n_workgroups, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, a0.counter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     ChemArgs a = a0;
-    a.coop_max = 2 * n_workgroups;
+    a.coop_max = 8 * n_workgroups;
     if (a.order && a.list && a.list_n && a.stats && a.layout == 0) {
         hipLaunchKernelGGL(k_heavy_first, dim3(1), dim3(1024), 0, st, a.list, a.list_n, a.stats, a.order);
         if ((e = hipGetLastError()) != hipSuccess) return e;
