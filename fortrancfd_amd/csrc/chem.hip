@@ -581,6 +581,33 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
                 steps = 1;
             }
         }
+        // ---- explicit retry: two Heun steps of dt/2, each accepted by its own embedded Euler
+        //      estimate (weights at the step's start).  Trace radicals at the atol level in a cold
+        //      mixture relax so fast that one explicit step of dt overshoots (dt times their
+        //      consumption rate near 2) while a half step does not; such cells need no Jacobian ----
+        if (t < dt) {
+            const double hs = dt * 0.5;
+            double yh = y;
+            bool ok = true;
+            for (int sub = 0; sub < 2 && ok; ++sub) {
+                const double w = lane < NSP ? 1.0 / (a.atol + a.rtol * fabs(yh))
+                                            : (lane == NSP ? 1.0 / (a.rtol * fabs(yh)) : 0.0);
+                const RhsOut fa = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, yh, bcast(yh, NSP), false, wave);
+                const double yb = lane < N ? yh + hs * fa.f : 0.0;
+                const RhsOut fb = rhs_call<NSP, NW>(glb(th), glb(K), lds(sh), rho, yb, bcast(yb, NSP), false, wave);
+                const double err = wnorm(lane < N ? (hs * 0.5) * (fb.f - fa.f) : 0.0, w, N);
+                if (err <= 1.0) {
+                    if (lane < N) yh = yh + (hs * 0.5) * (fa.f + fb.f);
+                } else {
+                    ok = false;
+                }
+            }
+            if (ok) {
+                y = yh;
+                t = dt;
+                steps = 2;
+            }
+        }
         // ---- SDIRK3 over [0, dt] ----
         while (t < dt && steps + rejects < CH_MAXSTEPS) {
             if (h > dt - t) h = dt - t;

@@ -1267,6 +1267,30 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
             s.steps = 1;
         }
     }
+    if (t < dt) {   /* explicit retry: two Heun steps of dt/2, each within tolerance on its own */
+        const double hs = dt * 0.5;
+        double yh[65], fa[65], fb[65], yb[65];
+        memcpy(yh, y, sizeof(double) * n);
+        int ok = 1;
+        for (int sub = 0; sub < 2 && ok; ++sub) {
+            for (int i = 0; i < kk; ++i) w[i] = 1.0 / (atol + rtol * fabs(yh[i]));
+            w[kk] = 1.0 / (rtol * fabs(yh[kk]));
+            fo_chem_rhs(m, rho, yh, fa);
+            for (int i = 0; i < n; ++i) yb[i] = yh[i] + hs * fa[i];
+            fo_chem_rhs(m, rho, yb, fb);
+            for (int i = 0; i < n; ++i) e[i] = (hs * 0.5) * (fb[i] - fa[i]);
+            if (ch_norm(e, w, n) <= 1.0) {
+                for (int i = 0; i < n; ++i) yh[i] = yh[i] + (hs * 0.5) * (fa[i] + fb[i]);
+            } else {
+                ok = 0;
+            }
+        }
+        if (ok) {
+            memcpy(y, yh, sizeof(double) * n);
+            t = dt;
+            s.steps = 2;
+        }
+    }
     while (t < dt && s.steps + s.rejects < CH_MAXSTEPS) {
         if (h > dt - t) h = dt - t;
         for (int i = 0; i < kk; ++i) w[i] = 1.0 / (atol + rtol * fabs(y[i]));
