@@ -92,6 +92,7 @@ struct fcfd_ctx {
     int *d_accepted = nullptr;
     int *d_list = nullptr;               // cells the explicit screen sends to the implicit integrator
     int *d_order = nullptr;              // that list (or the surrogate's fallback list) reordered, slowest first
+    double *d_hmem = nullptr;            // per interior cell: the integrator's last step-size proposal
     unsigned *d_list_n = nullptr;
     double *acc = nullptr;               // RK4 stage accumulator
     int graph_next[3] = {0, 0, 0};       // state buffer after the captured step from buffer i
@@ -263,7 +264,7 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
     a.thermo = c->thermo; a.kin = static_cast<const DevKin *>(c->mech->d_kin); a.counter = c->d_counter;
     a.ncells = c->nblk * c->ncell; a.rtol = c->o.rtol; a.atol = c->o.atol;
     a.qf = c->qf; a.P = c->P; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S;
-    a.dt = dt; a.dt_dev = dt_dev; a.stats = c->d_stats; a.order = c->d_order;
+    a.dt = dt; a.dt_dev = dt_dev; a.stats = c->d_stats; a.order = c->d_order; a.hmem = c->d_hmem;
     if (want_hrr) { a.hrr = c->d_hrr; a.hrr_dot = c->d_hrr_dot; }
     a.layout = 0;
     a.route = c->d_accepted;
@@ -414,6 +415,8 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
         HIP_TRY(hipMemset(c->d_accepted, 0, sizeof(int) * nint));
         HIP_TRY(hipMalloc(&c->d_list, sizeof(int) * nint));
         HIP_TRY(hipMalloc(&c->d_order, sizeof(int) * nint));
+        HIP_TRY(hipMalloc(&c->d_hmem, sizeof(double) * nint));
+        HIP_TRY(hipMemset(c->d_hmem, 0, sizeof(double) * nint));
         HIP_TRY(hipMalloc(&c->d_list_n, sizeof(unsigned)));
         HIP_TRY(hipMemset(c->d_hrr, 0, sizeof(double) * nint));
         HIP_TRY(hipMemset(c->d_hrr_dot, 0, sizeof(double) * nint));
@@ -431,7 +434,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->qf); (void)hipFree(c->F); (void)hipFree(c->god); (void)hipFree(c->cells);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
-    (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_order); (void)hipFree(c->d_list_n);
+    (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_order); (void)hipFree(c->d_hmem); (void)hipFree(c->d_list_n);
     (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -451,6 +454,9 @@ int fcfd_upload(fcfd_ctx *c, int ib, const double *Qc, const double *qf) {
         HIP_TRY(hipStreamSynchronize(c->stream));
         return 0;
     };
+    if (c->d_hmem) {   // a new state: the integrator's step-size memory starts over
+        HIP_TRY(hipMemsetAsync(c->d_hmem, 0, sizeof(double) * size_t(c->nblk) * c->ncell, c->stream));
+    }
     if (int rc = put(Qc, c->Q[c->cur])) return rc;
     if (qf) {
         if (int rc = put(qf, c->qf)) return rc;
@@ -489,6 +495,7 @@ int fcfd_init_cells(fcfd_ctx *c, const double *cells) {
     a.thermo = c->thermo; a.cells = c->cells; a.Qc = c->Q[c->cur]; a.qf = c->qf;
     a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P;
     HIP_TRY(launch_init(a, c->kk, c->stream));
+    if (c->d_hmem) HIP_TRY(hipMemsetAsync(c->d_hmem, 0, sizeof(double) * size_t(c->nblk) * c->ncell, c->stream));
     // ghost fill + QcToqf on ghosts only (SRC/main.f90:96-114)
     StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
     s.ghosts_only = 1;

@@ -565,6 +565,11 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
         if (lane == NSP) y = T0;
 
         double t = 0.0, h = dt;
+        if (a.hmem) {   // the last call's proposal for this cell, if smaller than dt
+            const double hp = a.hmem[cell];
+            if (hp > 0.0 && hp < dt) h = hp;
+        }
+        bool expl = false;
         int steps = 0, rejects = 0, newton = 0;
         // ---- explicit pre-step: Heun over the whole dt, accepted when the embedded Euler error
         //      estimate is within tolerance (quiescent cells never build a Jacobian) ----
@@ -579,6 +584,7 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
                 if (lane < N) y = y + (dt * 0.5) * (f0.f + f1.f);
                 t = dt;
                 steps = 1;
+                expl = true;
             }
         }
         // ---- explicit retry: two Heun steps of dt/2, each accepted by its own embedded Euler
@@ -606,6 +612,7 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
                 y = yh;
                 t = dt;
                 steps = 2;
+                expl = true;
             }
         }
         // ---- SDIRK3 over [0, dt] ----
@@ -713,6 +720,7 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
             if (wave == 0 && lane == 0) a.b_T[cell] = T;
         }
         if (a.stats && wave == 0 && lane == 0) a.stats[cell] = ChemCellStats{steps, rejects, newton, failed};
+        if (a.hmem && wave == 0 && lane == 0) a.hmem[cell] = expl ? 0.0 : h;
     }
 }
 
@@ -778,6 +786,7 @@ __global__ void __launch_bounds__(64) k_chem_verify(ChemArgs a, VerifyArgs v) {
             if (lane == 0) {
                 if (a.route) a.route[cell] = 1;
                 if (a.stats) a.stats[cell] = ChemCellStats{1, 0, 0, 0};
+                if (a.hmem) a.hmem[cell] = 0.0;
             }
         } else if (lane == 0) {
             v.fallback[atomicAdd(v.nfallback, 1u)] = cell;

@@ -63,6 +63,9 @@ struct ChemArgs {
     // optional (domain layout with stats): buffer of the list's capacity that receives the list
     // reordered with the cells whose previous integration took the most attempts first
     int *order;
+    // optional (domain layout): per interior cell, the integrator's last step-size proposal, carried
+    // from one call to the next (0: none; the screen and the surrogate reset it for cells they finish)
+    double *hmem;
 };
 
 // SDIRK integrator over all cells or over a work list (persistent grid): one cell per wave for long

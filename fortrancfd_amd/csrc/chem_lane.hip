@@ -322,6 +322,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
     if (w == 0) {
         if (a.stats) a.stats[cell] = ChemCellStats{1, 0, 0, 0};
         if (a.route) a.route[cell] = 2;
+        if (a.hmem) a.hmem[cell] = 0.0;
     }
 }
 

@@ -93,7 +93,9 @@ int fcfd_solve_fluid(fcfd_ctx *c, double dt);
  * renormalises Y, writes T and Y into the primitive state and rebuilds the interior conserved
  * state (qfToQc).  hrr / hrr_dot (may be NULL) receive eos_obj(:)%HRR and %HRR_dot, one value per
  * interior cell, blocks in order.  Requires do_reaction = 1.  The time argument is unused (the
- * reference passes it for the ignition source, which is not part of this path). */
+ * reference passes it for the ignition source, which is not part of this path).  The context keeps,
+ * per cell, the implicit integrator's last step-size proposal and starts the next call's first step
+ * there (when below dt); fcfd_upload and fcfd_init_cells clear it. */
 int fcfd_solve_source(fcfd_ctx *c, double dt, double time, double *hrr, double *hrr_dot);
 /* Per-cell integrator counters of the last chemistry call: 4 ints per interior cell
  * (accepted steps, rejected attempts, Newton iterations, failed flag). */

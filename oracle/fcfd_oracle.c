@@ -817,6 +817,7 @@ fo_domain *fo_domain_create(const fo_mech *m, const fo_opts *o, int nblk, int nc
     d->Qc0 = calloc(nint, sizeof(double));
     d->k1 = calloc(nint, sizeof(double)); d->k2 = calloc(nint, sizeof(double));
     d->k3 = calloc(nint, sizeof(double)); d->k4 = calloc(nint, sizeof(double));
+    d->hmem = calloc((size_t)nblk * ncell, sizeof(double));
     for (int ib = 0; ib < nblk; ++ib)
         for (int i = 0; i < ncell; ++i) {
             const double *c = cells + ((size_t)ib * ncell + i) * (3 + kk);
@@ -845,7 +846,7 @@ fo_domain *fo_domain_create(const fo_mech *m, const fo_opts *o, int nblk, int nc
 void fo_domain_free(fo_domain *d) {
     if (!d) return;
     free(d->Qc); free(d->qf); free(d->Fplus); free(d->godunov);
-    free(d->Qc0); free(d->k1); free(d->k2); free(d->k3); free(d->k4);
+    free(d->Qc0); free(d->k1); free(d->k2); free(d->k3); free(d->k4); free(d->hmem);
     free(d);
 }
 
@@ -1238,8 +1239,11 @@ static void ch_energy(const fo_mech *m, const double *y, double T, double *e, do
     *cv = c;
 }
 
-int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double rtol, double atol,
-                 fo_chem_stats *st) {
+/* One integration over dt; h_io (may be NULL) carries the step-size proposal from one call to the
+ * next for the same cell: in, the first SDIRK step size if positive and below dt (else dt); out, the
+ * controller's proposal after the last step, or 0 when an explicit step finished the cell. */
+static int chem_step_h(const fo_mech *m, double dt, double rho_si, double *yT, double rtol, double atol,
+                       double *h_io, fo_chem_stats *st){
     const int kk = m->kk, n = kk + 1;
     const double rho = rho_si * 1.e-3, g = CH_GAMMA;
     double y[65], w[65], f[65], z[65], base[65], k[3][65], r[65], e[65], ya[64];
@@ -1253,6 +1257,8 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
     fo_adjust_sum(kk, ya);
     ch_energy(m, ya, y[kk], &e0, &cv0);
     double t = 0.0, h = dt;
+    if (h_io && *h_io > 0.0 && *h_io < dt) h = *h_io;
+    int expl = 0;
     {   /* explicit pre-step: Heun over dt, accepted when the embedded Euler error is in tolerance */
         double f0[65], f1[65], y1[65];
         for (int i = 0; i < kk; ++i) w[i] = 1.0 / (atol + rtol * fabs(y[i]));
@@ -1265,6 +1271,7 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
             for (int i = 0; i < n; ++i) y[i] = y[i] + (dt * 0.5) * (f0[i] + f1[i]);
             t = dt;
             s.steps = 1;
+            expl = 1;
         }
     }
     if (t < dt) {   /* explicit retry: two Heun steps of dt/2, each within tolerance on its own */
@@ -1289,6 +1296,7 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
             memcpy(y, yh, sizeof(double) * n);
             t = dt;
             s.steps = 2;
+            expl = 1;
         }
     }
     while (t < dt && s.steps + s.rejects < CH_MAXSTEPS) {
@@ -1350,6 +1358,7 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
         }
     }
     if (t < dt) s.failed = 1;
+    if (h_io) *h_io = expl ? 0.0 : h;
     /* energy correction: sum Y u(T)/W with the renormalised composition equals its initial value */
     memcpy(ya, y, sizeof(double) * kk);
     fo_adjust_sum(kk, ya);
@@ -1369,6 +1378,11 @@ int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double 
 
 /* reaction_macks (SRC/source.f90:17-84) on every interior cell of a domain; hrr / hrr_dot hold
  * one value per interior cell (block-major), as eos_obj(i)%HRR / %HRR_dot. */
+int fo_chem_step(const fo_mech *m, double dt, double rho_si, double *yT, double rtol, double atol,
+                 fo_chem_stats *st) {
+    return chem_step_h(m, dt, rho_si, yT, rtol, atol, NULL, st);
+}
+
 void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *hrr,
                      double *hrr_dot, int *nfail) {
     const fo_mech *m = d->m;
@@ -1382,7 +1396,7 @@ void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *
             memcpy(yT, q + 3, sizeof(double) * kk);
             fo_adjust_sum(kk, yT);
             yT[kk] = q[2];
-            if (fo_chem_step(m, dt, q[0], yT, rtol, atol, NULL) != 0) ++fails;
+            if (chem_step_h(m, dt, q[0], yT, rtol, atol, d->hmem + (size_t)ib * d->ncell + i, NULL) != 0) ++fails;
             fo_adjust_sum(kk, yT);
             q[2] = yT[kk];
             memcpy(q + 3, yT, sizeof(double) * kk);

@@ -98,6 +98,7 @@ typedef struct {
     double *Fplus;       /* nblk * (ncell+1) * nequ        */
     double *godunov;     /* nblk * (ncell+1) * 3           */
     double *Qc0, *k1, *k2, *k3, *k4;  /* interior work   */
+    double *hmem;        /* nblk * ncell: the integrator's last step-size proposal (0: none) */
 } fo_domain;
 
 /* Allocates and initialises from (T, p, u, X) per interior cell the way initialize_variables /
