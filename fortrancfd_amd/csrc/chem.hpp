@@ -6,10 +6,9 @@
 namespace fcfd {
 
 // Reaction data in the form CKRATT / CKRATX / CKWYR consume it (cklib.f:6630-6993, 9107-9204),
-// re-indexed 0-based with one row per reaction, plus per-species accumulation lists that keep
-// CKWYR's summation order (slot-major, then reaction).
+// re-indexed 0-based with one row per reaction.
 struct DevKin {
-    int ii, kk, nthb, nfal, ncsr;
+    int ii, kk, nthb, nfal;
     int nplain0;                        // rows nplain0 .. ii-1 suit reaction_plain (chem_device.hpp)
     double small, exparg;
     double A[MAXR], B[MAXR], E[MAXR];   // PAR(1..3): A, temperature exponent, E/R
@@ -21,9 +20,6 @@ struct DevKin {
     double tba[MAXR][MAXTB];            // their efficiencies
     int ifop[MAXR], kfal[MAXR];         // fall-off form (1 Lindemann, 2 SRI, 3/4 Troe), [M] species
     double fpar[MAXR][8];
-    int csr_start[MAXK + 1];
-    int csr_rxn[MAXR * 6];
-    double csr_nu[MAXR * 6];
 };
 
 void build_dev_kin(const Mech &m, DevKin &k);

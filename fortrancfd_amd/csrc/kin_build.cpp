@@ -69,22 +69,6 @@ void build_dev_kin(const Mech &m, DevKin &k) {
         k.kfal[f] = m.kfl[f] - 1;
         for (int c = 0; c < 8 && c < m.nfar; ++c) k.fpar[f][c] = m.fpar[size_t(f) * m.nfar + c];
     }
-    // production-rate lists per species (CKWYR's slot-major, reaction-minor order)
-    int e = 0;
-    for (int sp = 0; sp < m.kk; ++sp) {
-        k.csr_start[sp] = e;
-        for (int n = 0; n < 6; ++n)
-            for (int i = 0; i < m.ii; ++i)
-                if (m.nk[size_t(i) * 6 + n] - 1 == sp) {
-                    int r = 0;
-                    while (order[r] != i) ++r;
-                    k.csr_rxn[e] = r;
-                    k.csr_nu[e] = double(m.nu[size_t(i) * 6 + n]);
-                    ++e;
-                }
-    }
-    k.csr_start[m.kk] = e;
-    k.ncsr = e;
 }
 
 // CKHMS(298.15) (cklib.f:3598-3618): formation enthalpies for the heat-release diagnostics.
