@@ -6,13 +6,16 @@
 //   embedded order-2 solution, simplified Newton per stage with M = I - h*gamma*J (LU with partial
 //   pivoting), filtered error M^{-1} h sum (b - b^)_i k_i, analytic mass-action Jacobian (third-body
 //   and fall-off factors held fixed) with a finite-difference temperature column, and a final
-//   temperature correction that keeps sum_k Y_k u_k(T) / W_k at its initial value.
+//   temperature correction that keeps sum_k Y_k u_k(T) / W_k at its initial value.  Before SDIRK,
+//   an explicit Heun step over dt (unless the screen already tried it) and a retry with two Heun
+//   steps of dt/2; SDIRK's first step starts from the cell's step-size proposal of the last call.
 //
-// GPU mapping: ONE CELL PER WAVEFRONT.  A persistent grid pulls cells from a device work queue, so
-// cells of very different stiffness (1 step vs. hundreds) balance themselves.  Inside a wave,
-// lanes run over reactions (rates, Jacobian scatter) and species (thermo, production rates); the
-// 54x54 Newton matrix lives one row per lane in VGPRs and is factored by pivoted elimination
-// across lanes; vectors of the state are distributed one component per lane.
+// GPU mapping: a persistent grid pulls cells from a device work queue, so cells of very different
+// stiffness (1 step vs. hundreds) balance themselves; one cell per wave for long lists, one cell per
+// four-wave workgroup for short ones (k_chem<NSP, NW>).  Lanes run over reactions (rates, Jacobian
+// scatter) and species (thermo, production rates); the 54x54 Newton matrix lives in LDS one row per
+// lane and is factored by pivoted elimination across lanes; vectors of the state are distributed one
+// component per lane.
 #include <algorithm>
 
 #include "chem.hpp"
