@@ -442,7 +442,7 @@ __global__ void k_qc_to_qf(const DevThermo *__restrict__ m, int n, const double 
 // qfToQc on the interior columns of a padded domain (reaction_macks' final loop,
 // SRC/source.f90:78-82).
 template <int NSP>
-__global__ void k_qf_to_qc_interior(const DevThermo *__restrict__ m, int nblk, int ncell, int lv, int S,
+__global__ void __launch_bounds__(64) k_qf_to_qc_interior(const DevThermo *__restrict__ m, int nblk, int ncell, int lv, int S,
                                     const double *__restrict__ qf, double *__restrict__ Qc, long P) {
     const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= long(nblk) * ncell) return;
