@@ -63,8 +63,24 @@ __device__ __forceinline__ Rop reaction(const DevKin *__restrict__ K, int i, dou
     double ctb = 1.0;
     const int t = K->tb[i];
     if (t >= 0) {
+        // the row's efficiencies are loaded eight at a time, so their memory latency is paid once per
+        // eight entries rather than once per entry; the sum keeps the entry order
         ctb = ctot;
-        for (int l = 0; l < K->ntbs[t]; ++l) ctb = ctb + (K->tba[t][l] - 1.0) * c[K->tbk[t][l] * STRIDE];
+        const int nl = K->ntbs[t];
+        constexpr int TB = 8;
+        for (int l0 = 0; l0 < nl; l0 += TB) {
+            double ta[TB];
+            int tk[TB];
+#pragma unroll
+            for (int u = 0; u < TB; ++u) {
+                const int l = min(l0 + u, MAXTB - 1);
+                ta[u] = K->tba[t][l];
+                tk[u] = K->tbk[t][l];
+            }
+#pragma unroll
+            for (int u = 0; u < TB; ++u)
+                if (l0 + u < nl) ctb = ctb + (ta[u] - 1.0) * c[(tk[u] >= 0 ? tk[u] : 0) * STRIDE];
+        }
     }
     const int f = K->fo[i];
     if (f >= 0) {
