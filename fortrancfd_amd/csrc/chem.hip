@@ -377,10 +377,12 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
 // LU with partial pivoting of M = I - hg*J, in place in sh.J[0]: lane r owns row r (LDS row stride
 // odd in 64-bit words, so the lanes of one column hit distinct banks).  Rows are not swapped:
 // sh.perm[k] is the pivot row of step k; ties go to the lower row.  Every wave finds the same pivot
-// and multipliers; the update of the trailing columns is split into NW contiguous column ranges,
-// one per wave.  A step's multipliers are stored into column k at the start of the next step, once
-// every wave has read that column.  Returns false if a pivot is zero.  The loops stay rolled
-// (unrolled, the 55-row factorisation is some 66 kB of straight-line code).
+// and multipliers.  Column k + 1, the next pivot column, is updated by every wave in a register and
+// not stored during step k; the update of columns k + 2 .. N - 1 is split into NW contiguous column
+// ranges, one per wave, so between two barriers no wave writes a column that another wave reads.
+// Column k gets its final values at the start of step k + 1, after the barrier: the multiplier in
+// each row still open, the pivot itself in the pivot row.  Returns false if a pivot is zero.  The
+// loops stay rolled (unrolled, the 55-row factorisation is some 66 kB of straight-line code).
 template <int NSP, int NW>
 __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
     constexpr int N = NSP + 1;
@@ -396,15 +398,18 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
     }
     __syncthreads();
     bool ok = true;
-    int kl = -1;          // column of this lane's multiplier still to be stored (-1: none)
-    double l = 0.0;
+    int kl = -1;          // column of this lane's final value still to be stored (-1: none)
+    double l = 0.0;       // that value: the multiplier, or the pivot in the pivot row
     // this row's entry in the current pivot column, carried in a register from step to step: every
-    // wave applies the step's update to column k + 1 itself (the same fused multiply-add the wave
-    // owning that column stores), so the pivot search never waits for LDS
+    // wave applies the step's update to column k + 1 itself, so the pivot search never waits for LDS
     double rk = lane < N ? row[0] : 0.0;
     for (int k = 0; k < N; ++k) {
         if (wave == 0 && kl >= 0) row[kl] = l;
         kl = -1;
+        // this row's entry in column k + 1 after the updates of steps < k (stored by the owner of
+        // that column in step k - 1), read before the pivot search so that the read overlaps it
+        const int k1 = k + 1 < N ? k + 1 : k;
+        const double next_r = row[k1];
         // pivot: the unpivoted row with the largest |entry| in column k, the lowest such row on ties
         const bool open = rank == N;
         const double v = open ? fabs(rk) : -1.0;
@@ -415,21 +420,21 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
             ok = false;
         }
         const int p = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
-        if (lane == p) rank = k;
         const double *prow = sh.J[0] + p * LDJ;
         const double piv = bcast(rk, p);
-        const int k1 = k + 1 < N ? k + 1 : k;
-        const double next_r = row[k1], next_p = prow[k1];
+        const double next_p = bcast(next_r, p);   // the pivot row's entry in column k + 1
         if (wave == 0 && lane == 0) sh.perm[k] = p;
         if (!(piv != 0.0)) ok = false;
-        if (rank == N) {
+        if (lane == p) {
+            rank = k;
+            l = rk;
+            kl = k;
+        } else if (rank == N) {
             l = rk / piv;
             kl = k;
-            const int len = N - (k + 1);
-            eliminate(row, prow, l, k + 1 + len * wave / NW, k + 1 + len * (wave + 1) / NW);
+            const int len = max(N - (k + 2), 0);
+            eliminate(row, prow, l, k + 2 + len * wave / NW, k + 2 + len * (wave + 1) / NW);
             rk = fma(-l, next_p, next_r);
-        } else {
-            rk = next_r;
         }
         __syncthreads();
     }
