@@ -446,8 +446,10 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
     return ok;
 }
 
-// dst[0 .. n) = src[0 .. n), by the whole workgroup.
+// dst[0 .. n) = src[0 .. n), by the whole workgroup.  The leading barrier makes every wave finish
+// its reads of dst first (a rejected attempt restores J over the factors its last solve read).
 __device__ __forceinline__ void copy_j(double *dst, const double *src, int n) {
+    __syncthreads();
     for (int e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
     __syncthreads();
 }
