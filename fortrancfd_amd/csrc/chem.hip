@@ -89,6 +89,19 @@ __device__ __forceinline__ double wave_max(double v) {
     return bcast(v, 63);
 }
 
+// Largest value over the wave of an unsigned key, the same scheme as wave_max with one DPP max per
+// stage (lanes without a source and rows outside the mask contribute 0, the identity).  Every lane
+// of the wave must be active.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, true));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, true));
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // row[j] = fma(-l, prow[j], row[j]) for j in [j0, n): the pivot row is never the row updated, so
 // the loads of a group of eight do not wait for the stores of the previous group.
 __device__ __forceinline__ void eliminate(double *__restrict__ row, const double *__restrict__ prow, double l,
@@ -390,19 +403,25 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
     int rank = lane < N ? N : -1;   // elimination step of this lane's row (N: not pivoted yet)
     double *row = sh.J[0] + (lane < N ? lane : 0) * LDJ;
     if (lane < N) {
-        for (int j = wave; j < N; j += NW) {
+        for (int j = 1 + wave; j < N; j += NW) {
             double m = -hg * row[j];
             if (j == lane) m = m + 1.0;
             row[j] = m;
         }
     }
+    // this row's entry in the current pivot column, carried in a register from step to step: every
+    // wave applies the step's update to column k + 1 itself, so the pivot search never waits for LDS.
+    // Column 0 is formed here, in registers only (read before the barrier, so no load of it is still
+    // outstanding inside the loop), and gets its final values at step 1 like every pivot column.
+    double rk = 0.0;
+    if (lane < N) {
+        rk = -hg * row[0];
+        if (lane == 0) rk = rk + 1.0;
+    }
     __syncthreads();
     bool ok = true;
     int kl = -1;          // column of this lane's final value still to be stored (-1: none)
     double l = 0.0;       // that value: the multiplier, or the pivot in the pivot row
-    // this row's entry in the current pivot column, carried in a register from step to step: every
-    // wave applies the step's update to column k + 1 itself, so the pivot search never waits for LDS
-    double rk = lane < N ? row[0] : 0.0;
     for (int k = 0; k < N; ++k) {
         if (wave == 0 && kl >= 0) row[kl] = l;
         kl = -1;
@@ -411,10 +430,21 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
         const int k1 = k + 1 < N ? k + 1 : k;
         const double next_r = row[k1];
         // pivot: the unpivoted row with the largest |entry| in column k, the lowest such row on ties
+        // (NaN entries are passed over).  For |x| >= 0 the upper 32 bits of the double order like the
+        // value, so one 32-bit wave max finds the candidates; only when several rows share those bits
+        // are their full values compared.
         const bool open = rank == N;
-        const double v = open ? fabs(rk) : -1.0;
-        const double vmax = wave_max(v);
-        unsigned long long hit = __ballot(open && v == vmax);
+        const double av = fabs(rk);
+        const bool num = open && av == av;
+        const unsigned key = num ? (unsigned)__double2hiint(av) : 0u;
+        const unsigned kmax = wave_max_u32(key);   // outside any condition: every lane takes part
+        const bool top = num && key == kmax;
+        unsigned long long hit = __ballot(top);
+        if (__popcll(hit) > 1) {
+            const double v = top ? av : -1.0;
+            const double vmax = wave_max(v);
+            hit = __ballot(top && v == vmax);
+        }
         if (hit == 0ull) {                      // column k of every open row is NaN
             hit = __ballot(open);
             ok = false;
