@@ -32,7 +32,7 @@ using namespace fcfd;
 
 namespace {
 
-constexpr int NTIMERS = 8;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry SDIRK, 4 source Qc refresh, 5 surrogate, 6 explicit screen, 7 transport flux
+constexpr int NTIMERS = 9;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry SDIRK, 4 source Qc refresh, 5 surrogate, 6 explicit screen, 7 transport flux, 8 explicit retry
 
 int lvir_of(const fcfd_opts &o) {                                  // SRC/params.f90:161-173
     if (o.flag_acu == 0) return 1;
@@ -69,6 +69,7 @@ struct fcfd_ctx {
     int nblk = 0, ncell = 0, lv = 0, S = 0, kk = 0, nequ = 0;
     long P = 0, PF = 0;
     double dx = 0.0;
+    int chem_wg = 0;                     // integrator grid (0: chem_workgroups of the mechanism)
     hipStream_t stream = nullptr;
     DevThermo *thermo = nullptr;
     double *Q[3] = {nullptr, nullptr, nullptr};
@@ -93,6 +94,9 @@ struct fcfd_ctx {
     int *d_list = nullptr;               // cells the explicit screen sends to the implicit integrator
     int *d_order = nullptr;              // that list (or the surrogate's fallback list) reordered, slowest first
     double *d_hmem = nullptr;            // per interior cell: the integrator's last step-size proposal
+    int *d_list2 = nullptr;              // the stiff list after the lane-per-cell retry of a long list
+    unsigned *d_list2_n = nullptr;
+    int *d_retry = nullptr;              // 1 when the cells of d_list2 already failed that retry
     unsigned *d_list_n = nullptr;
     double *acc = nullptr;               // RK4 stage accumulator
     int graph_next[3] = {0, 0, 0};       // state buffer after the captured step from buffer i
@@ -267,6 +271,7 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
     a.dt = dt; a.dt_dev = dt_dev; a.stats = c->d_stats; a.order = c->d_order; a.hmem = c->d_hmem;
     if (want_hrr) { a.hrr = c->d_hrr; a.hrr_dot = c->d_hrr_dot; }
     a.layout = 0;
+    const int chem_wg = c->chem_wg > 0 ? c->chem_wg : chem_workgroups(c->mech);
     a.route = c->d_accepted;
     const int n = c->nblk * c->ncell;
     // explicit screen, one cell per lane: quiescent cells are final, the rest form the stiff list
@@ -278,26 +283,42 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
         return launch_chem_screen(a, c->kk, c->stream);
     });
     if (e != hipSuccess) return e;
+    // a long stiff list (the cold region failing the screen at once) first gets the explicit
+    // half-step retry lane-per-cell; what remains, or a short list unchanged, is d_list2
+    e = timed(c, 8, [&]() {
+        hipError_t e2 = launch_zero_u32(c->d_list2_n, c->stream);
+        if (e2 != hipSuccess) return e2;
+        ChemArgs r = a;
+        r.list = c->d_list;
+        r.list_n = c->d_list_n;
+        r.stiff = c->d_list2;
+        r.stiff_n = c->d_list2_n;
+        r.retry_done = c->d_retry;
+        r.coop_max = chem_coop_max(chem_wg);
+        return launch_chem_retry(r, c->kk, c->stream);
+    });
+    if (e != hipSuccess) return e;
+    a.retry_done = c->d_retry;
     if (c->sur) {
         fcfd_surrogate *s = c->sur;
         e = timed(c, 5, [&]() {
             FeatArgs fa{};
             fa.layout = 0; fa.n = n; fa.qf = c->qf; fa.P = c->P; fa.ncell = c->ncell; fa.lv = c->lv; fa.S = c->S;
             fa.dt_dev = dt_dev; fa.dt = dt;
-            hipError_t e2 = s->network(fa, c->d_list, c->d_list_n, n, c->stream);
+            hipError_t e2 = s->network(fa, c->d_list2, c->d_list2_n, n, c->stream);
             if (e2 != hipSuccess) return e2;
             ApplyArgs ap{};
             ap.layout = 0; ap.qf = c->qf; ap.P = c->P; ap.ncell = c->ncell; ap.lv = c->lv; ap.S = c->S;
             ap.route = c->d_accepted;
-            return s->accept(ap, a, c->d_list, c->d_list_n, n, c->stream);
+            return s->accept(ap, a, c->d_list2, c->d_list2_n, n, c->stream);
         });
         if (e != hipSuccess) return e;
-        e = timed(c, 3, [&]() { return s->fallback(a, chem_workgroups(c->mech), c->stream); });
+        e = timed(c, 3, [&]() { return s->fallback(a, chem_wg, c->stream); });
     } else {
-        a.list = c->d_list;
-        a.list_n = c->d_list_n;
+        a.list = c->d_list2;
+        a.list_n = c->d_list2_n;
         a.skip_explicit = 1;
-        e = timed(c, 3, [&]() { return launch_chem(a, c->kk, chem_workgroups(c->mech), c->stream); });
+        e = timed(c, 3, [&]() { return launch_chem(a, c->kk, chem_wg, c->stream); });
     }
     if (e != hipSuccess) return e;
     return timed(c, 4, [&]() {
@@ -418,6 +439,10 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
         HIP_TRY(hipMalloc(&c->d_hmem, sizeof(double) * nint));
         HIP_TRY(hipMemset(c->d_hmem, 0, sizeof(double) * nint));
         HIP_TRY(hipMalloc(&c->d_list_n, sizeof(unsigned)));
+        HIP_TRY(hipMalloc(&c->d_list2, sizeof(int) * nint));
+        HIP_TRY(hipMalloc(&c->d_list2_n, sizeof(unsigned)));
+        HIP_TRY(hipMalloc(&c->d_retry, sizeof(int)));
+        HIP_TRY(hipMemset(c->d_retry, 0, sizeof(int)));
         HIP_TRY(hipMemset(c->d_hrr, 0, sizeof(double) * nint));
         HIP_TRY(hipMemset(c->d_hrr_dot, 0, sizeof(double) * nint));
         HIP_TRY(hipMemset(c->d_stats, 0, sizeof(ChemCellStats) * nint));
@@ -435,6 +460,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
     (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_order); (void)hipFree(c->d_hmem); (void)hipFree(c->d_list_n);
+    (void)hipFree(c->d_list2); (void)hipFree(c->d_list2_n); (void)hipFree(c->d_retry);
     (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -636,6 +662,13 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s) {
     }
     drop_graphs(c);
     c->sur = s;
+    return FCFD_OK;
+}
+
+int fcfd_set_chem_workgroups(fcfd_ctx *c, int n) {
+    if (!c || n < 0) { set_error("fcfd_set_chem_workgroups: invalid argument"); return FCFD_E_ARG; }
+    c->chem_wg = n;
+    drop_graphs(c);   // captured steps hold the old grid
     return FCFD_OK;
 }
 

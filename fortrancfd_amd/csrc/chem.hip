@@ -631,7 +631,7 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
         //      estimate (weights at the step's start).  Trace radicals at the atol level in a cold
         //      mixture relax so fast that one explicit step of dt overshoots (dt times their
         //      consumption rate near 2) while a half step does not; such cells need no Jacobian ----
-        if (t < dt) {
+        if (t < dt && !(a.retry_done && *a.retry_done)) {
             const double hs = dt * 0.5;
             double yh = y;
             bool ok = true;
@@ -923,7 +923,7 @@ hipError_t launch_chem(const ChemArgs &a0, int nsp, int n_workgroups, hipStream_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     ChemArgs a = a0;
-    a.coop_max = 8 * n_workgroups;
+    a.coop_max = chem_coop_max(n_workgroups);
     if (a.order && a.list && a.list_n && a.stats && a.layout == 0) {
         hipLaunchKernelGGL(k_heavy_first, dim3(1), dim3(1024), 0, st, a.list, a.list_n, a.stats, a.order);
         if ((e = hipGetLastError()) != hipSuccess) return e;

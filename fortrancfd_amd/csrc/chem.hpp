@@ -62,6 +62,9 @@ struct ChemArgs {
     // optional (domain layout): per interior cell, the integrator's last step-size proposal, carried
     // from one call to the next (0: none; the screen and the surrogate reset it for cells they finish)
     double *hmem;
+    // optional (domain layout): set by k_chem_retry to 1 when every listed cell has already failed
+    // the explicit half-step retry (k_chem then goes straight to its implicit steps), else 0
+    int *retry_done;
 };
 
 // SDIRK integrator over all cells or over a work list (persistent grid): one cell per wave for long
@@ -70,6 +73,12 @@ hipError_t launch_chem(const ChemArgs &a, int nsp, int n_workgroups, hipStream_t
 // Lane-per-cell explicit screen over cells 0 .. ncells: accepted cells are final, the others are
 // appended to a.stiff (a.stiff_n must be zero before the launch).
 hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st);
+// Lane-per-cell explicit half-step retry of the list a.list[0 .. *a.list_n) when it is longer than
+// a.coop_max: accepted cells are final, the others go to a.stiff (a.stiff_n must be zero before
+// the launch) and *a.retry_done = 1; a shorter list is copied to a.stiff unchanged, *a.retry_done = 0.
+hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st);
+// Length up to which a stiff list runs four waves per cell in launch_chem.
+inline int chem_coop_max(int n_workgroups) { return 8 * n_workgroups; }
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
 // FP64 verification of surrogate answers for the cells cand_list[0 .. *ncand): accepted when the
 // trapezoidal-rule residual Y1 - Y0 - dt/2 (f(Y0) + f(Y1)) of the candidate (Y1, T1) = cand[i],

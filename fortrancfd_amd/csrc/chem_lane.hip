@@ -326,7 +326,160 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
     }
 }
 
+// Explicit retry of a LONG stiff list, ONE CELL PER LANE like k_chem_screen: two Heun steps of dt/2,
+// each accepted by its own embedded Euler estimate with weights at its start -- the retry k_chem
+// makes before its implicit steps, with the same formulas.  In: cells a.list[0 .. *a.list_n).
+// Accepted cells are final (state, heat release, stats {2, 0, 0, 0}, step-size memory 0); the
+// others go to a.stiff (*a.stiff_n zero before the launch) for the implicit integrator, which
+// *a.retry_done = 1 tells to skip its own retry.  A list of at most a.coop_max cells is copied to
+// a.stiff unchanged with *a.retry_done = 0: the integrator then runs it four waves per cell and
+// retries each cell itself.  Domain layout only.
+template <int NSP>
+__global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
+    constexpr int NS = slots<NSP>();
+    __shared__ LaneShared<NSP> sh;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const DevThermo *__restrict__ th = a.thermo;
+    const DevKin *__restrict__ K = a.kin;
+    const int n = int(*a.list_n);
+    if (n <= a.coop_max) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *a.retry_done = 0;
+            *a.stiff_n = unsigned(n);
+        }
+        for (int i = int(blockIdx.x) * 256 + int(threadIdx.x); i < n; i += int(gridDim.x) * 256) a.stiff[i] = a.list[i];
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.retry_done = 1;
+    const int item0 = int(blockIdx.x) * 64;
+    if (item0 >= n) return;                     // the whole workgroup: no barrier is skipped
+    const int item = item0 + lane;
+    const bool live = item < n;
+    const int cell = a.list[live ? item : item0];   // idle lanes shadow a live cell and store nothing
+    double *slot = sh.W[0];
+
+    const int b = cell / a.ncell, i = cell % a.ncell;
+    const long col = long(b) * a.S + a.lv + i;
+    const double rho_si = a.qf[col];
+    const double T0 = a.qf[2 * a.P + col];
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    double y[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const int k = w + NW * j;
+        y[j] = k < NSP ? a.qf[(3 + k) * a.P + col] : 0.0;
+    }
+    adjust_lane<NSP>(slot, w, lane, y);
+    const double rho = rho_si * 1.e-3;
+    double e0, cv0;
+    {
+        double ya[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) ya[j] = y[j];
+        adjust_lane<NSP>(slot, w, lane, ya);
+        energy_lane<NSP>(th, slot, w, lane, ya, T0, e0, cv0);
+    }
+
+    // ---- two Heun steps of dt/2; a lane keeps going only while its half steps are accepted ----
+    const double hs = dt * 0.5;
+    double Th = T0;
+    bool ok = live;
+    for (int sub = 0; sub < 2; ++sub) {
+        double fa[NS], faT, fb[NS], fbT;
+        rhs_lane<NSP, false>(th, K, sh, w, lane, rho, y, y, 0.0, Th, fa, faT);
+        rhs_lane<NSP, true>(th, K, sh, w, lane, rho, y, fa, hs, Th + hs * faT, fb, fbT);   // at y + hs fa
+        double err_w = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (w + NW * j < NSP) {
+                const double wt = 1.0 / (a.atol + a.rtol * fabs(y[j]));
+                const double t = ((hs * 0.5) * (fb[j] - fa[j])) * wt;
+                err_w = err_w + t * t;
+            }
+        }
+        if (w == 0) {
+            const double t = ((hs * 0.5) * (fbT - faT)) * (1.0 / (a.rtol * fabs(Th)));
+            err_w = err_w + t * t;
+        }
+        const double err = sqrt(sum4(slot, w, lane, err_w) / double(NSP + 1));
+        ok = ok && err <= 1.0;
+        if (ok) {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) y[j] = y[j] + (hs * 0.5) * (fa[j] + fb[j]);
+            Th = Th + (hs * 0.5) * (faT + fbT);
+        }
+    }
+    if (!ok) {
+        if (live && w == 0) {
+            const unsigned long long bal = __ballot(1);
+            const int leader = __ffsll((long long)bal) - 1;
+            const int rank = __popcll(bal & ((1ull << lane) - 1ull));
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(a.stiff_n, unsigned(__popcll(bal)));
+            base = __shfl(base, leader, 64);
+            a.stiff[base + rank] = cell;
+        }
+    }
+    // ---- accepted cells: energy-consistent temperature ----
+    adjust_lane<NSP>(slot, w, lane, y);
+    double T = Th;
+    bool done = !ok;
+    for (int it = 0; it < 20; ++it) {
+        if (!__syncthreads_or(done ? 0 : 1)) break;
+        double ek, cvk;
+        energy_lane<NSP>(th, slot, w, lane, y, T, ek, cvk);
+        if (!done) {
+            const double d = (e0 - ek) / cvk;
+            T = T + d;
+            if (fabs(d) <= 1.0e-12 * T) done = true;
+        }
+    }
+    double hrr_s = 0.0, hrr_d = 0.0;
+    if (a.hrr) {
+        // heat_release_rate and heat_release_rate_omegadot (SRC/thermo.f90:98-101, 128-132), as
+        // in k_chem_screen
+        double fn[NS], fnT;
+        rhs_lane<NSP, false>(th, K, sh, w, lane, rho, y, y, 0.0, T, fn, fnT);
+        double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int k = w + NW * j;
+            if (k < NSP) {
+                const double y_in = a.qf[(3 + k) * a.P + col];
+                s1 = s1 + rho * th->h298[k] * (y[j] - y_in) / dt;
+                s2 = s2 + th->h298[k] * fn[j] * rho;
+            }
+        }
+        hrr_s = sum4(slot, w, lane, s1);
+        hrr_d = sum4(slot, w, lane, s2);
+    }
+    if (!ok) return;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const int k = w + NW * j;
+        if (k < NSP) a.qf[(3 + k) * a.P + col] = y[j];
+    }
+    if (w == 0) {
+        a.qf[2 * a.P + col] = T;
+        if (a.hrr) {
+            a.hrr[cell] = -hrr_s * 1.e-1;
+            a.hrr_dot[cell] = -hrr_d * 1.e-1;
+        }
+        if (a.stats) a.stats[cell] = ChemCellStats{2, 0, 0, 0};
+        if (a.hmem) a.hmem[cell] = 0.0;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st) {
+    const unsigned nb = unsigned((a.ncells + 63) / 64);
+    if (nb == 0) return hipSuccess;
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_retry<N>), dim3(nb), dim3(256), 0, st, a); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
 
 hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st) {
     const unsigned nb = unsigned((a.ncells + 63) / 64);

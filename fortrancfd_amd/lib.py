@@ -238,6 +238,11 @@ class Solver:
         _check(lib().fcfd_set_surrogate(self.handle, sur.handle if sur is not None else None))
         self._sur = sur
 
+    def set_chem_workgroups(self, n):
+        """Grid of the implicit integrator (0: default): n one-wave workgroups for lists longer
+        than 8 n cells, else n / 4 four-wave workgroups."""
+        _check(lib().fcfd_set_chem_workgroups(self.handle, C.c_int(n)))
+
     def source_accepted(self):
         """(per-cell route of the last solve_source: 2 explicit screen, 1 surrogate, 0 implicit
         FP64 integrator; number of implicit-integrator cells)."""

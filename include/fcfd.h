@@ -114,7 +114,7 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time);
  * one kernel at a time instead of from the captured graph).  Kernel ids: 0 advection flux,
  * 1 stage update + primitive recovery, 2 CFL reduction, 3 chemistry implicit integrator,
  * 4 interior qfToQc after the source, 5 surrogate (features, network, acceptance), 6 chemistry
- * explicit screen, 7 transport flux. */
+ * explicit screen, 7 transport flux, 8 chemistry explicit half-step retry of a long stiff list. */
 int fcfd_set_timing(fcfd_ctx *c, int enable);
 int fcfd_get_timing(fcfd_ctx *c, int kernel_id, double *ms_total, long long *launches);
 
@@ -209,6 +209,12 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s);
 /* Per interior cell of the last solve_source: 2 explicit screen, 1 surrogate, 0 FP64 implicit
  * integrator; *n_fallback = number of cells with 0. */
 int fcfd_source_accepted(fcfd_ctx *c, int *per_cell, int *n_fallback);
+/* Grid of the FP64 implicit integrator in this context's solve_source: n one-wave workgroups for
+ * lists of stiff cells longer than 8 n, else n / 4 four-wave workgroups (0: the default, four per
+ * compute unit).  No counterpart in the reference: a tuning knob (a smaller grid leaves compute
+ * units to concurrent work).  A cell's answer can differ in the last bits with the workgroup kind
+ * that integrates it. */
+int fcfd_set_chem_workgroups(fcfd_ctx *c, int n);
 
 /* Plain BF16 GEMM C (FP32) = A B on device arrays with element (m, k) of A at A[m*sam + k*sak]
  * and (k, n) of B at B[k*sbk + n*sbn]; M, N multiples of 64, K / ksplit a multiple of 16;

@@ -74,13 +74,9 @@ def test_macks_invariants(mech, orc):
         assert abs(e1 - e0) <= 1e-10 * abs(e0) + 1e-6, c
 
 
-def test_solve_source_vs_oracle(mech, orc):
-    """reaction_macks on a 2-block domain with a hot spot, three calls in a row (the second and third
-    start from the integrator's per-cell step-size memory on both sides): qf, Qc, HRR, HRR_dot vs
-    the oracle."""
-    from oracle.pyoracle import Domain
-    from fortrancfd_amd.lib import Solver
-    kk = mech.kk
+def hot_spot_cells(kk):
+    """2 blocks x 32 cells of stoichiometric CH4/air at 1 atm, 300 K except 12 cells from 2200 K
+    down to 1500 K (each of them needs 7 to 65 integrator attempts over dt = 2e-7 s)."""
     names = list(golden("mech")["names"])
     nblk, ncell, dx = 2, 32, 25e-6
     cells = np.zeros((nblk * ncell, 3 + kk))
@@ -90,6 +86,16 @@ def test_solve_source_vs_oracle(mech, orc):
     x = np.zeros(kk)
     x[names.index("CH4")], x[names.index("O2")], x[names.index("N2")] = 0.0950570342, 0.1901140684, 0.7148288973
     cells[:, 3:] = x
+    return cells, nblk, ncell, dx
+
+
+def test_solve_source_vs_oracle(mech, orc):
+    """reaction_macks on a 2-block domain with a hot spot, three calls in a row (the second and third
+    start from the integrator's per-cell step-size memory on both sides): qf, Qc, HRR, HRR_dot vs
+    the oracle."""
+    from oracle.pyoracle import Domain
+    from fortrancfd_amd.lib import Solver
+    cells, nblk, ncell, dx = hot_spot_cells(mech.kk)
     d = Domain(orc, cells, nblk, ncell, dx)
     s = Solver(mech, nblk, ncell, dx, do_reaction=1)
     s.init_cells(cells)
@@ -99,6 +105,65 @@ def test_solve_source_vs_oracle(mech, orc):
         hrr_o, hrd_o, nf = d.solve_source(dt)
         hrr_g, hrd_g = s.solve_source(dt)
         assert nf == 0
+    compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o)
+
+
+def test_solve_source_long_list_vs_oracle(mech, orc):
+    """A stiff list longer than the four-wave limit, which a one-workgroup integrator grid lowers
+    to 8 cells, so the 12 stiff cells go to the one-wave kernel (the mode of long lists, which the
+    other domain tests never reach).  Three calls vs the oracle."""
+    from oracle.pyoracle import Domain
+    from fortrancfd_amd.lib import Solver
+    cells, nblk, ncell, dx = hot_spot_cells(mech.kk)
+    d = Domain(orc, cells, nblk, ncell, dx)
+    s = Solver(mech, nblk, ncell, dx, do_reaction=1)
+    s.set_chem_workgroups(1)
+    s.init_cells(cells)
+    dt = 2e-7
+    for call in range(3):
+        print("solve_source (one-workgroup grid) call", call, flush=True)
+        hrr_o, hrd_o, nf = d.solve_source(dt)
+        hrr_g, hrd_g = s.solve_source(dt)
+        assert nf == 0
+        acc, _ = s.source_accepted()
+        assert (acc == 0).sum() > 8     # a long list
+    compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o)
+
+
+def test_solve_source_lane_retry_vs_oracle(mech, orc):
+    """Cold cells with trace radicals (1e-12 mole fraction of CO, O, C, CH3, OH, H, HO2) fail one
+    explicit step of dt = 1e-8 s (error norm 2.5) but pass two half steps (0.6, then 0.3), as the
+    cold region of a flame run does.  With a one-workgroup integrator grid (four-wave limit 8 cells)
+    the 64-cell stiff list takes the lane-per-cell retry, which finishes 53 cells and leaves 11 to
+    the implicit integrator (without a second retry); with the default grid the same list goes to
+    the integrator, which retries each cell itself.  Both vs the oracle over three calls."""
+    from oracle.pyoracle import Domain
+    from fortrancfd_amd.lib import Solver
+    names = list(golden("mech")["names"])
+    for grid in (1, 0):
+        cells, nblk, ncell, dx = hot_spot_cells(mech.kk)
+        for sp in ("CO", "O", "C", "CH3", "OH", "H", "HO2"):
+            cells[12:, 3 + names.index(sp)] = 1e-12
+        cells[:, 3:] /= cells[:, 3:].sum(axis=1, keepdims=True)
+        d = Domain(orc, cells, nblk, ncell, dx)
+        s = Solver(mech, nblk, ncell, dx, do_reaction=1)
+        s.set_chem_workgroups(grid)
+        s.init_cells(cells)
+        dt = 1e-8
+        for call in range(3):
+            print("solve_source grid", grid, "call", call, flush=True)
+            hrr_o, hrd_o, nf = d.solve_source(dt)
+            hrr_g, hrd_g = s.solve_source(dt)
+            assert nf == 0
+            if call == 0:
+                acc, _ = s.source_accepted()
+                st = s.chem_stats()
+                assert (acc == 0).sum() == 64                           # every cell fails the screen
+                assert ((st[:, 0] == 2) & (st[:, 2] == 0)).sum() == 53   # finished by the retry
+        compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o)
+
+
+def compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o):
     Qc, qf = s.download()
     lv = s.lv
     np.testing.assert_allclose(qf[:, lv:-lv, 2], d.qf[:, lv:-lv, 2], rtol=1e-9)
