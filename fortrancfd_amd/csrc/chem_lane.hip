@@ -5,7 +5,8 @@
 // energy-consistent temperature).  Quiescent cells -- the cold bulk of a reacting domain -- end
 // here; the others are appended to a work list for the wave-per-cell SDIRK kernel (chem.hip),
 // which then starts from the unchanged state.  The arithmetic is the pre-step of that kernel; only
-// the order of the species sums differs.
+// the order of the species sums differs.  k_chem_retry runs the integrator's next explicit attempt
+// (two half steps) the same way over a long work list, before the SDIRK kernel sees it.
 //
 // Mapping: a workgroup of four waves owns 64 cells (lane l of every wave is cell l).  Concentrations
 // and S/R - H/RT of the 64 cells live in LDS as species-major [k][64] blocks; each wave evaluates a
