@@ -97,6 +97,7 @@ struct fcfd_ctx {
     int *d_list2 = nullptr;              // the stiff list after the lane-per-cell retry of a long list
     unsigned *d_list2_n = nullptr;
     int *d_retry = nullptr;              // 1 when the cells of d_list2 already failed that retry
+    double *d_f0 = nullptr;              // [kk + 1][cells]: f at the start state of the screen's rejects
     unsigned *d_list_n = nullptr;
     double *acc = nullptr;               // RK4 stage accumulator
     int graph_next[3] = {0, 0, 0};       // state buffer after the captured step from buffer i
@@ -273,6 +274,7 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
     a.layout = 0;
     const int chem_wg = c->chem_wg > 0 ? c->chem_wg : chem_workgroups(c->mech);
     a.route = c->d_accepted;
+    a.f0 = c->d_f0;
     const int n = c->nblk * c->ncell;
     // explicit screen, one cell per lane: quiescent cells are final, the rest form the stiff list
     hipError_t e = timed(c, 6, [&]() {
@@ -442,6 +444,7 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
         HIP_TRY(hipMalloc(&c->d_list2, sizeof(int) * nint));
         HIP_TRY(hipMalloc(&c->d_list2_n, sizeof(unsigned)));
         HIP_TRY(hipMalloc(&c->d_retry, sizeof(int)));
+        HIP_TRY(hipMalloc(&c->d_f0, sizeof(double) * size_t(c->kk + 1) * nint));
         HIP_TRY(hipMemset(c->d_retry, 0, sizeof(int)));
         HIP_TRY(hipMemset(c->d_hrr, 0, sizeof(double) * nint));
         HIP_TRY(hipMemset(c->d_hrr_dot, 0, sizeof(double) * nint));
@@ -460,7 +463,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->d_dt); (void)hipFree(c->d_time); (void)hipFree(c->d_cvel); (void)hipFree(c->thermo);
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
     (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_order); (void)hipFree(c->d_hmem); (void)hipFree(c->d_list_n);
-    (void)hipFree(c->d_list2); (void)hipFree(c->d_list2_n); (void)hipFree(c->d_retry);
+    (void)hipFree(c->d_list2); (void)hipFree(c->d_list2_n); (void)hipFree(c->d_retry); (void)hipFree(c->d_f0);
     (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

@@ -65,6 +65,9 @@ struct ChemArgs {
     // optional (domain layout): set by k_chem_retry to 1 when every listed cell has already failed
     // the explicit half-step retry (k_chem then goes straight to its implicit steps), else 0
     int *retry_done;
+    // optional (domain layout): f at the start state of each cell the explicit screen rejects,
+    // [NSP + 1][ncells] (species, then T); k_chem_retry starts from it instead of evaluating it again
+    double *f0;
 };
 
 // SDIRK integrator over all cells or over a work list (persistent grid): one cell per wave for long

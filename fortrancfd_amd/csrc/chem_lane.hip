@@ -250,6 +250,14 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
     const double err = sqrt(sum4(slot, w, lane, err_w) / double(NSP + 1));
     const bool ok = live && err <= 1.0;
 
+    if (!ok && live && a.f0) {   // f at the start state, for the retry of a long list
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int k = w + NW * j;
+            if (k < NSP) a.f0[long(k) * a.ncells + cell] = f0[j];
+        }
+        if (w == 0) a.f0[long(NSP) * a.ncells + cell] = f0T;
+    }
     if (!ok) {
         if (live && w == 0) {
             const unsigned long long bal = __ballot(1);
@@ -329,7 +337,8 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
 
 // Explicit retry of a LONG stiff list, ONE CELL PER LANE like k_chem_screen: two Heun steps of dt/2,
 // each accepted by its own embedded Euler estimate with weights at its start -- the retry k_chem
-// makes before its implicit steps, with the same formulas.  In: cells a.list[0 .. *a.list_n).
+// makes before its implicit steps, with the same formulas.  In: cells a.list[0 .. *a.list_n), and
+// (when a.f0 is given) f at their start states, which the screen computed at the same states.
 // Accepted cells are final (state, heat release, stats {2, 0, 0, 0}, step-size memory 0); the
 // others go to a.stiff (*a.stiff_n zero before the launch) for the implicit integrator, which
 // *a.retry_done = 1 tells to skip its own retry.  A list of at most a.coop_max cells is copied to
@@ -387,7 +396,16 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
     bool ok = live;
     for (int sub = 0; sub < 2; ++sub) {
         double fa[NS], faT, fb[NS], fbT;
-        rhs_lane<NSP, false>(th, K, sh, w, lane, rho, y, y, 0.0, Th, fa, faT);
+        if (sub == 0 && a.f0) {   // f at the start state, stored by the screen
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                const int k = w + NW * j;
+                fa[j] = k < NSP ? a.f0[long(k) * a.ncells + cell] : 0.0;
+            }
+            faT = a.f0[long(NSP) * a.ncells + cell];
+        } else {
+            rhs_lane<NSP, false>(th, K, sh, w, lane, rho, y, y, 0.0, Th, fa, faT);
+        }
         rhs_lane<NSP, true>(th, K, sh, w, lane, rho, y, fa, hs, Th + hs * faT, fb, fbT);   // at y + hs fa
         double err_w = 0.0;
 #pragma unroll
