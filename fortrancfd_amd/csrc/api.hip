@@ -669,7 +669,7 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s) {
 }
 
 int fcfd_set_chem_workgroups(fcfd_ctx *c, int n) {
-    if (!c || n < 0) { set_error("fcfd_set_chem_workgroups: invalid argument"); return FCFD_E_ARG; }
+    if (!c || n < 0 || n > (1 << 24)) { set_error("fcfd_set_chem_workgroups: n must be in [0, 2^24]"); return FCFD_E_ARG; }
     c->chem_wg = n;
     drop_graphs(c);   // captured steps hold the old grid
     return FCFD_OK;

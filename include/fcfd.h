@@ -211,7 +211,7 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s);
 int fcfd_source_accepted(fcfd_ctx *c, int *per_cell, int *n_fallback);
 /* Grid of the FP64 implicit integrator in this context's solve_source: n one-wave workgroups for
  * lists of stiff cells longer than 8 n, else n / 4 four-wave workgroups (0: the default, four per
- * compute unit).  No counterpart in the reference: a tuning knob (a smaller grid leaves compute
+ * compute unit; at most 2^24).  No counterpart in the reference: a tuning knob (a smaller grid leaves compute
  * units to concurrent work).  A cell's answer can differ in the last bits with the workgroup kind
  * that integrates it. */
 int fcfd_set_chem_workgroups(fcfd_ctx *c, int n);
