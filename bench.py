@@ -263,6 +263,7 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
     _, qf_fp = s_fp.download()
     route_fp, _ = s_fp.source_accepted()
     st_fp = s_fp.chem_stats()[route_fp == 0]
+    st_fp = st_fp[st_fp[:, 2] > 0]          # cells with Newton work (implicit steps)
     fp64_stiff = {"cells": int(st_fp.shape[0])}
     if st_fp.shape[0]:
         fp64_stiff.update({"steps_mean": float(st_fp[:, 0].mean()), "steps_max": int(st_fp[:, 0].max()),
@@ -314,6 +315,7 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
         "speedup_vs_fp64": value / fp64_value,
         "error_vs_fp64": dict(err_sur, major_species=list(MAJOR), after_steps=args.warmup + args.steps),
         "chemistry_routes_last_step": {"explicit_screen": int((route == 2).sum()),
+                                       "explicit_half_steps": int((route == 3).sum()),
                                        "surrogate": int((route == 1).sum()),
                                        "fp64_implicit": int((route == 0).sum())},
         "unverified_surrogate": {
@@ -323,6 +325,7 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
             "speedup_vs_fp64": (ntot * args.steps / el_u) / fp64_value,
             "error_vs_fp64": err_u,
             "chemistry_routes_last_step": {"explicit_screen": int((route_u == 2).sum()),
+                                           "explicit_half_steps": int((route_u == 3).sum()),
                                            "surrogate": int((route_u == 1).sum()),
                                            "fp64_implicit": int((route_u == 0).sum())}},
         "sim_time_s": t_sur,
@@ -424,6 +427,7 @@ def bench_chem(args, mech, surrogate):
             q = lambda v: {p: float(np.quantile(v, p)) for p in (0.5, 0.9, 0.99, 1.0)}
             return el, {"value": n / el, "speedup_vs_fp64": el_fp / el,
                         "routes": {"explicit_screen": int((r == 2).sum()),
+                                   "explicit_half_steps": int((r == 3).sum()),
                                    "surrogate": int((r == 1).sum()), "fp64_implicit": int(nf)},
                         "T_relerr_quantiles": q(te), "major_abserr_quantiles": q(ye),
                         "frac_T_within_1e-3": float((te < 1e-3).mean()),

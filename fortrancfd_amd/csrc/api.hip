@@ -109,6 +109,12 @@ struct fcfd_ctx {
 
 namespace fcfd {
 
+// Device memory released when it goes out of scope (host-array entry points, error paths).
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
 // Device copies of a mechanism's tables on the current device (created once).
 int ensure_dev_tables(const fcfd_mech *cm) {
     fcfd_mech *m = const_cast<fcfd_mech *>(cm);
@@ -116,14 +122,22 @@ int ensure_dev_tables(const fcfd_mech *cm) {
     HIP_TRY(hipGetDevice(&dev));
     if (m->d_thermo && m->device == dev) return FCFD_OK;
     if (m->d_thermo) { set_error("mechanism tables already live on another device"); return FCFD_E_ARG; }
-    HIP_TRY(hipDeviceGetAttribute(&m->ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_TRY(hipMalloc(&m->d_thermo, sizeof(DevThermo)));
-    HIP_TRY(hipMemcpy(m->d_thermo, &m->host.thermo, sizeof(DevThermo), hipMemcpyHostToDevice));
+    // the tables go into the mechanism only when every step succeeded; a failure frees what was made
+    int ncu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    DevBuf th, kin, counter;
+    HIP_TRY(hipMalloc(&th.p, sizeof(DevThermo)));
+    HIP_TRY(hipMemcpy(th.p, &m->host.thermo, sizeof(DevThermo), hipMemcpyHostToDevice));
     auto k = std::make_unique<DevKin>();
     build_dev_kin(m->host, *k);
-    HIP_TRY(hipMalloc(&m->d_kin, sizeof(DevKin)));
-    HIP_TRY(hipMemcpy(m->d_kin, k.get(), sizeof(DevKin), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&m->d_counter, sizeof(unsigned int)));
+    HIP_TRY(hipMalloc(&kin.p, sizeof(DevKin)));
+    HIP_TRY(hipMemcpy(kin.p, k.get(), sizeof(DevKin), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&counter.p, sizeof(unsigned int)));
+    m->ncu = ncu;
+    m->d_thermo = static_cast<DevThermo *>(th.p);
+    m->d_kin = kin.p;
+    m->d_counter = static_cast<unsigned int *>(counter.p);
+    th.p = kin.p = counter.p = nullptr;
     m->device = dev;
     return FCFD_OK;
 }
@@ -391,7 +405,8 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
     *out = nullptr;
     int rc = check_opts(*o, m->host.kk);
     if (rc) return rc;
-    auto c = std::make_unique<fcfd_ctx>();
+    // every error return below releases what was allocated so far through fcfd_destroy
+    std::unique_ptr<fcfd_ctx, void (*)(fcfd_ctx *)> c(new fcfd_ctx(), fcfd_destroy);
     c->mech = m; c->o = *o; c->nblk = nblk; c->ncell = ncell; c->dx = dx;
     c->lv = lvir_of(*o);
     if (ncell < c->lv) { set_error("ncell must be at least lvir"); return FCFD_E_ARG; }
@@ -709,11 +724,6 @@ int fcfd_get_timing(fcfd_ctx *c, int id, double *ms_total, long long *launches) 
 // ---- stateless host-array entry points ----------------------------------------------------
 namespace {
 
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-};
-
 // rows x cols (row-major, AoS) -> cols x rows (SoA)
 std::vector<double> transpose(const double *in, long rows, int cols) {
     std::vector<double> out(size_t(rows) * cols);
@@ -848,6 +858,39 @@ int fcfd_chem_step_host(const fcfd_mech *m, const fcfd_opts *o, int n, const dou
         yT[size_t(c) * (kk + 1) + kk] = hT[c];
         for (int k = 0; k < kk; ++k) yT[size_t(c) * (kk + 1) + k] = hY[size_t(k) * n + c];
     }
+    return FCFD_OK;
+}
+
+int fcfd_rates_host(const fcfd_mech *m, int n, const double *rho, const double *T, const double *Y,
+                    double *wdot, double *cdot, double *ddot, double *qf, double *qr) {
+    if (!m || n < 1 || !rho || !T || !Y || !wdot || !cdot || !ddot || !qf || !qr) {
+        set_error("fcfd_rates_host: invalid argument");
+        return FCFD_E_ARG;
+    }
+    if (int rc = ensure_dev_tables(m)) return rc;
+    const int kk = m->host.kk, ii = m->host.ii;
+    const size_t nk = size_t(n) * kk, ni = size_t(n) * ii;
+    DevBuf dr, dT, dY, dw, dc, dd, dqf, dqr;
+    HIP_TRY(hipMalloc(&dr.p, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&dT.p, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&dY.p, sizeof(double) * nk));
+    HIP_TRY(hipMalloc(&dw.p, sizeof(double) * nk));
+    HIP_TRY(hipMalloc(&dc.p, sizeof(double) * nk));
+    HIP_TRY(hipMalloc(&dd.p, sizeof(double) * nk));
+    HIP_TRY(hipMalloc(&dqf.p, sizeof(double) * ni));
+    HIP_TRY(hipMalloc(&dqr.p, sizeof(double) * ni));
+    HIP_TRY(hipMemcpy(dr.p, rho, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dT.p, T, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dY.p, Y, sizeof(double) * nk, hipMemcpyHostToDevice));
+    auto d = [](DevBuf &b) { return static_cast<double *>(b.p); };
+    HIP_TRY(launch_rates(m->d_thermo, static_cast<const DevKin *>(m->d_kin), kk, n, d(dr), d(dT), d(dY), d(dw),
+                         d(dc), d(dd), d(dqf), d(dqr), nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(wdot, dw.p, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cdot, dc.p, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ddot, dd.p, sizeof(double) * nk, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(qf, dqf.p, sizeof(double) * ni, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(qr, dqr.p, sizeof(double) * ni, hipMemcpyDeviceToHost));
     return FCFD_OK;
 }
 

@@ -254,7 +254,7 @@ template <int NSP, int NW>
 __device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, Shared<NSP, NW> &sh,
                       double rho, double y_l, double T, bool split, int wave) {
     const RhsPre pre = rhs_begin<NSP, NW>(th, sh, rho, y_l, T, wave);
-    const double alogt = log(T), pfac = th->pa / (th->ru * T);
+    const double alogt = fm_log(T), pfac = th->pa / (th->ru * T);
     for (int b = first_block<NW>(wave); b * 64 < K->ii; b += block_step<NW>(wave)) {
         const int i = b * 64 + lane_id();
         if (i >= K->ii) continue;
@@ -310,7 +310,7 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
     double *Jw = sh.J[wave];
     double *J = sh.J[0];
     const double ctot = pre.ctot;
-    const double alogt = log(T), pfac = th->pa / (th->ru * T);
+    const double alogt = fm_log(T), pfac = th->pa / (th->ru * T);
     // two of the wave's blocks per pass: their rate chains are independent and overlap; the
     // scatters follow in block order
     constexpr int RB = 2;
@@ -711,7 +711,7 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
                 double ev = lane < N ? h * (CH_E1 * k0 + CH_E2 * k1 + CH_E3 * k2) : 0.0;
                 ev = solve<NSP, NW>(sh, ev);
                 const double err = wnorm(lane < N ? ev : 0.0, w, N);
-                const double fac = 0.9 * pow(fmax(err, 1.0e-10), -1.0 / 3.0);
+                const double fac = 0.9 * fm_pow(fmax(err, 1.0e-10), -1.0 / 3.0);
                 if (err <= 1.0) {
                     if (lane < N) y = z;
                     t = t + h;
@@ -760,6 +760,8 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
             if (wave == 0 && lane == 0) a.b_T[cell] = T;
         }
         if (a.stats && wave == 0 && lane == 0) a.stats[cell] = ChemCellStats{steps, rejects, newton, failed};
+        // route of a cell the explicit attempts finished: 2 the step over dt, 3 the two half steps
+        if (a.route && expl && wave == 0 && lane == 0) a.route[cell] = steps == 2 ? 3 : 2;
         if (a.hmem && wave == 0 && lane == 0) a.hmem[cell] = expl ? 0.0 : h;
     }
 }
@@ -834,6 +836,48 @@ __global__ void __launch_bounds__(64) k_chem_verify(ChemArgs a, VerifyArgs v) {
     }
 }
 
+// CKWYR and CKCDYR (cklib.f:9107-9204, 1422-1541) of n states through the integrator's own rate
+// code: rhs_begin (concentrations, S/R - H/RT, ctot), reaction / reaction_plain per row and
+// rhs_scatter into the production-rate sums, exactly as an integrator step evaluates them; the
+// rates of progress are also stored per reaction in the mechanism's order.  One state per wave.
+template <int NSP>
+__global__ void __launch_bounds__(64) k_rates(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, int n,
+                                              const double *__restrict__ rho, const double *__restrict__ Tin,
+                                              const double *__restrict__ Y, double *wdot, double *cdot,
+                                              double *ddot, double *qf, double *qr) {
+    __shared__ Shared<NSP, 1> sh;
+    const int lane = lane_id();
+    for (int c = blockIdx.x; c < n; c += gridDim.x) {
+        const double r = rho[c], T = Tin[c];
+        const double y_l = lane < NSP ? Y[long(c) * NSP + lane] : 0.0;
+        const double alogt = fm_log(T), pfac = th->pa / (th->ru * T);
+        for (int split = 0; split < 2; ++split) {
+            const RhsPre pre = rhs_begin<NSP, 1>(th, sh, r, y_l, T, 0);
+            for (int b = 0; b * 64 < K->ii; ++b) {
+                const int i = b * 64 + lane;
+                if (i >= K->ii) continue;
+                const Rop q = b * 64 >= K->nplain0 ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
+                                                   : reaction<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
+                rhs_scatter<NSP, 1>(K, sh, i, q, split == 1, 0);
+                if (split == 0) {
+                    qf[long(c) * K->ii + K->orig[i]] = q.qf;
+                    qr[long(c) * K->ii + K->orig[i]] = q.qr;
+                }
+            }
+            __syncthreads();
+            if (lane < NSP) {
+                if (split == 0) {
+                    wdot[long(c) * NSP + lane] = sh.W[0][lane];
+                } else {
+                    cdot[long(c) * NSP + lane] = sh.Cd[0][lane];
+                    ddot[long(c) * NSP + lane] = sh.Dd[0][lane];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 __global__ void k_zero_counter(unsigned int *c) {
     if (threadIdx.x == 0) *c = 0u;
 }
@@ -884,6 +928,18 @@ __global__ void __launch_bounds__(1024) k_heavy_first(const int *list, const uns
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, p);
     return hipGetLastError();
+}
+
+hipError_t launch_rates(const DevThermo *th, const DevKin *K, int nsp, int n, const double *rho, const double *T,
+                        const double *Y, double *wdot, double *cdot, double *ddot, double *qf, double *qr,
+                        hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const unsigned nb = unsigned(std::min(n, 4096));
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_rates<N>), dim3(nb), dim3(64), 0, st, th, K, n, rho, T, Y, \
+                                                        wdot, cdot, ddot, qf, qr); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_chem_verify(const ChemArgs &a, int nsp, const int *cand_list, const unsigned *ncand,

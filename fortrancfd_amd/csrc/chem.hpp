@@ -20,6 +20,7 @@ struct DevKin {
     double tba[MAXR][MAXTB];            // their efficiencies
     int ifop[MAXR], kfal[MAXR];         // fall-off form (1 Lindemann, 2 SRI, 3/4 Troe), [M] species
     double fpar[MAXR][8];
+    int orig[MAXR];                     // the mechanism's (0-based) index of each device row
 };
 
 void build_dev_kin(const Mech &m, DevKin &k);
@@ -53,7 +54,8 @@ struct ChemArgs {
     // explicit screen output: cells that need the implicit integrator (list + length)
     int *stiff;
     unsigned *stiff_n;
-    int *route;                  // optional per-cell route: 2 accepted by the explicit screen, 0 stiff
+    int *route;                  // optional per-cell route: 2 explicit step over dt, 3 two explicit half
+                                 // steps, 1 surrogate, 0 implicit (SDIRK) integrator
     int layout;                  // 0: domain, 1: batch
     int coop_max;                // set by launch_chem: lists up to this length run four waves per cell
     // optional (domain layout with stats): buffer of the list's capacity that receives the list
@@ -83,6 +85,12 @@ hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st);
 // Length up to which a stiff list runs four waves per cell in launch_chem.
 inline int chem_coop_max(int n_workgroups) { return 8 * n_workgroups; }
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
+// CKWYR / CKCDYR (cklib.f:9107-9204, 1422-1541) on n states with the integrator's own rate code:
+// rho [g/cm^3], T [K], Y [n][KK] in; wdot, cdot, ddot [n][KK] (mol/(cm^3 s)) and the rates of
+// progress qf, qr [n][II] (mechanism order) out.  One state per wave.
+hipError_t launch_rates(const DevThermo *th, const DevKin *K, int nsp, int n, const double *rho, const double *T,
+                        const double *Y, double *wdot, double *cdot, double *ddot, double *qf, double *qr,
+                        hipStream_t st);
 // FP64 verification of surrogate answers for the cells cand_list[0 .. *ncand): accepted when the
 // trapezoidal-rule residual Y1 - Y0 - dt/2 (f(Y0) + f(Y1)) of the candidate (Y1, T1) = cand[i],
 // weighted by the integrator's tolerances, has an RMS <= tol (1: the integrator's own bar); accepted

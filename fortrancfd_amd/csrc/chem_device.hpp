@@ -32,7 +32,7 @@ __device__ __forceinline__ double ipow(double x, int n) {
 // CKSMH (cklib.f:7540-7560): S/R - H/RT of one species.
 __device__ __forceinline__ double smh_species(const Nasa &a, double T) {
     const double t2 = T * T, t3 = T * t2, t4 = t2 * t2;
-    double s = (log(T) - 1.0) * a.a0;
+    double s = (fm_log(T) - 1.0) * a.a0;
     s = s + T / 2.0 * a.a1;
     s = s + t2 / 6.0 * a.a2;
     s = s + t3 / 12.0 * a.a3;
@@ -51,13 +51,13 @@ struct Rop {
 template <int STRIDE>
 __device__ __forceinline__ Rop reaction(const DevKin *__restrict__ K, int i, double T, double alogt, double pfac,
                                         double ctot, const double *__restrict__ c, const double *__restrict__ smh) {
-    double kf = K->A[i] * exp(K->B[i] * alogt - K->E[i] / T);
+    double kf = K->A[i] * fm_exp(K->B[i] * alogt - K->E[i] / T);
     double s = 0.0;
     for (int n = 0; n < 6; ++n) {
         const int k = K->nk[i][n];
         if (k >= 0) s = s + (double)K->nu[i][n] * smh[k * STRIDE];
     }
-    double eqk = s != 0.0 ? exp(fmin(s, K->exparg)) : 1.0;
+    double eqk = s != 0.0 ? fm_exp(fmin(s, K->exparg)) : 1.0;
     eqk = eqk * ipow(pfac, K->nus[i]);
     double kr = K->rev[i] ? kf / fmax(eqk, K->small) : 0.0;
     double ctb = 1.0;
@@ -85,26 +85,26 @@ __device__ __forceinline__ Rop reaction(const DevKin *__restrict__ K, int i, dou
     const int f = K->fo[i];
     if (f >= 0) {
         const double *fp = K->fpar[f];
-        const double rklow = fp[0] * exp(fp[1] * alogt - fp[2] / T);
+        const double rklow = fp[0] * fm_exp(fp[1] * alogt - fp[2] / T);
         double pr;
         if (K->kfal[f] < 0) { pr = rklow * ctb / kf; ctb = 1.0; }
         else pr = rklow * c[K->kfal[f] * STRIDE] / kf;
         double pcor = pr / (1.0 + pr);
         if (K->ifop[f] > 1) {
-            const double prlog = log10(fmax(pr, K->small));
+            const double prlog = fm_log10(fmax(pr, K->small));
             double fc;
             if (K->ifop[f] == 2) {
                 const double xp = 1.0 / (1.0 + prlog * prlog);
-                fc = pow(fp[3] * exp(-fp[4] / T) + exp(-T / fp[5]), xp) * fp[6] * pow(T, fp[7]);
+                fc = fm_pow(fp[3] * fm_exp(-fp[4] / T) + fm_exp(-T / fp[5]), xp) * fp[6] * fm_pow(T, fp[7]);
             } else {
-                double fcent = (1.0 - fp[3]) * exp(-T / fp[4]) + fp[3] * exp(-T / fp[5]);
-                if (K->ifop[f] == 4) fcent = fcent + exp(-fp[6] / T);
-                const double fclog = log10(fmax(fcent, K->small));
+                double fcent = (1.0 - fp[3]) * fm_exp(-T / fp[4]) + fp[3] * fm_exp(-T / fp[5]);
+                if (K->ifop[f] == 4) fcent = fcent + fm_exp(-fp[6] / T);
+                const double fclog = fm_log10(fmax(fcent, K->small));
                 // single-precision literals of cklib.f:6915-6917
                 const double xn = 0.75 - (double)1.27f * fclog;
                 const double cprlog = prlog - ((double)0.4f + (double)0.67f * fclog);
                 const double q = cprlog / (xn - (double)0.14f * cprlog);
-                fc = pow(10.0, fclog / (1.0 + q * q));
+                fc = fm_pow(10.0, fclog / (1.0 + q * q));
             }
             pcor = fc * pcor;
         }
@@ -145,7 +145,7 @@ __device__ __forceinline__ double ipow3(double x, int m) {
 template <int STRIDE>
 __device__ __forceinline__ Rop reaction_plain(const DevKin *__restrict__ K, int i, double T, double alogt, double pfac,
                                               const double *__restrict__ c, const double *__restrict__ smh) {
-    const double kf = K->A[i] * exp(K->B[i] * alogt - K->E[i] / T);
+    const double kf = K->A[i] * fm_exp(K->B[i] * alogt - K->E[i] / T);
     const int *nu = K->nu[i], *nk = K->nk[i];
     int k[6], a[6];
     double s = 0.0;
@@ -156,7 +156,7 @@ __device__ __forceinline__ Rop reaction_plain(const DevKin *__restrict__ K, int 
         const double t = (double)a[n] * smh[(k[n] >= 0 ? k[n] : 0) * STRIDE];
         s = k[n] >= 0 ? s + t : s;
     }
-    const double ex = exp(fmin(s, K->exparg));
+    const double ex = fm_exp(fmin(s, K->exparg));
     double eqk = s != 0.0 ? ex : 1.0;
     const int ns = K->nus[i];
     const double pp = ipow3(pfac, ns < 0 ? -ns : ns);

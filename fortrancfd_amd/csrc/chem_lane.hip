@@ -59,7 +59,7 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
     const TPow p = tpow(T, th->ru);
     double uk[NS];
     double ctot_w = 0.0, cvm_w = 0.0;
-    const double lnT = log(T);
+    const double lnT = fm_log(T);
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
         const int k = w + NW * j;
@@ -427,6 +427,10 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
             for (int j = 0; j < NS; ++j) y[j] = y[j] + (hs * 0.5) * (fa[j] + fb[j]);
             Th = Th + (hs * 0.5) * (faT + fbT);
         }
+        // no lane of the workgroup passed the first half step: the second would be wasted work.
+        // ok is the same in the four waves of a lane (err comes from sum4), and every wave reaches
+        // this barrier, so the exit is uniform
+        if (sub == 0 && !__syncthreads_or(ok ? 1 : 0)) break;
     }
     if (!ok) {
         if (live && w == 0) {
@@ -485,6 +489,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
             a.hrr_dot[cell] = -hrr_d * 1.e-1;
         }
         if (a.stats) a.stats[cell] = ChemCellStats{2, 0, 0, 0};
+        if (a.route) a.route[cell] = 3;
         if (a.hmem) a.hmem[cell] = 0.0;
     }
 }

@@ -8,6 +8,7 @@
 #pragma once
 
 #include "fcfd_common.hpp"
+#include "fcfd_math.h"
 
 namespace fcfd {
 

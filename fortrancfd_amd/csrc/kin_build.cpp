@@ -43,6 +43,7 @@ void build_dev_kin(const Mech &m, DevKin &k) {
     const int np1 = m.npar + 1;
     for (int r = 0; r < m.ii; ++r) {
         const int i = order[r];
+        k.orig[r] = i;
         k.A[r] = m.par[size_t(i) * np1 + 0];
         k.B[r] = m.par[size_t(i) * np1 + 1];
         k.E[r] = m.par[size_t(i) * np1 + 2];

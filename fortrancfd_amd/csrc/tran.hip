@@ -57,11 +57,11 @@ __global__ void __launch_bounds__(64) k_tran_props(TranArgs a) {
 #pragma unroll
     for (int k = 0; k < NSP; ++k)
         if (lane == k) xk = x[k];
-    const double lt = log(T);
+    const double lt = fm_log(T);
     double visk = 1.0, lamk = 1.0;
     if (lane < NSP) {
-        visk = exp(mceval(tr->cofeta[lane], tr->no, lt));
-        lamk = exp(mceval(tr->coflam[lane], tr->no, lt));
+        visk = fm_exp(mceval(tr->cofeta[lane], tr->no, lt));
+        lamk = fm_exp(mceval(tr->coflam[lane], tr->no, lt));
     }
     sx[lane] = xk;
     svis[lane] = visk;
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(64) k_tran_props(TranArgs a) {
             double djk = 0.0;
             if (j != lane) {
                 const int hi = j > lane ? j : lane, lo = j > lane ? lane : j;
-                djk = exp(-mceval(tr->cofd[hi][lo], tr->no, lt));
+                djk = fm_exp(-mceval(tr->cofd[hi][lo], tr->no, lt));
             }
             sxw = sxw + sxx[j] * tr->twt[j];
             sxod = sxod + sxx[j] * djk;

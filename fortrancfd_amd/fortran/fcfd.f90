@@ -3,7 +3,8 @@
 ! The reference driver's call sites keep their shape:
 !   calc_dt      (SRC/main.f90:134)  -> fcfd_calc_dt(ctx, cfl, dt)
 !   solve_fluid  (SRC/main.f90:142)  -> fcfd_solve_fluid(ctx, dt)
-!   solve_source (SRC/main.f90:145)  -> fcfd_solve_source(ctx, dt, time, hrr, hrr_dot)
+!   solve_source (SRC/main.f90:145)  -> fcfd_solve_source(ctx, dt, time, c_loc(hrr), c_loc(hrr_dot))
+!                                       (c_null_ptr for either diagnostic skips it)
 !   refresh      (SRC/main.f90:149-163) -> fcfd_refresh(ctx)
 ! with the block state moved to the device once (fcfd_upload from b_obj(ib)%Qc / %qf, whose storage
 ! order -- equation index fastest, ghosts included -- the library takes as is) and back for output
@@ -17,7 +18,10 @@ module fcfd
     public :: fcfd_mech_load, fcfd_mech_free, fcfd_mech_dims
     public :: fcfd_create, fcfd_destroy, fcfd_lvir, fcfd_upload, fcfd_download, fcfd_init_cells
     public :: fcfd_calc_dt, fcfd_solve_fluid, fcfd_solve_source, fcfd_refresh, fcfd_advance
-    public :: fcfd_flux_advection_host, fcfd_flux_tran_host, fcfd_chem_step_host
+    public :: fcfd_flux_advection_host, fcfd_flux_tran_host, fcfd_chem_step_host, fcfd_rates_host
+    public :: fcfd_chem_stats, fcfd_source_accepted, fcfd_set_chem_workgroups
+    public :: fcfd_sur_create, fcfd_sur_free, fcfd_sur_nparams, fcfd_sur_set_params, fcfd_sur_get_params
+    public :: fcfd_sur_set_norm, fcfd_sur_set_thresholds, fcfd_set_surrogate
 
     ! Same fields and codes as the reference's &params switches (SRC/params.f90:106-111).
     type, bind(C) :: fcfd_opts
@@ -88,11 +92,69 @@ module fcfd
             type(c_ptr), value :: ctx
             real(c_double), value :: dt
         end function
+        ! hrr / hrr_dot: c_loc of one value per interior cell (eos_obj%HRR / %HRR_dot, blocks in
+        ! order), or c_null_ptr
         integer(c_int) function fcfd_solve_source(ctx, dt, time, hrr, hrr_dot) bind(C, name="fcfd_solve_source")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx
             real(c_double), value :: dt, time
-            real(c_double), intent(out) :: hrr(*), hrr_dot(*)
+            type(c_ptr), value :: hrr, hrr_dot
+        end function
+        ! steps, rejects, Newton iterations, failed flag of the last chemistry call, per interior cell
+        integer(c_int) function fcfd_chem_stats(ctx, per_cell) bind(C, name="fcfd_chem_stats")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(out) :: per_cell(*)
+        end function
+        integer(c_int) function fcfd_source_accepted(ctx, per_cell, n_fallback) bind(C, name="fcfd_source_accepted")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), intent(out) :: per_cell(*), n_fallback
+        end function
+        integer(c_int) function fcfd_set_chem_workgroups(ctx, n) bind(C, name="fcfd_set_chem_workgroups")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: n
+        end function
+        ! the chemistry surrogate: create, load parameters / normalisation, attach to a context
+        integer(c_int) function fcfd_sur_create(mech, sur) bind(C, name="fcfd_sur_create")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: mech
+            type(c_ptr), intent(out) :: sur
+        end function
+        subroutine fcfd_sur_free(sur) bind(C, name="fcfd_sur_free")
+            import :: c_ptr
+            type(c_ptr), value :: sur
+        end subroutine
+        integer(c_int) function fcfd_sur_nparams() bind(C, name="fcfd_sur_nparams")
+            import :: c_int
+        end function
+        integer(c_int) function fcfd_sur_set_params(sur, params) bind(C, name="fcfd_sur_set_params")
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: sur
+            real(c_float), intent(in) :: params(*)
+        end function
+        integer(c_int) function fcfd_sur_get_params(sur, params) bind(C, name="fcfd_sur_get_params")
+            import :: c_int, c_ptr, c_float
+            type(c_ptr), value :: sur
+            real(c_float), intent(out) :: params(*)
+        end function
+        ! norm(64, 6): input mean, input std, training-range low, high, output mean, output std
+        integer(c_int) function fcfd_sur_set_norm(sur, norm) bind(C, name="fcfd_sur_set_norm")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: sur
+            real(c_double), intent(in) :: norm(*)
+        end function
+        integer(c_int) function fcfd_sur_set_thresholds(sur, resid_tol, t_tol, verify_tol) &
+                bind(C, name="fcfd_sur_set_thresholds")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: sur
+            real(c_double), value :: resid_tol, t_tol, verify_tol
+        end function
+        ! sur = c_null_ptr: back to the FP64 integrator on every cell
+        integer(c_int) function fcfd_set_surrogate(ctx, sur) bind(C, name="fcfd_set_surrogate")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, sur
         end function
         integer(c_int) function fcfd_refresh(ctx) bind(C, name="fcfd_refresh")
             import :: c_int, c_ptr
@@ -132,6 +194,16 @@ module fcfd
             real(c_double), intent(in) :: dt(*), rho(*)
             real(c_double), intent(inout) :: yT(*)
             integer(c_int), intent(out) :: stats(*)
+        end function
+        ! CKWYR / CKCDYR on n states: rho [g/cm^3], T [K], Y(KK, n) -> wdot, cdot, ddot (KK, n),
+        ! qf, qr (II, n)
+        integer(c_int) function fcfd_rates_host(mech, n, rho, T, Y, wdot, cdot, ddot, qf, qr) &
+                bind(C, name="fcfd_rates_host")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: mech
+            integer(c_int), value :: n
+            real(c_double), intent(in) :: rho(*), T(*), Y(*)
+            real(c_double), intent(out) :: wdot(*), cdot(*), ddot(*), qf(*), qr(*)
         end function
         type(c_ptr) function fcfd_last_error_c() bind(C, name="fcfd_last_error")
             import :: c_ptr

@@ -136,6 +136,19 @@ class Mechanism:
         _check(lib().fcfd_qc_to_qf_host(self.handle, C.c_int(Qc.shape[0]), _p(Qc), _p(out)))
         return out
 
+    def rates(self, rho, T, Y):
+        """CKWYR + CKCDYR on the device (the integrator's rate code): rho [g/cm^3] and T of shape
+        [n], Y [n, KK] -> (wdot, cdot, ddot [n, KK], qf, qr [n, II])."""
+        rho, T = _f64(np.atleast_1d(rho)), _f64(np.atleast_1d(T))
+        Y = _f64(np.atleast_2d(Y))
+        n = T.size
+        assert rho.size == n and Y.shape == (n, self.kk)
+        w, c, d = (np.zeros((n, self.kk)) for _ in range(3))
+        qf, qr = np.zeros((n, self.ii)), np.zeros((n, self.ii))
+        _check(lib().fcfd_rates_host(self.handle, C.c_int(n), _p(rho), _p(T), _p(Y), _p(w), _p(c),
+                                     _p(d), _p(qf), _p(qr)))
+        return w, c, d, qf, qr
+
     def macks(self, dt, rho, yT, rtol=1e-5, atol=1e-13):
         """macks(dt, nspe+1, yiT, rho) (SRC/source.f90:58) on n cells: yT rows (Y..., T), dt and
         rho [kg/m^3] per cell.  Returns (yT_new, stats[n, 4] = steps, rejects, newton, failed)."""
@@ -244,8 +257,8 @@ class Solver:
         _check(lib().fcfd_set_chem_workgroups(self.handle, C.c_int(n)))
 
     def source_accepted(self):
-        """(per-cell route of the last solve_source: 2 explicit screen, 1 surrogate, 0 implicit
-        FP64 integrator; number of implicit-integrator cells)."""
+        """(per-cell route of the last solve_source: 2 explicit step over dt, 3 two explicit half
+        steps, 1 surrogate, 0 implicit FP64 integrator; number of implicit-integrator cells)."""
         acc = np.zeros(self.nblk * self.ncell, np.int32)
         nf = C.c_int()
         _check(lib().fcfd_source_accepted(self.handle, acc.ctypes.data_as(C.POINTER(C.c_int)),

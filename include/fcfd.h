@@ -37,7 +37,7 @@ typedef struct fcfd_opts {
     int do_hydro;        /* 1: advection fluxes                                   */
     int do_tran;         /* 1: viscous/heat/diffusive fluxes (needs tran.bin)     */
     int do_reaction;     /* 1: chemistry source                                   */
-    int select_time;     /* 0 explicit Euler, 1 TVD-RK3                           */
+    int select_time;     /* 0 explicit Euler, 1 TVD-RK3, 2 RK4                    */
     int select_riemann;  /* 0 LxF, 1 HLL, 2 HLLC                                  */
     int flag_acu;        /* 0 first order, 1 MUSCL, 2 WENO                        */
     int flag_muscl;      /* 1 unlimited, 2 minmod, 3 superbee, 4 MC               */
@@ -104,10 +104,12 @@ int fcfd_chem_stats(fcfd_ctx *c, int *per_cell);
 /* Post-step boundary / halo / primitive refresh (SRC/main.f90:149-163). */
 int fcfd_refresh(fcfd_ctx *c);
 
-/* Device-resident stepping for production loops: nsteps x {calc_dt, solve_fluid, refresh}
- * with dt kept on the device and the step replayed from a captured HIP graph.  *time is
- * advanced by the sum of the steps' dt.  Equivalent to the driver loop SRC/main.f90:130-163
- * without output. */
+/* Device-resident stepping for production loops: nsteps x {calc_dt, solve_fluid, solve_source
+ * (when do_reaction = 1), refresh} with dt kept on the device and the step replayed from a
+ * captured HIP graph.  *time is advanced by the sum of the steps' dt.  Equivalent to the driver
+ * loop SRC/main.f90:130-163 without output; the source step computes no HRR / HRR_dot there (the
+ * reference computes them on every call for its output, SRC/source.f90:70-73: call
+ * fcfd_solve_source with hrr / hrr_dot to get them). */
 int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time);
 
 /* Kernel timing with HIP events on the context's stream (0 = off; while on, steps are launched
@@ -138,6 +140,13 @@ int fcfd_qc_to_qf_host(const fcfd_mech *m, int n, const double *Qc, double *qf);
  * weno3z(s1..4), weno5 JS(s1..6), weno5 Z, weno7z(s1..8), muscl3(s1..4) flags 1..4, (l, r) pairs,
  * last two columns zero. */
 int fcfd_recon_host(int n, const double *s, double *out);
+/* CKWYR (WORK/KineticModels/Chem_bin/GRI3.0_CH4/cklib.f:9107-9204) and CKCDYR (:1422-1541), the
+ * rates SRC/thermo.f90:105-134 and the integrator's right-hand side call, on n states: rho
+ * [g/cm^3], T [K], Y (n x KK mass fractions) in; wdot, cdot, ddot (n x KK, mol/(cm^3 s)) and the
+ * forward / reverse rates of progress qf, qr (n x II, mechanism order) out.  Evaluated by the
+ * device rate code the integrator uses (one state per wavefront). */
+int fcfd_rates_host(const fcfd_mech *m, int n, const double *rho, const double *T, const double *Y,
+                    double *wdot, double *cdot, double *ddot, double *qf, double *qr);
 
 /* The macks call (SRC/source.f90:58, macks(dt, nspe+1, yiT, rho)) on n independent cells:
  * yT is n rows of (Y_1..Y_KK, T), updated in place; dt and rho (kg/m^3) per cell; stats
@@ -197,17 +206,18 @@ int fcfd_sur_train_step(fcfd_surrogate *s, const void *X_all, const float *Tg_al
                         int batch, float lr, float *loss, void *stream);
 /* Chemistry of n independent cells (arrays as fcfd_chem_batch) through the explicit screen, the
  * surrogate for the cells the screen rejects, and the FP64 implicit integrator for the cells the
- * surrogate does not accept.  route[n] (may be NULL) receives 2 (explicit screen), 1 (surrogate)
- * or 0 (implicit integrator) per cell; stats[4n] may be NULL; *n_fallback (may be NULL) receives
- * the number of implicit-integrator cells. */
+ * surrogate does not accept.  route[n] (may be NULL) receives 2 (explicit step over dt), 3 (two
+ * explicit half steps), 1 (surrogate) or 0 (implicit integrator) per cell; stats[4n] may be NULL;
+ * *n_fallback (may be NULL) receives the number of implicit-integrator cells. */
 int fcfd_sur_infer_batch(fcfd_surrogate *s, const fcfd_opts *o, int n, const double *rho,
                          const double *dt, double *T, double *Y, int *route, int *n_fallback,
                          int *stats, void *stream);
 /* solve_source of a context through the surrogate (NULL: the FP64 integrator on every cell).
  * hrr / hrr_dot of fcfd_solve_source are not available in surrogate mode. */
 int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s);
-/* Per interior cell of the last solve_source: 2 explicit screen, 1 surrogate, 0 FP64 implicit
- * integrator; *n_fallback = number of cells with 0. */
+/* Per interior cell of the last solve_source: 2 explicit step over dt (the screen), 3 two explicit
+ * half steps (the retry), 1 surrogate, 0 FP64 implicit integrator; *n_fallback = number of cells
+ * with 0. */
 int fcfd_source_accepted(fcfd_ctx *c, int *per_cell, int *n_fallback);
 /* Grid of the FP64 implicit integrator in this context's solve_source: n one-wave workgroups for
  * lists of stiff cells longer than 8 n, else n / 4 four-wave workgroups (0: the default, four per

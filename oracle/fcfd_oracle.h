@@ -99,6 +99,7 @@ typedef struct {
     double *godunov;     /* nblk * (ncell+1) * 3           */
     double *Qc0, *k1, *k2, *k3, *k4;  /* interior work   */
     double *hmem;        /* nblk * ncell: the integrator's last step-size proposal (0: none) */
+    int *cstats;         /* nblk * ncell * 4: steps, rejects, newton, failed of the last solve_source */
 } fo_domain;
 
 /* Allocates and initialises from (T, p, u, X) per interior cell the way initialize_variables /
@@ -133,6 +134,9 @@ int fo_chem_step(const fo_mech *m, double dt, double rho, double *yT, double rto
 /* reaction_macks (SRC/source.f90:17-84) over a domain's interior cells. */
 void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *hrr,
                      double *hrr_dot, int *nfail);
+
+/* exp (0), log (1), log10 (2), pow (3) of the build's math on n arguments */
+void fo_math_eval(int which, int n, const double *x, const double *y, double *out);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libfcfd_oracle.so")
+# the same restatement with the C library's exp / log / log10 / pow (what the reference calls)
+LIBM_PATH = os.path.join(HERE, "_build", "libfcfd_oracle_libm.so")
 REPO = os.path.dirname(HERE)
 MECH_DIR = os.path.join(REPO, "fortrancfd_amd", "data", "GRI3.0_CH4")
 
@@ -24,10 +26,11 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
 
 
-def _lib():
-    if not os.path.exists(LIB_PATH):
+def _lib(libm=False):
+    path = LIBM_PATH if libm else LIB_PATH
+    if not os.path.exists(path):
         build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(path)
     lib.fo_mech_load.restype = C.c_void_p
     lib.fo_mech_load.argtypes = [C.c_char_p, C.c_char_p]
     for name in ["fo_pressure", "fo_density", "fo_specific_energy", "fo_cp", "fo_cv",
@@ -50,6 +53,12 @@ class FoDomain(C.Structure):
                 ("Qc", _D), ("qf", _D), ("Fplus", _D), ("godunov", _D)]
 
 
+class FoDomainStats(C.Structure):
+    """fo_domain up to its per-cell integrator counters."""
+    _fields_ = FoDomain._fields_ + [("Qc0", _D), ("k1", _D), ("k2", _D), ("k3", _D), ("k4", _D),
+                                    ("hmem", _D), ("cstats", _I)]
+
+
 def _p(a):
     return a.ctypes.data_as(_D)
 
@@ -61,8 +70,10 @@ def _f64(a):
 class Oracle:
     """CPU restatement of the reference path on one mechanism."""
 
-    def __init__(self, mech_dir: str = MECH_DIR, transport: bool = True):
-        self.lib = _lib()
+    def __init__(self, mech_dir: str = MECH_DIR, transport: bool = True, libm: bool = False):
+        """libm=False: transcendental functions from the library's fcfd_math.h (the GPU's
+        operations); libm=True: the C library's (the reference's)."""
+        self.lib = _lib(libm)
         chem = os.path.join(mech_dir, "chem.bin").encode()
         tran = os.path.join(mech_dir, "tran.bin").encode() if transport else None
         self.m = C.c_void_p(self.lib.fo_mech_load(chem, tran))
@@ -317,3 +328,10 @@ class Domain:
         self.orc.lib.fo_solve_source(self.ptr, C.c_double(dt), C.c_double(rtol), C.c_double(atol),
                                      _p(hrr), _p(hrd), C.byref(nf))
         return hrr, hrd, nf.value
+
+    def chem_stats(self):
+        """Per interior cell (block-major): steps, rejects, newton, failed of the last
+        solve_source, as the library's fcfd_chem_stats."""
+        d = FoDomainStats.from_address(self.ptr.value)
+        n = self.nblk * self.ncell
+        return np.ctypeslib.as_array(d.cstats, shape=(n * 4,)).reshape(n, 4).copy()

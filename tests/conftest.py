@@ -15,8 +15,17 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def orc():
+    """The oracle with the library's own exp/log/log10/pow (the operations the GPU executes)."""
     from oracle.pyoracle import Oracle
     return Oracle()
+
+
+@pytest.fixture(scope="session")
+def orc_libm():
+    """The same oracle built with the C library's exp/log/log10/pow, i.e. the functions the
+    reference Fortran calls: it reproduces the reference's golden outputs bit for bit."""
+    from oracle.pyoracle import Oracle
+    return Oracle(libm=True)
 
 
 def golden(name):

@@ -1,10 +1,16 @@
-"""HIP chemistry source (one cell per wavefront, SDIRK3 + pivoted LU) against the CPU restatement
-of the same integrator, and the invariants the physics fixes.  The integrator is this project's own
-design (the reference's MACKS source is not shipped), so parity is GPU vs CPU restatement; the CPU
-restatement is pinned to scipy's Radau5 and to the invariants in tests/test_chem_oracle.py.
-Tolerances: transcendental functions (exp/log/pow) differ by ~1 ulp between the GPU's and the host's
-math libraries, so bit-equality is not expected; the written bound is 1e-9 relative on T and 1e-9
-absolute on Y (the integration itself runs at rtol 1e-5 / atol 1e-13)."""
+"""HIP chemistry source against the CPU restatement of the same integrator (the reference's MACKS
+source is not shipped, so parity is GPU vs CPU restatement; the restatement is pinned to scipy's
+Radau5 and to the invariants in tests/test_chem_oracle.py), and the device rate code against the
+reference's own CKWYR / CKCDYR outputs (tests/golden/rates.npz).
+
+Both sides evaluate exp / log / log10 / pow with the same code (fortrancfd_amd/csrc/fcfd_math.h), so
+the remaining differences come from the order of floating-point sums (wave-wide tree sums on the
+GPU, sequential loops on the CPU), i.e. round-off.  Comparisons of the integrator first require the
+per-cell counters (steps, rejects, Newton iterations) to be identical, then hold the result to 1e-12
+relative: T relative, each mass fraction relative to max(|Y_k|, 1e-13) (the integrator's absolute
+tolerance; below it a mass fraction is noise at the integrator's own tolerance).  Heat-release
+rates are differences of nearly equal sums; they are held to 1e-12 of the magnitude of their terms.
+"""
 import numpy as np
 import pytest
 
@@ -43,16 +49,89 @@ def mixing_line_cells(orc, n, seed):
     return np.column_stack([Y, T]), rho, dt
 
 
+REL = 1e-12
+Y_FLOOR = 1e-13
+
+
+def cell_errors(out, ref):
+    """Per cell, the larger of the T relative difference and max_k |dY_k| / max(|Y_k|, Y_FLOOR);
+    prints where the largest one sits."""
+    et = np.abs(out[..., -1] - ref[..., -1]) / np.abs(ref[..., -1])
+    eyk = np.abs(out[..., :-1] - ref[..., :-1]) / np.maximum(np.abs(ref[..., :-1]), Y_FLOOR)
+    ey = eyk.max(axis=-1)
+    c = int(np.argmax(np.maximum(et, ey)))
+    k = int(np.argmax(eyk[c]))
+    print(f"  largest difference: cell {c}, T {et[c]:.3g}, species {k} {ey[c]:.3g} "
+          f"(Y {ref[c, k]:.3g} vs {out[c, k]:.3g})", flush=True)
+    return np.maximum(et, ey)
+
+
 def test_macks_batch_vs_oracle(mech, orc):
-    yT, rho, dt = mixing_line_cells(orc, 48, 11)
-    print("macks batch: 48 cells", flush=True)
+    yT, rho, dt = mixing_line_cells(orc, 96, 11)
+    print("macks batch: 96 cells", flush=True)
     out, st = mech.macks(dt, rho, yT)
     assert (st[:, 3] == 0).all()
+    refs, rsts = [], []
     for c in range(yT.shape[0]):
         ref, rst, rc = orc.chem_step(dt[c], rho[c], yT[c])
         assert rc == 0
-        assert abs(out[c, -1] - ref[-1]) <= 1e-9 * ref[-1], (c, out[c, -1], ref[-1], st[c], rst)
-        assert np.abs(out[c, :-1] - ref[:-1]).max() <= 1e-9, c
+        refs.append(ref)
+        rsts.append((rst["steps"], rst["rejects"], rst["newton"]))
+    refs, rsts = np.array(refs), np.array(rsts)
+    same = (st[:, :3] == rsts).all(axis=1)
+    err = cell_errors(out, refs)
+    print(f"identical counters in {same.sum()} of {same.size} cells; max difference {err.max():.3g}")
+    assert same.all(), np.nonzero(~same)
+    assert err.max() <= REL, err.max()
+
+
+def test_rates_vs_reference_golden(mech, orc_libm):
+    """Device CKWYR / CKCDYR (the integrator's rate code) against the reference's own outputs for
+    40 states (tests/golden/rates.npz): rates of progress within 1e-12 relative; production,
+    creation and destruction rates within 1e-12 of the magnitude of the terms they sum (the wave
+    adds them in another order than CKWYR's loop, and they cancel)."""
+    g = golden("rates")
+    ii, kk = mech.ii, mech.kk
+    rho = g["rho"] * 1e-3
+    w, c, d, qf, qr = mech.rates(rho, g["T"], g["Y"])
+    par, nu, nk, ns, nr = orc_libm.arrhenius()
+    inc = np.zeros((kk, ii))      # |nu| of each species in each reaction (both sides counted)
+    for i in range(ii):
+        for n in range(6):
+            if nk[i, n] > 0:
+                inc[nk[i, n] - 1, i] += abs(nu[i, n])
+    worst = 0.0
+    for s_ in range(len(g["T"])):
+        ref = g["out"][s_]
+        rqf, rqr = ref[3 * kk:3 * kk + ii], ref[3 * kk + ii:3 * kk + 2 * ii]
+        rel = lambda a, b: np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+        assert rel(qf[s_], rqf).max() <= REL and rel(qr[s_], rqr).max() <= REL, s_
+        scale = inc @ (np.abs(rqf) + np.abs(rqr))
+        for got, want in ((w[s_], ref[:kk]), (c[s_], ref[kk:2 * kk]), (d[s_], ref[2 * kk:3 * kk])):
+            e = np.abs(got - want) / np.maximum(scale, 1e-300)
+            assert e.max() <= REL, s_
+            worst = max(worst, e.max())
+    print("device rates vs reference CKWYR: worst scaled difference", worst)
+
+
+def test_rates_vs_oracle(mech, orc):
+    """Device rates against the restatement with the same math: every rate of progress of a
+    reaction without third body agrees bit for bit; third-body and fall-off rows use the total
+    concentration, which the wave sums in another order (1e-14 relative)."""
+    g = golden("rates")
+    kk = mech.kk
+    rho = g["rho"] * 1e-3
+    w, c, d, qf, qr = mech.rates(rho, g["T"], g["Y"])
+    par, nu, nk, ns, nr = orc.arrhenius()
+    exact = 0
+    for s_ in range(len(g["T"])):
+        ow, oqf, oqr = orc.ckwyr(rho[s_], g["T"][s_], g["Y"][s_])
+        rel = np.maximum(np.abs(qf[s_] - oqf) / np.maximum(np.abs(oqf), 1e-300),
+                         np.abs(qr[s_] - oqr) / np.maximum(np.abs(oqr), 1e-300))
+        assert rel.max() <= 1e-14, s_
+        exact += int((rel == 0).sum())
+    print(f"rates of progress bit-identical: {exact} of {len(g['T']) * mech.ii}")
+    assert exact >= 0.8 * len(g["T"]) * mech.ii
 
 
 def test_macks_invariants(mech, orc):
@@ -105,7 +184,7 @@ def test_solve_source_vs_oracle(mech, orc):
         hrr_o, hrd_o, nf = d.solve_source(dt)
         hrr_g, hrd_g = s.solve_source(dt)
         assert nf == 0
-    compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o)
+    compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o, orc, dt)
 
 
 def test_solve_source_long_list_vs_oracle(mech, orc):
@@ -127,7 +206,7 @@ def test_solve_source_long_list_vs_oracle(mech, orc):
         assert nf == 0
         acc, _ = s.source_accepted()
         assert (acc == 0).sum() > 8     # a long list
-    compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o)
+    compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o, orc, dt)
 
 
 def test_solve_source_lane_retry_vs_oracle(mech, orc):
@@ -158,19 +237,54 @@ def test_solve_source_lane_retry_vs_oracle(mech, orc):
             if call == 0:
                 acc, _ = s.source_accepted()
                 st = s.chem_stats()
-                assert (acc == 0).sum() == 64                           # every cell fails the screen
+                assert (acc != 2).sum() == 64                           # every cell fails the screen
                 assert ((st[:, 0] == 2) & (st[:, 2] == 0)).sum() == 53   # finished by the retry
-        compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o)
+                assert (acc == 3).sum() == 53 and (acc == 0).sum() == 11
+        compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o, orc, dt)
 
 
-def compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o):
+def compare_with_oracle(s, d, hrr_g, hrd_g, hrr_o, hrd_o, orc=None, dt=None):
+    """qf / Qc / HRR / HRR_dot of the last solve_source: counters first, then 1e-12 (see the module
+    docstring for the scales)."""
     Qc, qf = s.download()
     lv = s.lv
-    np.testing.assert_allclose(qf[:, lv:-lv, 2], d.qf[:, lv:-lv, 2], rtol=1e-9)
-    np.testing.assert_allclose(qf[:, lv:-lv, 3:], d.qf[:, lv:-lv, 3:], rtol=0, atol=1e-9)
-    np.testing.assert_allclose(Qc[:, lv:-lv, :], d.Qc[:, lv:-lv, :], rtol=1e-9, atol=1e-12)
-    np.testing.assert_allclose(hrr_g, hrr_o, rtol=1e-6, atol=1e-6 * np.abs(hrr_o).max())
-    np.testing.assert_allclose(hrd_g, hrd_o, rtol=1e-6, atol=1e-6 * np.abs(hrd_o).max())
+    st_g, st_o = s.chem_stats(), d.chem_stats()
+    same = (st_g[:, :3] == st_o[:, :3]).all(axis=1)
+    print(f"identical counters in {same.sum()} of {same.size} cells", flush=True)
+    assert same.all(), (np.nonzero(~same), st_g[~same], st_o[~same])
+    g = qf[:, lv:-lv, 2:].reshape(-1, qf.shape[-1] - 2)
+    o = d.qf[:, lv:-lv, 2:].reshape(-1, qf.shape[-1] - 2)
+    yt_g = np.concatenate([g[:, 1:], g[:, :1]], axis=1)
+    yt_o = np.concatenate([o[:, 1:], o[:, :1]], axis=1)
+    err = cell_errors(yt_g, yt_o)
+    print(f"max difference of T, Y: {err.max():.3g}", flush=True)
+    assert err.max() <= REL
+    rho_max = np.abs(d.Qc[:, lv:-lv, 0]).max()
+    np.testing.assert_allclose(Qc[:, lv:-lv, :], d.Qc[:, lv:-lv, :], rtol=REL, atol=REL * Y_FLOOR * rho_max)
+    if orc is not None:
+        eh, ed = hrr_errors(orc, d, dt, hrr_g, hrd_g, hrr_o, hrd_o)
+        print(f"HRR, HRR_dot: max scaled difference {eh:.3g}, {ed:.3g}", flush=True)
+        assert eh <= REL and ed <= REL
+
+
+def hrr_errors(orc, d, dt, hrr_g, hrd_g, hrr_o, hrd_o):
+    """Largest HRR and HRR_dot differences, each relative to the magnitude of the terms it sums:
+    HRR = -sum rho h_k (Y_k - Y0_k) / dt against sum |rho h_k Y_k| / dt (the terms before the
+    difference); HRR_dot = -sum W_k h_k (c_k - d_k) against sum |W_k h_k| (c_k + d_k)."""
+    lv = d.lv
+    h = orc.hms(298.15)
+    W = orc.weights()
+    rho = d.qf[:, lv:-lv, 0].reshape(-1)
+    T = d.qf[:, lv:-lv, 2].reshape(-1)
+    Y = d.qf[:, lv:-lv, 3:].reshape(-1, W.size)
+    sc_h = (np.abs(h) * Y).sum(axis=1) * rho * 1e-3 / dt * 0.1
+    sc_d = np.zeros_like(sc_h)
+    for c in range(Y.shape[0]):
+        cd, dd = orc.ckcdyr(rho[c] * 1e-3, T[c], Y[c])
+        sc_d[c] = (np.abs(W * h) * (cd + dd)).sum() * 0.1
+    eh = np.abs(hrr_g - hrr_o) / sc_h
+    ed = np.abs(hrd_g - hrd_o) / np.maximum(sc_d, 1e-300)
+    return float(eh.max()), float(ed.max())
 
 
 def test_coupled_advance_graph_matches_stepwise(mech):

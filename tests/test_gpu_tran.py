@@ -1,8 +1,9 @@
 """HIP transport flux (calc_flux_tran, SRC/flux.f90:18-127) against the CPU restatement, whose
 assembly is pinned in test_tran_oracle.py and whose TRANLIB property routines are pinned
-bit-for-bit to the reference in test_oracle.py.  The device exp / log round differently from the
-host's by an ulp or so and enter every coefficient, so the bound is relative: 1e-11 of each flux
-component's largest magnitude for one evaluation, 1e-9 after three coupled RK3 steps."""
+bit-for-bit to the reference in test_oracle.py.  Both sides evaluate exp / log with the same code
+(fortrancfd_amd/csrc/fcfd_math.h); the remaining differences are the order of the species sums of
+the mixture rules (wave sums on the GPU), so the bound is 1e-12 relative to each flux component's
+largest magnitude, for one evaluation and after three coupled RK3 steps."""
 import numpy as np
 import pytest
 
@@ -55,7 +56,9 @@ def test_flux_tran_vs_oracle(mech, orc):
     F = mech.calc_flux_tran(Qc, n, lv, dx)
     R = orc.flux_tran(Qc, n, lv, dx)
     scale = np.abs(R).max(axis=0) + 1e-300
-    assert np.all(np.abs(F - R) <= 1e-11 * scale)
+    err = (np.abs(F - R) / scale).max()
+    print(f"transport flux: max scaled difference {err:.3g}, bit-identical {np.mean(F == R):.3f}")
+    assert err <= 1e-12
     assert np.abs(R[:, 2]).max() > 0 and np.abs(R[:, 3:]).max() > 0
 
 
@@ -71,11 +74,13 @@ def test_coupled_steps_with_transport_vs_oracle(mech, orc):
     print("3 RK3 steps with transport, 2x48 cells", flush=True)
     for _ in range(3):
         dt_g, dt_o = s.calc_dt(0.5), d.calc_dt(0.5)
-        assert abs(dt_g - dt_o) <= 1e-12 * dt_o
+        assert abs(dt_g - dt_o) <= 1e-14 * dt_o
         s.solve_fluid(dt_o)
         d.solve_fluid(dt_o)
         d.refresh()
     Qc, qf = s.download()
     scale = np.abs(d.Qc).max(axis=(0, 1)) + 1e-300
-    assert np.all(np.abs(Qc - d.Qc) <= 1e-9 * scale)
-    assert np.all(np.abs(qf[..., 2] - d.qf[..., 2]) <= 1e-9 * d.qf[..., 2])
+    err = (np.abs(Qc - d.Qc) / scale).max()
+    errT = (np.abs(qf[..., 2] - d.qf[..., 2]) / d.qf[..., 2]).max()
+    print(f"3 steps with transport: max scaled difference Qc {err:.3g}, T {errT:.3g}")
+    assert err <= 1e-12 and errT <= 1e-12

@@ -45,7 +45,9 @@ def test_reconstruction_golden(orc):
     np.testing.assert_array_equal(orc.recon(g["s"]), g["out"])
 
 
-def test_thermo_golden(orc):
+@pytest.mark.parametrize("variant", ["libm", "fm"])
+def test_thermo_golden(orc, orc_libm, variant):
+    orc = orc_libm if variant == "libm" else orc
     g = golden("thermo")
     for i, (T, rho, Y) in enumerate(zip(g["T"], g["rho"], g["Y"])):
         X = orc.ytx(Y)
@@ -66,7 +68,10 @@ def test_eos_golden(orc):
     np.testing.assert_array_equal(orc.qc_to_qf(g["Qc"]), g["Qc2qf"])
 
 
-def test_rates_golden(orc):
+def test_rates_golden(orc_libm):
+    """The restatement built with the C library's math reproduces the reference's CKWYR, CKCDYR
+    and HRR_dot bit for bit."""
+    orc = orc_libm
     g = golden("rates")
     ii = orc.ii
     for i in range(len(g["T"])):
@@ -82,18 +87,62 @@ def test_rates_golden(orc):
         assert orc.heat_release_rate_omegadot(Y, rho, T) == ref[-1]
 
 
-def test_hrr_golden(orc):
+@pytest.mark.parametrize("variant", ["libm", "fm"])
+def test_hrr_golden(orc, orc_libm, variant):
+    orc = orc_libm if variant == "libm" else orc
     g = golden("hrr")
     for r, h in zip(g["rows"], g["hrr"]):
         assert orc.heat_release_rate(r[2:2 + KK], r[2 + KK:], r[0], r[1]) == h
 
 
-def test_transport_golden(orc):
+def test_transport_golden(orc_libm):
+    orc = orc_libm
     g = golden("trans")
     for i in range(len(g["T"])):
         T, p, X = g["T"][i], g["p"][i], g["X"][i]
         mine = [orc.mcavis(T, X) * 0.1, orc.mcacon(T, X) * 1e-5, *(orc.mcadif(p * 10, T, X) * 1e-4)]
         np.testing.assert_array_equal(mine, g["out"][i])
+
+
+def _terms_scale(nu_rows, q):
+    """Per species, the sum of |nu * q| over the terms CKWYR adds (the scale of its cancellation)."""
+    return nu_rows @ np.abs(q)
+
+
+def test_rates_golden_shared_math(orc):
+    """The restatement with the library's own exp/log/log10/pow (the build the GPU is compared
+    with) against the reference's CKWYR / CKCDYR: rates of progress within 1e-13 relative, and the
+    production rates within 1e-13 of the magnitude of the terms they sum (they cancel)."""
+    g = golden("rates")
+    ii = orc.ii
+    par, nu, nk, ns, nr = orc.arrhenius()
+    inc = np.zeros((KK, ii))
+    for i in range(ii):
+        for n in range(6):
+            if nk[i, n] > 0:
+                inc[nk[i, n] - 1, i] += abs(nu[i, n])
+    worst = 0.0
+    for i in range(len(g["T"])):
+        rho, T, Y = g["rho"][i], g["T"][i], g["Y"][i]
+        w, qf, qr = orc.ckwyr(rho * 1e-3, T, Y)
+        c, d = orc.ckcdyr(rho * 1e-3, T, Y)
+        ref = g["out"][i]
+        rqf, rqr = ref[3 * KK:3 * KK + ii], ref[3 * KK + ii:3 * KK + 2 * ii]
+        assert maxrel(qf, rqf) <= 1e-13 and maxrel(qr, rqr) <= 1e-13, i
+        scale = _terms_scale(inc, rqf) + _terms_scale(inc, rqr)
+        err = np.abs(w - ref[:KK]) / np.maximum(scale, 1e-300)
+        assert err.max() <= 1e-13, i
+        assert maxrel(c, ref[KK:2 * KK]) <= 1e-13 and maxrel(d, ref[2 * KK:3 * KK]) <= 1e-13, i
+        worst = max(worst, err.max(), maxrel(qf, rqf))
+    print("shared-math rates vs reference: worst relative difference", worst)
+
+
+def test_transport_golden_shared_math(orc):
+    g = golden("trans")
+    for i in range(len(g["T"])):
+        T, p, X = g["T"][i], g["p"][i], g["X"][i]
+        mine = [orc.mcavis(T, X) * 0.1, orc.mcacon(T, X) * 1e-5, *(orc.mcadif(p * 10, T, X) * 1e-4)]
+        assert maxrel(mine, g["out"][i]) <= 1e-13, i
 
 
 def test_flux_golden(orc):
@@ -146,10 +195,12 @@ def test_sod_config_golden_first_steps(orc):
 # ---------------------------------------------------------------------------------------------
 # what the mathematics fixes
 # ---------------------------------------------------------------------------------------------
-def test_equilibrium_detailed_balance(orc):
+@pytest.mark.parametrize("variant", ["libm", "fm"])
+def test_equilibrium_detailed_balance(orc, orc_libm, variant):
     """At the reference's Cantera TP-equilibrium state (1300 K, 1 atm) every reversible reaction's
     net rate vanishes relative to its forward rate: pins K_c (NASA Gibbs energies, pressure
     factor, sign conventions) independently of CKLIB."""
+    orc = orc_libm if variant == "libm" else orc
     g = golden("equil_tp_ch4_phi1_1atm_1300K")
     X = g["X"]
     Y = orc.xty(X)
