@@ -2,6 +2,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -350,6 +351,41 @@ void drop_graphs(fcfd_ctx *c) {
 
 }  // namespace
 
+// ---- scratch of the allocation-free batched chemistry entry point ---------------------------------
+// Work-queue counter (offset 0), stiff-list length (offset 64) and stiff list (offset 256, one int
+// per cell) of fcfd_chem_batch_noalloc, in device memory the library owns for the process lifetime
+// (each GPU holds its own copy).  Host code never uses the array's name as an address: the device
+// address of each GPU's copy is looked up once, with that GPU current, and kept per device.
+constexpr int NOALLOC_MAX_CELLS = 1 << 20;
+constexpr size_t NOALLOC_HEAD = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_HEAD + sizeof(int) * size_t(NOALLOC_MAX_CELLS);   // 4 MiB + 256 B
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 256;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// This device's copy of scratch (dev must be the current device).
+int noalloc_scratch(int dev, unsigned char **out) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) { set_error("device index out of range"); return FCFD_E_ARG; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return FCFD_E_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FCFD_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *fcfd_last_error(void) { return g_err.c_str(); }
@@ -666,6 +702,50 @@ int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double 
     if (!mm->d_list_n) HIP_TRY(hipMalloc(&mm->d_list_n, sizeof(unsigned)));
     HIP_TRY(launch_chem_full(a, m->host.kk, chem_workgroups(m), mm->d_list, mm->d_list_n,
                              static_cast<hipStream_t>(stream)));
+    HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return FCFD_OK;
+}
+
+int fcfd_mech_prepare(const fcfd_mech *m) {
+    if (!m) { set_error("fcfd_mech_prepare: invalid argument"); return FCFD_E_ARG; }
+    if (int rc = ensure_dev_tables(m)) return rc;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    unsigned char *base = nullptr;
+    return noalloc_scratch(dev, &base);
+}
+
+int fcfd_chem_batch_noalloc(const fcfd_mech *m, const fcfd_opts *o, int n, const double *rho, const double *dt,
+                            double *T, double *Y, int *stats, void *stream) {
+    if (!m || !o || n < 0 || (n > 0 && (!rho || !dt || !T || !Y))) { set_error("fcfd_chem_batch_noalloc: invalid argument"); return FCFD_E_ARG; }
+    if (!(o->rtol > 0.0 && o->atol > 0.0)) { set_error("fcfd_chem_batch_noalloc: atol and rtol must be positive"); return FCFD_E_ARG; }
+    if (n > NOALLOC_MAX_CELLS) {
+        set_error("fcfd_chem_batch_noalloc: n = " + std::to_string(n) + " exceeds the allocation-free limit of " +
+                  std::to_string(NOALLOC_MAX_CELLS) + " cells (use fcfd_chem_batch, or split the batch)");
+        return FCFD_E_ARG;
+    }
+    if (n == 0) return FCFD_OK;
+    if (int rc = check_opts(fcfd_opts{1, 0, 1, 1, 2, 2, 0, 2, 0, 0, 0, o->atol, o->rtol}, m->host.kk)) return rc;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (!m->d_thermo || m->device != dev) {
+        set_error("fcfd_chem_batch_noalloc: the mechanism's tables are not on this device; call fcfd_mech_prepare "
+                  "once (it allocates) before the allocation-free calls");
+        return FCFD_E_ARG;
+    }
+    unsigned char *base = nullptr;
+    if (int rc = noalloc_scratch(dev, &base)) return rc;
+    ChemArgs a{};
+    a.thermo = m->d_thermo; a.kin = static_cast<const DevKin *>(m->d_kin);
+    a.counter = reinterpret_cast<unsigned int *>(base);                 // work-queue head
+    a.ncells = n; a.rtol = o->rtol; a.atol = o->atol;
+    a.b_rho = rho; a.b_dt = dt; a.b_T = T; a.b_y = Y;
+    a.stats = reinterpret_cast<ChemCellStats *>(stats);
+    a.layout = 1;
+    unsigned *list_n = reinterpret_cast<unsigned *>(base + 64);          // stiff-list length
+    int *list = reinterpret_cast<int *>(base + NOALLOC_HEAD);            // stiff list
+    // both counters are cleared by kernels on the stream before use (launch_chem_full, launch_chem)
+    HIP_TRY(launch_chem_full(a, m->host.kk, chem_workgroups(m), list, list_n, static_cast<hipStream_t>(stream)));
     HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     return FCFD_OK;
 }

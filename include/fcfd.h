@@ -158,6 +158,19 @@ int fcfd_chem_step_host(const fcfd_mech *m, const fcfd_opts *o, int n, const dou
  * Calls that share a mechanism must not overlap (one work-queue counter per mechanism). */
 int fcfd_chem_batch(const fcfd_mech *m, const fcfd_opts *o, int n, const double *rho,
                     const double *dt, double *T, double *Y, int *stats, void *stream);
+/* Device copies of the mechanism's tables on the current device, and the address of this device's
+ * scratch of fcfd_chem_batch_noalloc: the set-up call (it allocates) that the allocation-free entry
+ * point needs once per device.  Other entry points do the same on first use. */
+int fcfd_mech_prepare(const fcfd_mech *m);
+/* fcfd_chem_batch without any device allocation: no counterpart in the reference.  For callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The work-queue counter and the stiff-cell list live in a static device array of the
+ * library (4 MiB + 256 B per GPU, held until the process ends), so n is limited to 2^20 cells
+ * (larger n fails with FCFD_E_ARG).  Needs fcfd_mech_prepare on this device first.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams or threads) race and must
+ * not be made.  Results are bitwise those of fcfd_chem_batch. */
+int fcfd_chem_batch_noalloc(const fcfd_mech *m, const fcfd_opts *o, int n, const double *rho,
+                            const double *dt, double *T, double *Y, int *stats, void *stream);
 
 /* ---- chemistry surrogate (no reference counterpart: stands in for the macks call of
  * SRC/source.f90:58 where its answer is accepted) ----------------------------------------------
