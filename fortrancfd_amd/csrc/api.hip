@@ -102,6 +102,10 @@ struct fcfd_ctx {
     unsigned *d_list_n = nullptr;
     double *acc = nullptr;               // RK4 stage accumulator
     int graph_next[3] = {0, 0, 0};       // state buffer after the captured step from buffer i
+    // step composition of fcfd_advance (fcfd_set_step)
+    int strang = 0;                      // 0: Lie (the reference driver's order), 1: Strang
+    int step_hrr = 0;                    // 1: the source calls of fcfd_advance compute HRR / HRR_dot
+    double *d_dth = nullptr;             // dt / 2 on the device (Strang)
     // transport
     DevTran *d_tran = nullptr;
     double *props = nullptr;
@@ -344,6 +348,30 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
     });
 }
 
+// One fcfd_advance step: calc_dt, then the fluid and chemistry in the context's splitting
+// (SRC/main.f90:134-163 for Lie; Strang puts half a chemistry step on either side of the fluid step,
+// with a refresh after the first so that the fluid stage reads updated ghost cells).
+hipError_t enqueue_step(fcfd_ctx *c, double cfl) {
+    const bool react = c->o.do_reaction == 1;
+    const bool fluid = c->o.do_hydro == 1 || c->o.do_tran == 1;
+    const bool hrr = c->step_hrr == 1;
+    hipError_t e = enqueue_dt(c, cfl, c->d_time);
+    if (e != hipSuccess) return e;
+    if (react && c->strang) {
+        if ((e = launch_scale_scalar(c->d_dt, c->d_dth, 0.5, c->stream)) != hipSuccess) return e;
+        if ((e = enqueue_source(c, 0.0, c->d_dth, false)) != hipSuccess) return e;
+        if ((e = enqueue_refresh(c)) != hipSuccess) return e;
+    }
+    if (fluid && (e = enqueue_fluid(c, 0.0, c->d_dt)) != hipSuccess) return e;
+    if (react) {
+        if ((e = enqueue_source(c, 0.0, c->strang ? c->d_dth : c->d_dt, hrr)) != hipSuccess) return e;
+        // without a source term the refresh would recompute identical primitives (QcToqf of an
+        // unchanged Qc), so it is only enqueued when the chemistry has changed the state
+        if ((e = enqueue_refresh(c)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 void drop_graphs(fcfd_ctx *c) {
     for (auto &g : c->graph)
         if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
@@ -515,7 +543,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->d_counter); (void)hipFree(c->d_hrr); (void)hipFree(c->d_hrr_dot); (void)hipFree(c->d_stats);
     (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_order); (void)hipFree(c->d_hmem); (void)hipFree(c->d_list_n);
     (void)hipFree(c->d_list2); (void)hipFree(c->d_list2_n); (void)hipFree(c->d_retry); (void)hipFree(c->d_f0);
-    (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc);
+    (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc); (void)hipFree(c->d_dth);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -614,19 +642,8 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
     if (!c || nsteps < 0 || !(cfl > 0.0)) { set_error("fcfd_advance: invalid argument"); return FCFD_E_ARG; }
     const double t0 = time ? *time : 0.0;
     HIP_TRY(hipMemcpyAsync(c->d_time, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    // Without a source term the post-step refresh recomputes identical primitives (QcToqf of
-    // an unchanged Qc), so it is only enqueued when the chemistry has changed the state.
-    // With do_reaction the step is the driver's Lie splitting (SRC/main.f90:134-163):
-    // calc_dt, solve_fluid, solve_source, refresh.
-    const bool refresh = c->o.do_reaction == 1;
-    const bool fluid = c->o.do_hydro == 1 || c->o.do_tran == 1;
     if (c->timing) {
-        for (int s = 0; s < nsteps; ++s) {
-            HIP_TRY(enqueue_dt(c, cfl, c->d_time));
-            if (fluid) HIP_TRY(enqueue_fluid(c, 0.0, c->d_dt));
-            if (refresh) HIP_TRY(enqueue_source(c, 0.0, c->d_dt, false));
-            if (refresh) HIP_TRY(enqueue_refresh(c));
-        }
+        for (int s = 0; s < nsteps; ++s) HIP_TRY(enqueue_step(c, cfl));
         HIP_TRY(collect_timing(c));
     } else {
         if (cfl != c->graph_cfl) { drop_graphs(c); c->graph_cfl = cfl; }
@@ -635,10 +652,7 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
             if (!c->graph[cur]) {
                 hipGraph_t g;
                 HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                hipError_t e = enqueue_dt(c, cfl, c->d_time);
-                if (e == hipSuccess && fluid) e = enqueue_fluid(c, 0.0, c->d_dt);
-                if (e == hipSuccess && refresh) e = enqueue_source(c, 0.0, c->d_dt, false);
-                if (e == hipSuccess && refresh) e = enqueue_refresh(c);
+                hipError_t e = enqueue_step(c, cfl);
                 hipError_t e2 = hipStreamEndCapture(c->stream, &g);
                 HIP_TRY(e);
                 HIP_TRY(e2);
@@ -660,13 +674,34 @@ int fcfd_solve_source(fcfd_ctx *c, double dt, double time, double *hrr, double *
     (void)time;
     if (!c) { set_error("fcfd_solve_source: invalid argument"); return FCFD_E_ARG; }
     if (c->o.do_reaction != 1) return FCFD_OK;
-    if (c->sur && (hrr || hrr_dot)) { set_error("fcfd_solve_source: hrr / hrr_dot are not available in surrogate mode"); return FCFD_E_UNSUPPORTED; }
     HIP_TRY(enqueue_source(c, dt, nullptr, hrr || hrr_dot));
     const size_t nint = size_t(c->nblk) * c->ncell;
     if (hrr) HIP_TRY(hipMemcpyAsync(hrr, c->d_hrr, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
     if (hrr_dot) HIP_TRY(hipMemcpyAsync(hrr_dot, c->d_hrr_dot, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(collect_timing(c));
+    return FCFD_OK;
+}
+
+int fcfd_set_step(fcfd_ctx *c, int strang, int hrr_each_step) {
+    if (!c || (strang != 0 && strang != 1) || (hrr_each_step != 0 && hrr_each_step != 1)) {
+        set_error("fcfd_set_step: strang and hrr_each_step must be 0 or 1");
+        return FCFD_E_ARG;
+    }
+    if (strang && c->o.do_reaction != 1) { set_error("fcfd_set_step: Strang splitting needs do_reaction = 1"); return FCFD_E_ARG; }
+    if (strang && !c->d_dth) HIP_TRY(hipMalloc(&c->d_dth, sizeof(double)));
+    drop_graphs(c);
+    c->strang = strang;
+    c->step_hrr = hrr_each_step;
+    return FCFD_OK;
+}
+
+int fcfd_get_hrr(fcfd_ctx *c, double *hrr, double *hrr_dot) {
+    if (!c || !c->d_hrr) { set_error("fcfd_get_hrr: no chemistry on this context"); return FCFD_E_ARG; }
+    const size_t nint = size_t(c->nblk) * c->ncell;
+    if (hrr) HIP_TRY(hipMemcpyAsync(hrr, c->d_hrr, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
+    if (hrr_dot) HIP_TRY(hipMemcpyAsync(hrr_dot, c->d_hrr_dot, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return FCFD_OK;
 }
 

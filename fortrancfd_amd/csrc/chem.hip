@@ -791,7 +791,7 @@ __global__ void __launch_bounds__(64) k_chem_verify(ChemArgs a, VerifyArgs v) {
     for (int q = blockIdx.x; q < n; q += gridDim.x) {
         const int cell = v.cand_list[q];
         long col = 0;
-        double rho_si, dt, T0, y0 = 0.0;
+        double rho_si, dt, T0, y0 = 0.0, y_raw = 0.0;
         if (a.layout == 0) {
             const int b = cell / a.ncell, i = cell % a.ncell;
             col = long(b) * a.S + a.lv + i;
@@ -799,6 +799,7 @@ __global__ void __launch_bounds__(64) k_chem_verify(ChemArgs a, VerifyArgs v) {
             T0 = a.qf[2 * a.P + col];
             dt = a.dt_dev ? *a.dt_dev : a.dt;
             if (lane < NSP) y0 = a.qf[(3 + lane) * a.P + col];
+            y_raw = y0;
             y0 = adjusted(y0, lane, NSP);                       // adjust_sum, as the integrator's load
         } else {
             rho_si = a.b_rho[cell];
@@ -811,6 +812,14 @@ __global__ void __launch_bounds__(64) k_chem_verify(ChemArgs a, VerifyArgs v) {
         const double T1 = bcast(y1, NSP);
         const double f0 = rhs_call<NSP, 1>(glb(th), glb(K), lds(sh), rho, y0, T0, false, 0).f;
         const double f1 = rhs_call<NSP, 1>(glb(th), glb(K), lds(sh), rho, y1, T1, false, 0).f;
+        // heat-release diagnostics of an accepted candidate (SRC/thermo.f90:98-101, 128-132): HRR from
+        // the raw composition before the call and the candidate; HRR_dot = -sum W_k h_k wdot_k at the
+        // candidate, from the production rates just evaluated there (wdot_k W_k = f_k rho)
+        double s1 = 0.0, s2 = 0.0;
+        if (a.hrr) {
+            s1 = wsum(lane < NSP ? rho * th->h298[lane] * (y1 - y_raw) / dt : 0.0);
+            s2 = wsum(lane < NSP ? th->h298[lane] * f1 * rho : 0.0);
+        }
         double r = 0.0;
         if (lane < NSP)
             r = (y1 - y0 - (dt * 0.5) * (f0 + f1)) * (1.0 / (a.atol + a.rtol * fmax(fabs(y0), fabs(y1))));
@@ -821,6 +830,10 @@ __global__ void __launch_bounds__(64) k_chem_verify(ChemArgs a, VerifyArgs v) {
             if (a.layout == 0) {
                 if (lane < NSP) a.qf[(3 + lane) * a.P + col] = y1;
                 if (lane == 0) a.qf[2 * a.P + col] = T1;
+                if (a.hrr && lane == 0) {
+                    a.hrr[cell] = -s1 * 1.e-1;
+                    a.hrr_dot[cell] = -s2 * 1.e-1;
+                }
             } else {
                 if (lane < NSP) a.b_y[long(lane) * a.ncells + cell] = y1;
                 if (lane == 0) a.b_T[cell] = T1;
@@ -882,6 +895,10 @@ __global__ void k_zero_counter(unsigned int *c) {
     if (threadIdx.x == 0) *c = 0u;
 }
 
+__global__ void k_scale_scalar(const double *in, double *out, double f) {
+    if (threadIdx.x == 0) *out = f * *in;
+}
+
 // Stable partition of a work list by the cells' attempt counts (steps + rejects) of the previous
 // integration: first those with at least CH_HEAVY, then the rest.  A launch lasts as long as its
 // slowest cell, and the cells that were slow in the last step are, near a flame, the slow ones in
@@ -924,6 +941,11 @@ __global__ void __launch_bounds__(1024) k_heavy_first(const int *list, const uns
 }
 
 }  // namespace
+
+hipError_t launch_scale_scalar(const double *in, double *out, double f, hipStream_t st) {
+    hipLaunchKernelGGL(k_scale_scalar, dim3(1), dim3(64), 0, st, in, out, f);
+    return hipGetLastError();
+}
 
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, p);

@@ -85,6 +85,8 @@ hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st);
 // Length up to which a stiff list runs four waves per cell in launch_chem.
 inline int chem_coop_max(int n_workgroups) { return 8 * n_workgroups; }
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
+// *out = f * *in on the device (one thread), e.g. the half step of Strang splitting.
+hipError_t launch_scale_scalar(const double *in, double *out, double f, hipStream_t st);
 // CKWYR / CKCDYR (cklib.f:9107-9204, 1422-1541) on n states with the integrator's own rate code:
 // rho [g/cm^3], T [K], Y [n][KK] in; wdot, cdot, ddot [n][KK] (mol/(cm^3 s)) and the rates of
 // progress qf, qr [n][II] (mechanism order) out.  One state per wave.

@@ -235,7 +235,7 @@ class Solver:
 
     def solve_source(self, dt, time=0.0, diagnostics=True):
         """reaction_macks on every interior cell; returns (HRR, HRR_dot) per interior cell when
-        diagnostics is set (not available with a surrogate attached), else None."""
+        diagnostics is set, else None."""
         n = self.nblk * self.ncell
         if not diagnostics:
             _check(lib().fcfd_solve_source(self.handle, C.c_double(dt), C.c_double(time), None,
@@ -244,6 +244,17 @@ class Solver:
         hrr, hrd = np.zeros(n), np.zeros(n)
         _check(lib().fcfd_solve_source(self.handle, C.c_double(dt), C.c_double(time), _p(hrr),
                                        _p(hrd)))
+        return hrr, hrd
+
+    def set_step(self, strang=False, hrr_each_step=False):
+        """Step composition of advance(): Lie (default, the reference's) or Strang splitting, and
+        whether each step's source call computes HRR / HRR_dot (read with get_hrr)."""
+        _check(lib().fcfd_set_step(self.handle, C.c_int(int(strang)), C.c_int(int(hrr_each_step))))
+
+    def get_hrr(self):
+        n = self.nblk * self.ncell
+        hrr, hrd = np.zeros(n), np.zeros(n)
+        _check(lib().fcfd_get_hrr(self.handle, _p(hrr), _p(hrd)))
         return hrr, hrd
 
     def set_surrogate(self, sur):

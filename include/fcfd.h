@@ -107,10 +107,21 @@ int fcfd_refresh(fcfd_ctx *c);
 /* Device-resident stepping for production loops: nsteps x {calc_dt, solve_fluid, solve_source
  * (when do_reaction = 1), refresh} with dt kept on the device and the step replayed from a
  * captured HIP graph.  *time is advanced by the sum of the steps' dt.  Equivalent to the driver
- * loop SRC/main.f90:130-163 without output; the source step computes no HRR / HRR_dot there (the
- * reference computes them on every call for its output, SRC/source.f90:70-73: call
- * fcfd_solve_source with hrr / hrr_dot to get them). */
+ * loop SRC/main.f90:130-163 without output.  fcfd_set_step chooses the splitting (Lie by default)
+ * and whether the source calls compute HRR / HRR_dot (the reference does on every call,
+ * SRC/source.f90:70-73; off by default here). */
 int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time);
+/* Step composition of fcfd_advance (no counterpart in the reference, whose driver splits Lie-style,
+ * SRC/main.f90:142-145).  strang = 0: calc_dt, solve_fluid(dt), solve_source(dt), refresh (the
+ * default, the reference's order and the parity mode); strang = 1: calc_dt, solve_source(dt/2),
+ * refresh, solve_fluid(dt), solve_source(dt/2), refresh (the Strang splitting the reference's
+ * documentation describes, DOCS/README.md:142,181).  hrr_each_step = 1: the last source call of
+ * every step computes HRR / HRR_dot (SRC/source.f90:70-73), read with fcfd_get_hrr; 0: it skips
+ * them (one rate evaluation per cell less). */
+int fcfd_set_step(fcfd_ctx *c, int strang, int hrr_each_step);
+/* HRR and HRR_dot of the last source call that computed them (one value per interior cell, blocks
+ * in order; either pointer may be NULL). */
+int fcfd_get_hrr(fcfd_ctx *c, double *hrr, double *hrr_dot);
 
 /* Kernel timing with HIP events on the context's stream (0 = off; while on, steps are launched
  * one kernel at a time instead of from the captured graph).  Kernel ids: 0 advection flux,
@@ -225,8 +236,9 @@ int fcfd_sur_train_step(fcfd_surrogate *s, const void *X_all, const float *Tg_al
 int fcfd_sur_infer_batch(fcfd_surrogate *s, const fcfd_opts *o, int n, const double *rho,
                          const double *dt, double *T, double *Y, int *route, int *n_fallback,
                          int *stats, void *stream);
-/* solve_source of a context through the surrogate (NULL: the FP64 integrator on every cell).
- * hrr / hrr_dot of fcfd_solve_source are not available in surrogate mode. */
+/* solve_source of a context through the surrogate (NULL: the FP64 integrator on every cell).  HRR /
+ * HRR_dot of cells the surrogate answers are computed from its answer like those of the integrator
+ * (HRR_dot from the production rates at the answer). */
 int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s);
 /* Per interior cell of the last solve_source: 2 explicit step over dt (the screen), 3 two explicit
  * half steps (the retry), 1 surrogate, 0 FP64 implicit integrator; *n_fallback = number of cells

@@ -13,6 +13,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* Transcendental functions of the kinetics, transport and step-size code.  Default build: the
  * library's own fcfd_math.h, the same operations the GPU kernels execute, so the restatement and the
@@ -661,6 +664,7 @@ static void interface_flux(const fo_mech *m, const fo_opts *o, const double *qfc
 void fo_flux_advection(const fo_mech *m, const fo_opts *o, int n, int lv, double dx_block,
                        const double *qf, double *Fplus, double *godunov) {
     const int nequ = 3 + m->kk;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i <= n; ++i)  /* interface ista-1+i, between cells (ista-1+i) and (ista+i) */
         interface_flux(m, o, qf + (size_t)(lv - 1 + i) * nequ, dx_block, Fplus + (size_t)i * nequ,
                        godunov + (size_t)i * 3);
@@ -705,7 +709,7 @@ double fo_mcacon(const fo_mech *m, double T, const double *X) {   /* tranlib.f:8
 void fo_mcadif(const fo_mech *m, double p, double T, const double *X, double *D) {
     const int kk = m->kk, no = m->no;                               /* MCEDIF tranlib.f:1095-1148 */
     double lt = FO_LOG(T), xx[64], wtm = 0.0;
-    static double djk[64 * 64];
+    static _Thread_local double djk[64 * 64];
     for (int k = 1; k < kk; ++k)
         for (int j = 0; j < k; ++j) djk[j + 64 * k] = horner(m->cofd + (k * kk + j) * no, no, lt);
     for (int k = 0; k < kk; ++k) {
@@ -732,6 +736,7 @@ void fo_mcadif(const fo_mech *m, double p, double T, const double *X, double *D)
 
 void fo_flux_tran(const fo_mech *m, int n, int lv, double dx, const double *Qc, double *Fplus) {
     const int kk = m->kk, nequ = 3 + kk;                            /* SRC/flux.f90:18-127 */
+#pragma omp parallel for schedule(static)
     for (int i = 0; i <= n; ++i) {
         const double *qa = Qc + (size_t)(lv - 1 + i) * nequ, *qb = qa + nequ;
         double y[2][64], x[2][64], T[2], p[2], mu[2], kap[2], dm[2][64], dif[2][64], u[2], rho[2];
@@ -809,6 +814,7 @@ static void interpolation_block(fo_domain *d) {                   /* SRC/block.f
 }
 
 static void qf_all(fo_domain *d) {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int ib = 0; ib < d->nblk; ++ib)
         for (int i = -d->lv; i < d->ncell + d->lv; ++i)
             fo_qc_to_qf(d->m, cellp(d, d->Qc, ib, i), cellp(d, d->qf, ib, i));
@@ -870,9 +876,11 @@ void fo_domain_free(fo_domain *d) {
 }
 
 double fo_calc_dt(fo_domain *d, double cfl) {                     /* SRC/update.f90:13-59 */
-    double cvel = 0.0, x[64];
+    double cvel = 0.0;
+#pragma omp parallel for collapse(2) reduction(max : cvel) schedule(static)
     for (int ib = 0; ib < d->nblk; ++ib)
         for (int i = 0; i < d->ncell; ++i) {
+            double x[64];
             const double *q = cellp(d, d->qf, ib, i);
             fo_ckytx(d->m, q + 3, x);
             double p = fo_pressure(d->m, q[0], q[2], x);
@@ -930,6 +938,7 @@ void fo_solve_fluid(fo_domain *d, double dt) {
                 memset(d->Qc, 0, sizeof(double) * d->nblk * (nc + 2 * d->lv) * nequ);
             } else if (st == 2) { h1 = 3.0 / 4.0; h2 = 1.0 / 4.0; }
             else { h1 = 1.0 / 3.0; h2 = 2.0 / 3.0; }
+#pragma omp parallel for collapse(2) schedule(static)
             for (int ib = 0; ib < d->nblk; ++ib)
                 for (int i = 0; i < nc; ++i) {
                     double *Q = cellp(d, d->Qc, ib, i), *Fr = fint(d, ib, i + 1), *Fl = fint(d, ib, i);
@@ -1151,7 +1160,7 @@ void fo_chem_jac(const fo_mech *m, double rho, const double *y, double *J) {
     const int kk = m->kk, n = kk + 1, mxsp = m->mxsp;
     const double T = y[kk];
     double qf[512], qr[512], kf[512], kr[512], c[64], u[64], cp[64], f[65], fp[65], yp[65];
-    static double JC[64 * 64];
+    static _Thread_local double JC[64 * 64];
     rop_k(m, rho, T, y, qf, qr, kf, kr);
     for (int k = 0; k < kk; ++k) c[k] = rho * y[k] / m->wt[k];
     memset(JC, 0, sizeof(JC));
@@ -1266,7 +1275,7 @@ static int chem_step_h(const fo_mech *m, double dt, double rho_si, double *yT, d
     const int kk = m->kk, n = kk + 1;
     const double rho = rho_si * 1.e-3, g = CH_GAMMA;
     double y[65], w[65], f[65], z[65], base[65], k[3][65], r[65], e[65], ya[64];
-    static double J[65 * 65], M[65 * 65];
+    static _Thread_local double J[65 * 65], M[65 * 65];
     int perm[65], rank[65];
     double rdiag[65];
     fo_chem_stats s = {0, 0, 0, 0, 0};
@@ -1407,6 +1416,7 @@ void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *
     const fo_mech *m = d->m;
     const int kk = m->kk;
     int fails = 0;
+#pragma omp parallel for collapse(2) reduction(+ : fails) schedule(dynamic, 16)
     for (int ib = 0; ib < d->nblk; ++ib)
         for (int i = 0; i < d->ncell; ++i) {
             double *q = cellp(d, d->qf, ib, i);
@@ -1426,9 +1436,42 @@ void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *
             if (hrr) hrr[c] = fo_heat_release_rate(m, y0, q + 3, q[0], dt);
             if (hrr_dot) hrr_dot[c] = fo_heat_release_rate_omegadot(m, q + 3, q[0], q[2]);
         }
+#pragma omp parallel for collapse(2) schedule(static)
     for (int ib = 0; ib < d->nblk; ++ib)
         for (int i = 0; i < d->ncell; ++i) fo_qf_to_qc(m, cellp(d, d->qf, ib, i), cellp(d, d->Qc, ib, i));
     if (nfail) *nfail = fails;
+}
+
+/* One time step of the driver loop (SRC/main.f90:130-163) with the chemistry split either the way
+ * the reference driver does it (strang = 0, Lie: calc_dt, solve_fluid(dt), solve_source(dt),
+ * refresh) or symmetrically (strang = 1: calc_dt, solve_source(dt/2), refresh, solve_fluid(dt),
+ * solve_source(dt/2), refresh; the refresh after the first half step fills the ghost cells the fluid
+ * stage reads).  react = 0: calc_dt, solve_fluid.  hrr / hrr_dot (may be NULL) receive the
+ * diagnostics of the last source call.  Returns dt. */
+double fo_step(fo_domain *d, double cfl, int react, int strang, double rtol, double atol, double *hrr,
+               double *hrr_dot, int *nfail) {
+    const double dt = fo_calc_dt(d, cfl);
+    int nf = 0, nf2 = 0;
+    if (react && strang) {
+        fo_solve_source(d, dt * 0.5, rtol, atol, NULL, NULL, &nf);
+        fo_refresh(d);
+    }
+    fo_solve_fluid(d, dt);
+    if (react) {
+        fo_solve_source(d, strang ? dt * 0.5 : dt, rtol, atol, hrr, hrr_dot, &nf2);
+        fo_refresh(d);
+    }
+    if (nfail) *nfail = nf + nf2;
+    return dt;
+}
+
+/* Threads the parallel loops of this library use (OpenMP; 1 without it). */
+int fo_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* The shared transcendental functions on n arguments (which: 0 exp, 1 log, 2 log10, 3 pow(x, y)),

@@ -138,6 +138,13 @@ void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *
 /* exp (0), log (1), log10 (2), pow (3) of the build's math on n arguments */
 void fo_math_eval(int which, int n, const double *x, const double *y, double *out);
 
+/* One driver time step (SRC/main.f90:130-163): Lie (strang = 0, the reference's order) or Strang
+ * splitting of the chemistry; returns dt. */
+double fo_step(fo_domain *d, double cfl, int react, int strang, double rtol, double atol, double *hrr,
+               double *hrr_dot, int *nfail);
+/* OpenMP threads of the parallel loops (1 without OpenMP). */
+int fo_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

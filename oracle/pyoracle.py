@@ -35,7 +35,7 @@ def _lib(libm=False):
     lib.fo_mech_load.argtypes = [C.c_char_p, C.c_char_p]
     for name in ["fo_pressure", "fo_density", "fo_specific_energy", "fo_cp", "fo_cv",
                  "fo_sonic_speed", "fo_temperature_con", "fo_heat_release_rate",
-                 "fo_heat_release_rate_omegadot", "fo_mcavis", "fo_mcacon", "fo_calc_dt"]:
+                 "fo_heat_release_rate_omegadot", "fo_mcavis", "fo_mcacon", "fo_calc_dt", "fo_step"]:
         getattr(lib, name).restype = C.c_double
     lib.fo_domain_create.restype = C.c_void_p
     return lib
@@ -328,6 +328,20 @@ class Domain:
         self.orc.lib.fo_solve_source(self.ptr, C.c_double(dt), C.c_double(rtol), C.c_double(atol),
                                      _p(hrr), _p(hrd), C.byref(nf))
         return hrr, hrd, nf.value
+
+    def step(self, cfl, react=True, strang=False, rtol=1e-5, atol=1e-13, diagnostics=False):
+        """One driver step (fo_step): Lie (the reference's order) or Strang splitting; returns
+        dt, or (dt, HRR, HRR_dot) of the last source call with diagnostics."""
+        n = self.nblk * self.ncell
+        hrr, hrd = (np.zeros(n), np.zeros(n)) if diagnostics else (None, None)
+        nf = C.c_int()
+        dt = self.orc.lib.fo_step(self.ptr, C.c_double(cfl), C.c_int(int(react)), C.c_int(int(strang)),
+                                  C.c_double(rtol), C.c_double(atol),
+                                  _p(hrr) if diagnostics else None, _p(hrd) if diagnostics else None,
+                                  C.byref(nf))
+        if nf.value:
+            raise RuntimeError(f"oracle: {nf.value} cells failed to integrate")
+        return (dt, hrr, hrd) if diagnostics else dt
 
     def chem_stats(self):
         """Per interior cell (block-major): steps, rejects, newton, failed of the last
