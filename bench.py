@@ -6,11 +6,13 @@
 Prints one JSON line.  The default workload is the coupled reacting run: 65536 cells in 16 blocks
 of 4096 (dx = 25 um), stoichiometric CH4/air at 1 atm and 300 K with three seeded 32-cell hot spots
 at 1800-2200 K, GRI-3.0 (53 species, 325 reactions), WENO5-JS + HLLC + SSP-RK3 at CFL 0.5 between
-reflective walls, chemistry split after each fluid step (calc_dt, solve_fluid, solve_source,
-refresh, replayed from a HIP graph).  K steps are timed with the chemistry through the trained
-surrogate + FP64 fallback (`value`), then K steps from the same initial state with the plain FP64
-integrator; the line reports both rates, their ratio and the final-state errors of the surrogate run
-against the FP64 run.  Synthetic data, FP64 state throughout (BF16 / FP32 inside the network).
+reflective walls, Strang-split chemistry (calc_dt, solve_source(dt/2), refresh, solve_fluid(dt),
+solve_source(dt/2) with the HRR / HRR_dot diagnostics, refresh; replayed from a HIP graph).  K steps
+are timed with the chemistry through the trained surrogate + FP64 fallback (`value`), then K steps
+from the same initial state with the plain FP64 integrator; the line reports both rates, their ratio
+and the final-state errors of the surrogate run against the FP64 run after all steps.  Synthetic
+data, FP64 state throughout (BF16 / FP32 inside the network).  --lie times the reference driver's
+own splitting (fluid, then a full chemistry step) instead.
 """
 from __future__ import annotations
 
@@ -199,22 +201,18 @@ def _roofline_for(kern, mech, ntot, nblk, ncell, screen_cells, sdirk_work=None):
             "algorithmic_flops_per_launch": f}
 
 
-def oracle_coupled_baseline(cells, nblk, ncell, dx, cfl, steps):
-    """The CPU restatement (oracle, gcc -O2, one core) on `steps` coupled steps of the same domain:
-    calc_dt, solve_fluid, solve_source (no heat-release diagnostics), refresh."""
-    import ctypes as C
+def oracle_coupled_baseline(cells, nblk, ncell, dx, cfl, steps, strang):
+    """The CPU restatement (oracle, gcc -O2, OpenMP over interfaces and cells on the host's cores) on
+    `steps` coupled steps of the same domain, split like the timed GPU steps, with the heat-release
+    diagnostics.  Returns (seconds, threads)."""
     from oracle.pyoracle import Domain, Oracle
     orc = Oracle()
     d = Domain(orc, cells, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2,
                select_time=1)
     t0 = time.perf_counter()
     for _ in range(steps):
-        dt = d.calc_dt(cfl)
-        d.solve_fluid(dt)
-        orc.lib.fo_solve_source(d.ptr, C.c_double(dt), C.c_double(1e-5), C.c_double(1e-13), None, None,
-                                None)
-        d.refresh()
-    return time.perf_counter() - t0
+        d.step(cfl, react=True, strang=strang, diagnostics=True)
+    return time.perf_counter() - t0, int(orc.lib.fo_threads())
 
 
 def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
@@ -229,9 +227,12 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
     # the acceptance test alone (training range, element residual, energy-consistent T)
     sur_u = load_surrogate(mech, verify_tol=1e30)
 
+    strang = not args.lie
+
     def run(surrogate):
         s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1,
                    do_reaction=1)
+        s.set_step(strang=strang, hrr_each_step=True)
         s.init_cells(cells)
         if surrogate is not None:
             s.set_surrogate(surrogate)
@@ -296,12 +297,13 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
 
     cpu = None
     if not args.no_cpu:
-        secs = oracle_coupled_baseline(cells, nblk, ncell, dx, cfl, args.cpu_steps)
-        cpu = {"value": ntot * args.cpu_steps / secs, "unit": "cell-steps/s", "cores": 1,
+        secs, threads = oracle_coupled_baseline(cells, nblk, ncell, dx, cfl, args.cpu_steps, strang)
+        cpu = {"value": ntot * args.cpu_steps / secs, "unit": "cell-steps/s", "cores": threads,
                "kind": "port",
-               "sample": f"{args.cpu_steps} coupled step(s) of the same {ntot}-cell domain by the C "
-                         f"restatement (gcc -O2, one core; the reference ships no chemistry "
-                         f"integrator): {secs:.2f} s"}
+               "sample": f"the first {args.cpu_steps} coupled steps of the same {ntot}-cell domain, "
+                         f"{'Strang' if strang else 'Lie'}-split with HRR, by the C restatement (gcc "
+                         f"-O2, OpenMP, {threads} threads; the host reports {os.cpu_count()} logical "
+                         f"CPUs; the reference ships no chemistry integrator): {secs:.2f} s"}
     value = ntot * args.steps / el_sur
     fp64_value = ntot * args.steps / el_fp
     return {
@@ -331,7 +333,9 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
         "sim_time_s": t_sur,
         "config": {"workload": f"coupled65536: {nblk} blocks x {ncell} cells, dx {dx} m, GRI-3.0 "
                                f"CH4/air 1 atm 300 K, {nhot} hot spots 1800-2200 K, WENO5-JS+HLLC+"
-                               f"SSP-RK3, CFL {cfl}, walls, split chemistry (atol 1e-13, rtol 1e-5)",
+                               f"SSP-RK3 as a HIP graph, CFL {cfl}, walls, "
+                               f"{'Strang' if strang else 'Lie'}-split chemistry with HRR/HRR_dot "
+                               f"every step (atol 1e-13, rtol 1e-5)",
                    "cells": ntot, "species": mech.kk, "reactions": mech.ii,
                    "surrogate": "56-4x256 GELU-54 BF16/FP32 MLP, "
                                 f"{sur.meta.get('pairs', '?')} training pairs"},
@@ -459,7 +463,9 @@ def main():
     ap.add_argument("--workload", default="coupled65536",
                     choices=sorted(WORKLOADS) + ["chem1m", "surrogate1m"])
     ap.add_argument("--cells", type=int, default=1 << 20)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--lie", action="store_true",
+                    help="coupled workload: the reference driver's Lie splitting instead of Strang")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="inert workloads: one device copy of this many MiB before the steps")

@@ -557,8 +557,9 @@ int fcfd_upload(fcfd_ctx *c, int ib, const double *Qc, const double *qf) {
     auto put = [&](const double *src, double *dst) -> int {
         for (int v = 0; v < ne; ++v)
             for (int j = 0; j < S; ++j) h[size_t(v) * S + j] = src[size_t(j) * ne + v];
-        HIP_TRY(hipMemcpy2DAsync(dst + long(ib) * S, sizeof(double) * c->P, h.data(), sizeof(double) * S,
-                                 sizeof(double) * S, ne, hipMemcpyHostToDevice, c->stream));
+        for (int v = 0; v < ne; ++v)
+            HIP_TRY(hipMemcpyAsync(dst + long(v) * c->P + long(ib) * S, h.data() + size_t(v) * S, sizeof(double) * S,
+                                   hipMemcpyHostToDevice, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         return 0;
     };
@@ -580,8 +581,10 @@ int fcfd_download(fcfd_ctx *c, int ib, double *Qc, double *qf) {
     const int S = c->S, ne = c->nequ;
     std::vector<double> h(size_t(ne) * S);
     auto get = [&](const double *src, double *dst) -> int {
-        HIP_TRY(hipMemcpy2DAsync(h.data(), sizeof(double) * S, src + long(ib) * S, sizeof(double) * c->P,
-                                 sizeof(double) * S, ne, hipMemcpyDeviceToHost, c->stream));
+        // one contiguous copy per variable (block ib's S columns of it)
+        for (int v = 0; v < ne; ++v)
+            HIP_TRY(hipMemcpyAsync(h.data() + size_t(v) * S, src + long(v) * c->P + long(ib) * S, sizeof(double) * S,
+                                   hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         for (int v = 0; v < ne; ++v)
             for (int j = 0; j < S; ++j) dst[size_t(j) * ne + v] = h[size_t(v) * S + j];

@@ -5,6 +5,9 @@
 
 namespace fcfd {
 
+constexpr int LANE_NW = 4;       // waves per workgroup of the lane-per-cell kernels
+constexpr int LANE_CHUNK = 8;    // reaction rows per chunk of their scatter (two per wave)
+
 // Reaction data in the form CKRATT / CKRATX / CKWYR consume it (cklib.f:6630-6993, 9107-9204),
 // re-indexed 0-based with one row per reaction.
 struct DevKin {
@@ -21,7 +24,15 @@ struct DevKin {
     int ifop[MAXR], kfal[MAXR];         // fall-off form (1 Lindemann, 2 SRI, 3/4 Troe), [M] species
     double fpar[MAXR][8];
     int orig[MAXR];                     // the mechanism's (0-based) index of each device row
+    // scatter lists of the lane-per-cell kernels (chem_lane.hip): rows are taken in chunks of
+    // LANE_CHUNK; within chunk c, wave w adds the entries lane_ent[lane_ent0[c * LANE_NW + w] ..
+    // lane_ent0[c * LANE_NW + w + 1]) to the production rates of the species it owns (k mod LANE_NW
+    // = w), in row then slot order.  Entry: species k (bits 0-7), row in the chunk (bits 8-15),
+    // stoichiometric coefficient (bits 16-31, signed).
+    int lane_ent0[(MAXR + LANE_CHUNK - 1) / LANE_CHUNK * LANE_NW + 1];
+    int lane_ent[MAXR * 6];
 };
+
 
 void build_dev_kin(const Mech &m, DevKin &k);
 

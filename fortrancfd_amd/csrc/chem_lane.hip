@@ -8,13 +8,17 @@
 // the order of the species sums differs.  k_chem_retry runs the integrator's next explicit attempt
 // (two half steps) the same way over a long work list, before the SDIRK kernel sees it.
 //
-// Mapping: a workgroup of four waves owns 64 cells (lane l of every wave is cell l).  Concentrations
-// and S/R - H/RT of the 64 cells live in LDS as species-major [k][64] blocks; each wave evaluates a
-// quarter of the reactions (i = wave mod 4) for all 64 cells and scatters its rates of progress
-// into its own production-rate block, so no two threads ever add into the same word and the four
-// partial sums are combined in a fixed order.  Species-wise work (thermo, state update, norms) is
-// split over the waves by k = wave mod 4; cell-wise scalars (T, sums) are combined through LDS.
-// Reaction tables are read with a wave-uniform reaction index (scalar loads).
+// Mapping: a workgroup of four waves owns NC = 56 cells (lane l < NC of every wave is cell l; lanes
+// NC..63 repeat cell NC - 1 and store nothing).  Concentrations and S/R - H/RT of the cells live in
+// LDS as species-major [k][NC] blocks.  The reactions are taken in chunks of 16 rows: each wave
+// evaluates four rows of a chunk for all cells and puts their rates of progress in LDS; after a
+// barrier every wave adds the chunk's contributions to the species it owns (k mod 4 = wave) from a
+// precomputed list, in row order, into one shared production-rate block whose rows no two waves
+// write.  Each species' production rate is therefore one sequential sum over the rows, independent
+// of the wave count.  Species-wise work (thermo, state update, norms) is split over the waves by
+// k = wave mod 4; cell-wise scalars (T, sums) are combined through LDS in wave order.  The LDS
+// footprint (80 KB) lets two workgroups share a compute unit: two waves per SIMD.  Reaction tables
+// are read with a wave-uniform row index (scalar loads).
 #include <algorithm>
 
 #include "chem.hpp"
@@ -25,34 +29,38 @@ namespace fcfd {
 
 namespace {
 
-constexpr int NW = 4;          // waves per workgroup
+constexpr int NW = LANE_NW;    // waves per workgroup
+constexpr int NC = 56;         // cells per workgroup
+constexpr int CH = LANE_CHUNK; // reaction rows per chunk, CH / NW per wave
+constexpr int RB = CH / NW;
 
 template <int NSP>
 struct LaneShared {
-    double C[NSP * 64];        // concentrations, mol/cm^3
-    double G[NSP * 64];        // S/R - H/RT; reused for cross-wave sums after the reaction loop
-    double W[NW][NSP * 64];    // per-wave production-rate partial sums; W[0] start doubles as sum slots
+    double C[NSP * NC];        // concentrations, mol/cm^3
+    double G[NSP * NC];        // S/R - H/RT
+    double W[NSP * NC];        // production rates; row k is written by wave k mod NW only
+    double Q[CH * NC];         // rates of progress (forward minus reverse) of the current chunk
+    double S[NW * NC];         // per-wave values of a cross-wave sum
 };
 
 // Species slots of wave w: k = w + NW * j.
 template <int NSP>
 constexpr int slots() { return (NSP + NW - 1) / NW; }
 
-// Sum over the four waves of one value per lane, in wave order; `slot` is a [NW][64] LDS area that
-// no wave uses between the surrounding barriers.
-__device__ __forceinline__ double sum4(double *slot, int w, int lane, double v) {
-    slot[w * 64 + lane] = v;
+// Sum over the four waves of one value per cell, in wave order, through sh.S.
+__device__ __forceinline__ double sum4(double *slot, int w, int cl, double v) {
+    slot[w * NC + cl] = v;
     __syncthreads();
-    const double s = ((slot[lane] + slot[64 + lane]) + slot[128 + lane]) + slot[192 + lane];
+    const double s = ((slot[cl] + slot[NC + cl]) + slot[2 * NC + cl]) + slot[3 * NC + cl];
     __syncthreads();
     return s;
 }
 
 // f = dy/dt at (y + s d, T) for this wave's species slots (f[j] of species w + NW*j) and fT for
-// all waves (SHIFT = false: at y).  rho in g/cm^3.  Slots beyond NSP are ignored.
+// all waves (SHIFT = false: at y).  rho in g/cm^3.  cl: this lane's LDS column.
 template <int NSP, bool SHIFT>
 __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const DevKin *__restrict__ K, LaneShared<NSP> &sh,
-                         int w, int lane, double rho, const double (&y)[slots<NSP>()],
+                         int w, int cl, double rho, const double (&y)[slots<NSP>()],
                          const double (&d)[slots<NSP>()], double s, double T, double (&f)[slots<NSP>()],
                          double &fT) {
     constexpr int NS = slots<NSP>();
@@ -67,7 +75,8 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
         if (k < NSP) {
             const double yj = SHIFT ? y[j] + s * d[j] : y[j];
             const double ck = rho * yj / th->wt[k];
-            sh.C[k * 64 + lane] = ck;
+            sh.C[k * NC + cl] = ck;
+            sh.W[k * NC + cl] = 0.0;
             ctot_w = ctot_w + ck;
             const Nasa a = nasa(th, k, T);
             uk[j] = h_mol(a, p) - p.rut;
@@ -78,28 +87,25 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
             s = s + t2 / 6.0 * a.a2;
             s = s + t3 / 12.0 * a.a3;
             s = s + t4 / 20.0 * a.a4;
-            sh.G[k * 64 + lane] = s + a.a6 - a.a5 / T;
+            sh.G[k * NC + cl] = s + a.a6 - a.a5 / T;
         }
     }
-    double *slot = sh.W[0];
-    const double ctot = sum4(slot, w, lane, ctot_w);
-    const double cvm = sum4(slot, w, lane, cvm_w);
-    double *Wp = sh.W[w] + lane;
-    for (int k = 0; k < NSP; ++k) Wp[k * 64] = 0.0;
-    __syncthreads();
+    const double ctot = sum4(sh.S, w, cl, ctot_w);    // its barrier also publishes C and G
+    const double cvm = sum4(sh.S, w, cl, cvm_w);
     const double pfac = th->pa / (th->ru * T);
-    const double *Cp = sh.C + lane, *Gp = sh.G + lane;
-    // RB reactions per pass: their rates are independent, so their table loads, LDS reads and
-    // transcendental chains overlap; the scatter follows in reaction order.
-    constexpr int RB = 4;
-    for (int i0 = w; i0 < K->ii; i0 += NW * RB) {
+    const double *Cp = sh.C + cl, *Gp = sh.G + cl;
+    double *Wp = sh.W + cl;
+    const double *Qp = sh.Q + cl;
+    for (int c0 = 0, chunk = 0; c0 < K->ii; c0 += CH, ++chunk) {
+        // this wave's RB rows of the chunk: independent rate chains, so their table loads, LDS
+        // reads and transcendental evaluations overlap
+        const int i0 = c0 + w;
         double q[RB];
         if (i0 >= K->nplain0) {
-            // branch-free rows: the RB rate chains are straight-line code and interleave
 #pragma unroll
             for (int u = 0; u < RB; ++u) {
                 const int i = i0 + NW * u;
-                const Rop r = reaction_plain<64>(K, i < K->ii ? i : K->ii - 1, T, lnT, pfac, Cp, Gp);
+                const Rop r = reaction_plain<NC>(K, i < K->ii ? i : K->ii - 1, T, lnT, pfac, Cp, Gp);
                 q[u] = i < K->ii ? r.qf - r.qr : 0.0;
             }
         } else {
@@ -108,37 +114,35 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
                 const int i = i0 + NW * u;
                 q[u] = 0.0;
                 if (i < K->ii) {
-                    const Rop r = reaction<64>(K, i, T, lnT, pfac, ctot, Cp, Gp);
+                    const Rop r = reaction<NC>(K, i, T, lnT, pfac, ctot, Cp, Gp);
                     q[u] = r.qf - r.qr;
                 }
             }
         }
 #pragma unroll
-        for (int u = 0; u < RB; ++u) {
-            const int i = i0 + NW * u;
-            if (i < K->ii) {
-                const int *nu = K->nu[i], *nk = K->nk[i];
-#pragma unroll
-                for (int n = 0; n < 6; ++n)
-                    if (nk[n] >= 0) Wp[nk[n] * 64] = Wp[nk[n] * 64] + q[u] * (double)nu[n];
-            }
+        for (int u = 0; u < RB; ++u) sh.Q[(w + NW * u) * NC + cl] = q[u];
+        __syncthreads();
+        // the chunk's contributions to this wave's species, in row then slot order
+        const int e1 = K->lane_ent0[chunk * NW + w + 1];
+        for (int e = K->lane_ent0[chunk * NW + w]; e < e1; ++e) {
+            const int pk = K->lane_ent[e];
+            const int k = pk & 0xff, r = (pk >> 8) & 0xff;
+            Wp[k * NC] = Wp[k * NC] + Qp[r * NC] * (double)(pk >> 16);
         }
+        __syncthreads();                   // Q is rewritten by the next chunk
     }
-    __syncthreads();
     double su_w = 0.0;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
         const int k = w + NW * j;
         f[j] = 0.0;
         if (k < NSP) {
-            const int o = k * 64 + lane;
-            const double wd = ((sh.W[0][o] + sh.W[1][o]) + sh.W[2][o]) + sh.W[3][o];
+            const double wd = Wp[k * NC];  // a row this wave wrote
             f[j] = wd * th->wt[k] / rho;
             su_w = su_w + uk[j] * wd;
         }
     }
-    __syncthreads();                       // every wave has read W before G is reused
-    const double su = sum4(sh.G, w, lane, su_w);
+    const double su = sum4(sh.S, w, cl, su_w);
     fT = -su / (rho * cvm);
 }
 
@@ -178,20 +182,24 @@ __device__ __forceinline__ void adjust_lane(double *slot, int w, int lane, doubl
 }
 
 template <int NSP>
-__global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
+__global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
     constexpr int NS = slots<NSP>();
     __shared__ LaneShared<NSP> sh;
     // the wave index is uniform across the wave; saying so turns every table read indexed by it
     // (reactions i = w + 4 n, species k = w + 4 j) into a scalar load and every branch on the
     // reaction kind into a scalar branch
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = min(int(threadIdx.x & 63), NC - 1);   // the LDS column (lanes >= NC repeat NC - 1)
     const DevThermo *__restrict__ th = a.thermo;
     const DevKin *__restrict__ K = a.kin;
-    const int cell0 = blockIdx.x * 64;
+    const int cell0 = blockIdx.x * NC;
     const int cell = cell0 + lane;
+    // every lane of a column takes the column's decisions (lanes >= NC share column NC - 1 and must
+    // write the same LDS values as its owner); only the owner lane stores to global memory
     const bool live = cell < a.ncells;
-    const int c = live ? cell : cell0;          // idle lanes shadow a live cell and store nothing
-    double *slot = sh.W[0];
+    const bool owner = live && int(threadIdx.x & 63) < NC;
+    const int c = live ? cell : cell0;          // columns past the end shadow a live cell
+    double *slot = sh.S;
 
     long col = 0;
     double rho_si, dt, T0;
@@ -250,7 +258,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
     const double err = sqrt(sum4(slot, w, lane, err_w) / double(NSP + 1));
     const bool ok = live && err <= 1.0;
 
-    if (!ok && live && a.f0) {   // f at the start state, for the retry of a long list
+    if (!ok && owner && a.f0) {   // f at the start state, for the retry of a long list
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             const int k = w + NW * j;
@@ -259,7 +267,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
         if (w == 0) a.f0[long(NSP) * a.ncells + cell] = f0T;
     }
     if (!ok) {
-        if (live && w == 0) {
+        if (owner && w == 0) {
             const unsigned long long bal = __ballot(1);
             const int leader = __ffsll((long long)bal) - 1;
             const int rank = __popcll(bal & ((1ull << lane) - 1ull));
@@ -306,7 +314,7 @@ __global__ void __launch_bounds__(256) k_chem_screen(ChemArgs a) {
         hrr_s = sum4(slot, w, lane, s1);
         hrr_d = sum4(slot, w, lane, s2);
     }
-    if (!ok) return;
+    if (!ok || !owner) return;
     if (a.layout == 0) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
@@ -348,7 +356,8 @@ template <int NSP>
 __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
     constexpr int NS = slots<NSP>();
     __shared__ LaneShared<NSP> sh;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = min(int(threadIdx.x & 63), NC - 1);   // the LDS column (lanes >= NC repeat NC - 1)
     const DevThermo *__restrict__ th = a.thermo;
     const DevKin *__restrict__ K = a.kin;
     const int n = int(*a.list_n);
@@ -361,12 +370,15 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.retry_done = 1;
-    const int item0 = int(blockIdx.x) * 64;
+    const int item0 = int(blockIdx.x) * NC;
     if (item0 >= n) return;                     // the whole workgroup: no barrier is skipped
     const int item = item0 + lane;
+    // every lane of a column takes the column's decisions (lanes >= NC share column NC - 1); only the
+    // owner lane stores to global memory
     const bool live = item < n;
-    const int cell = a.list[live ? item : item0];   // idle lanes shadow a live cell and store nothing
-    double *slot = sh.W[0];
+    const bool owner = live && int(threadIdx.x & 63) < NC;
+    const int cell = a.list[live ? item : item0];   // columns past the end shadow a live cell
+    double *slot = sh.S;
 
     const int b = cell / a.ncell, i = cell % a.ncell;
     const long col = long(b) * a.S + a.lv + i;
@@ -433,7 +445,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
         if (sub == 0 && !__syncthreads_or(ok ? 1 : 0)) break;
     }
     if (!ok) {
-        if (live && w == 0) {
+        if (owner && w == 0) {
             const unsigned long long bal = __ballot(1);
             const int leader = __ffsll((long long)bal) - 1;
             const int rank = __popcll(bal & ((1ull << lane) - 1ull));
@@ -476,7 +488,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
         hrr_s = sum4(slot, w, lane, s1);
         hrr_d = sum4(slot, w, lane, s2);
     }
-    if (!ok) return;
+    if (!ok || !owner) return;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
         const int k = w + NW * j;
@@ -497,7 +509,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
 }  // namespace
 
 hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st) {
-    const unsigned nb = unsigned((a.ncells + 63) / 64);
+    const unsigned nb = unsigned((a.ncells + NC - 1) / NC);
     if (nb == 0) return hipSuccess;
 #define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_retry<N>), dim3(nb), dim3(256), 0, st, a); return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)
@@ -506,7 +518,7 @@ hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st) {
 }
 
 hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st) {
-    const unsigned nb = unsigned((a.ncells + 63) / 64);
+    const unsigned nb = unsigned((a.ncells + NC - 1) / NC);
     if (nb == 0) return hipSuccess;
 #define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_screen<N>), dim3(nb), dim3(256), 0, st, a); return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)

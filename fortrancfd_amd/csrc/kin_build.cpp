@@ -1,5 +1,6 @@
 // kin_build.cpp -- device kinetics tables from the CHEMKIN-II link-file data (host side).
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -58,6 +59,25 @@ void build_dev_kin(const Mech &m, DevKin &k) {
         k.tb[r] = tb_of[i];
         k.fo[r] = fo_of[i];
     }
+    // scatter lists of the lane-per-cell kernels, over the device rows
+    int ne = 0;
+    const int nchunk = (m.ii + LANE_CHUNK - 1) / LANE_CHUNK;
+    static_assert(sizeof(k.lane_ent0) / sizeof(int) >= (MAXR + LANE_CHUNK - 1) / LANE_CHUNK * LANE_NW + 1,
+                  "lane_ent0 holds one start per chunk and wave");
+    for (int c = 0; c < nchunk; ++c)
+        for (int w = 0; w < LANE_NW; ++w) {
+            k.lane_ent0[c * LANE_NW + w] = ne;
+            for (int r = 0; r < LANE_CHUNK; ++r) {
+                const int row = c * LANE_CHUNK + r;
+                if (row >= m.ii) break;
+                for (int n = 0; n < 6; ++n) {
+                    const int sp = k.nk[row][n];
+                    if (sp >= 0 && sp % LANE_NW == w)
+                        k.lane_ent[ne++] = int(uint32_t(sp) | (uint32_t(r) << 8) | (uint32_t(uint16_t(int16_t(k.nu[row][n]))) << 16));
+                }
+            }
+        }
+    k.lane_ent0[nchunk * LANE_NW] = ne;
     for (int t = 0; t < k.nthb; ++t) {
         k.ntbs[t] = m.ntbs[t];
         for (int l = 0; l < m.mxtb && l < MAXTB; ++l) {
