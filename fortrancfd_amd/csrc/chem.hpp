@@ -16,6 +16,7 @@ struct DevKin {
     double small, exparg;
     double A[MAXR], B[MAXR], E[MAXR];   // PAR(1..3): A, temperature exponent, E/R
     int rev[MAXR];                      // NSPEC > 0: reverse rate from equilibrium
+    int arr0[MAXR];                     // B = E = 0: the rate constant is A (exp(0) = 1)
     int nus[MAXR];                      // sum of the six stoichiometric coefficients
     int nu[MAXR][6], nk[MAXR][6];       // coefficients and 0-based species (-1: empty slot)
     int tb[MAXR], fo[MAXR];             // third-body / fall-off table row, -1 if none

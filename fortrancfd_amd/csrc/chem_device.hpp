@@ -51,15 +51,20 @@ struct Rop {
 template <int STRIDE>
 __device__ __forceinline__ Rop reaction(const DevKin *__restrict__ K, int i, double T, double alogt, double pfac,
                                         double ctot, const double *__restrict__ c, const double *__restrict__ smh) {
-    double kf = K->A[i] * fm_exp(K->B[i] * alogt - K->E[i] / T);
-    double s = 0.0;
-    for (int n = 0; n < 6; ++n) {
-        const int k = K->nk[i][n];
-        if (k >= 0) s = s + (double)K->nu[i][n] * smh[k * STRIDE];
+    // exp(0) = 1 exactly: rows without temperature dependence (B = E = 0) skip the exponential
+    double kf = K->A[i];
+    if (!K->arr0[i]) kf = kf * fm_exp(K->B[i] * alogt - K->E[i] / T);
+    double kr = 0.0;
+    if (K->rev[i]) {   // irreversible rows need no equilibrium constant
+        double s = 0.0;
+        for (int n = 0; n < 6; ++n) {
+            const int k = K->nk[i][n];
+            if (k >= 0) s = s + (double)K->nu[i][n] * smh[k * STRIDE];
+        }
+        double eqk = s != 0.0 ? fm_exp(fmin(s, K->exparg)) : 1.0;
+        eqk = eqk * ipow(pfac, K->nus[i]);
+        kr = kf / fmax(eqk, K->small);
     }
-    double eqk = s != 0.0 ? fm_exp(fmin(s, K->exparg)) : 1.0;
-    eqk = eqk * ipow(pfac, K->nus[i]);
-    double kr = K->rev[i] ? kf / fmax(eqk, K->small) : 0.0;
     double ctb = 1.0;
     const int t = K->tb[i];
     if (t >= 0) {
@@ -104,7 +109,7 @@ __device__ __forceinline__ Rop reaction(const DevKin *__restrict__ K, int i, dou
                 const double xn = 0.75 - (double)1.27f * fclog;
                 const double cprlog = prlog - ((double)0.4f + (double)0.67f * fclog);
                 const double q = cprlog / (xn - (double)0.14f * cprlog);
-                fc = fm_pow(10.0, fclog / (1.0 + q * q));
+                fc = fm_exp10(fclog / (1.0 + q * q));
             }
             pcor = fc * pcor;
         }
@@ -145,7 +150,10 @@ __device__ __forceinline__ double ipow3(double x, int m) {
 template <int STRIDE>
 __device__ __forceinline__ Rop reaction_plain(const DevKin *__restrict__ K, int i, double T, double alogt, double pfac,
                                               const double *__restrict__ c, const double *__restrict__ smh) {
-    const double kf = K->A[i] * fm_exp(K->B[i] * alogt - K->E[i] / T);
+    // exp(0) = 1 exactly: rows without temperature dependence skip the exponential; irreversible
+    // rows skip the equilibrium constant (with a wave-uniform row index both are scalar branches)
+    double kf = K->A[i];
+    if (!K->arr0[i]) kf = kf * fm_exp(K->B[i] * alogt - K->E[i] / T);
     const int *nu = K->nu[i], *nk = K->nk[i];
     int k[6], a[6];
     double s = 0.0;
@@ -156,7 +164,8 @@ __device__ __forceinline__ Rop reaction_plain(const DevKin *__restrict__ K, int 
         const double t = (double)a[n] * smh[(k[n] >= 0 ? k[n] : 0) * STRIDE];
         s = k[n] >= 0 ? s + t : s;
     }
-    const double ex = fm_exp(fmin(s, K->exparg));
+    double ex = 1.0;
+    if (K->rev[i]) ex = fm_exp(fmin(s, K->exparg));
     double eqk = s != 0.0 ? ex : 1.0;
     const int ns = K->nus[i];
     const double pp = ipow3(pfac, ns < 0 ? -ns : ns);

@@ -21,7 +21,7 @@
  *           TwoSum).  The two-term result is accurate to about 2^-60.
  *   log10(x) = log(x) * (1/ln 10) with both factors as two-term values.
  *   pow(x, y) = exp(y log x), the product y*log x carried as a two-term value into exp's
- *           reduction.
+ *           reduction; exp10(y) the same with a stored two-term ln 10.
  * Special values follow C99 Annex F for the cases the kinetics and transport code meets (NaN in,
  * NaN out; overflow to +inf; underflow to 0; log(0) = -inf; log of a negative number NaN; pow with a
  * zero base or a zero exponent; pow of a negative base with an integer exponent).
@@ -166,6 +166,18 @@ FM_FN double fm_log10(double x) {
     const double ph = hi * FM_INV_LN10_HI;
     const double pl = __builtin_fma(hi, FM_INV_LN10_HI, -ph) + (hi * FM_INV_LN10_LO + lo * FM_INV_LN10_HI);
     return ph + pl;
+}
+
+#define FM_LN10_HI 0x1.26bb1bbb55516p+1   /* ln 10 rounded to double */
+#define FM_LN10_LO -0x1.f48ad494ea3e9p-53 /* ln 10 - FM_LN10_HI */
+
+/* 10^y = exp(y ln 10) with y ln 10 carried as a two-term value (the fall-off blending of CKRATX
+ * raises 10 to a power; a stored ln 10 saves the logarithm pow would take of its base) */
+FM_FN double fm_exp10(double y) {
+    if (y != y) return y + y;
+    const double ph = y * FM_LN10_HI;
+    const double pl = __builtin_fma(y, FM_LN10_HI, -ph) + y * FM_LN10_LO;
+    return fm_exp_dd(ph, pl);
 }
 
 FM_FN double fm_pow(double x, double y) {

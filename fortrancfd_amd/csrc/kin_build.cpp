@@ -49,6 +49,7 @@ void build_dev_kin(const Mech &m, DevKin &k) {
         k.B[r] = m.par[size_t(i) * np1 + 1];
         k.E[r] = m.par[size_t(i) * np1 + 2];
         k.rev[r] = m.ns[i] > 0 ? 1 : 0;
+        k.arr0[r] = (k.B[r] == 0.0 && k.E[r] == 0.0) ? 1 : 0;
         int s = 0;
         for (int n = 0; n < 6; ++n) {
             k.nu[r][n] = m.nu[size_t(i) * 6 + n];

@@ -27,12 +27,14 @@
 #define FO_LOG log
 #define FO_LOG10 log10
 #define FO_POW pow
+#define FO_EXP10(x) pow(10.0, (x))
 #else
 #include "../fortrancfd_amd/csrc/fcfd_math.h"
 #define FO_EXP fm_exp
 #define FO_LOG fm_log
 #define FO_LOG10 fm_log10
 #define FO_POW fm_pow
+#define FO_EXP10 fm_exp10
 #endif
 
 /* ============================================================================================ */
@@ -1047,7 +1049,7 @@ static void rop_k(const fo_mech *m, double rho, double T, const double *Y, doubl
                 double cprlog = prlog - ((double)0.4f + (double)0.67f * fclog);
                 double q = cprlog / (xn - (double)0.14f * cprlog);
                 double flog = fclog / (1.0 + q * q);
-                fc = FO_POW(10.0, flog);
+                fc = FO_EXP10(flog);                                /* 10**flog */
             }
             pcor = fc * pcor;
         }
@@ -1474,7 +1476,7 @@ int fo_threads(void) {
 #endif
 }
 
-/* The shared transcendental functions on n arguments (which: 0 exp, 1 log, 2 log10, 3 pow(x, y)),
+/* The shared transcendental functions on n arguments (which: 0 exp, 1 log, 2 log10, 3 pow(x, y), 4 10^x),
  * for the tests that compare them with the C library and with extended-precision values.  In the
  * FO_LIBM_MATH build these are the C library's functions. */
 void fo_math_eval(int which, int n, const double *x, const double *y, double *out) {
@@ -1483,6 +1485,7 @@ void fo_math_eval(int which, int n, const double *x, const double *y, double *ou
         case 0: out[i] = FO_EXP(x[i]); break;
         case 1: out[i] = FO_LOG(x[i]); break;
         case 2: out[i] = FO_LOG10(x[i]); break;
+        case 4: out[i] = FO_EXP10(x[i]); break;
         default: out[i] = FO_POW(x[i], y[i]); break;
         }
     }

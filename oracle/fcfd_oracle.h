@@ -135,7 +135,7 @@ int fo_chem_step(const fo_mech *m, double dt, double rho, double *yT, double rto
 void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *hrr,
                      double *hrr_dot, int *nfail);
 
-/* exp (0), log (1), log10 (2), pow (3) of the build's math on n arguments */
+/* exp (0), log (1), log10 (2), pow (3), 10^x (4) of the build's math on n arguments */
 void fo_math_eval(int which, int n, const double *x, const double *y, double *out);
 
 /* One driver time step (SRC/main.f90:130-163): Lie (strang = 0, the reference's order) or Strang
