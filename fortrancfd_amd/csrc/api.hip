@@ -429,6 +429,66 @@ int fcfd_mech_load(const char *chem_bin, const char *tran_bin, fcfd_mech **out) 
     return FCFD_OK;
 }
 
+int fcfd_mech_load_text(const char *chem_inp, const char *therm_dat, const char *tran_bin, fcfd_mech **out) {
+    if (!out) { set_error("out is NULL"); return FCFD_E_ARG; }
+    *out = nullptr;
+    auto m = std::make_unique<fcfd_mech>();
+    std::string err;
+    int rc = load_mech_text(chem_inp, therm_dat, m->host, err);
+    if (rc == 0 && tran_bin) rc = load_tran_bin(tran_bin, m->host, err);
+    if (rc != 0) { set_error(err); return rc; }
+    *out = m.release();
+    return FCFD_OK;
+}
+
+int fcfd_mech_thermo(const fcfd_mech *m, int k, double *temps, double *coef) {
+    if (!m || k < 0 || k >= m->host.kk || !temps || !coef) { set_error("fcfd_mech_thermo: invalid argument"); return FCFD_E_ARG; }
+    for (int c = 0; c < 3; ++c) temps[c] = m->host.tt[size_t(k) * 3 + c];
+    for (int c = 0; c < 14; ++c) coef[c] = m->host.aa[size_t(k) * 14 + c];
+    return FCFD_OK;
+}
+
+int fcfd_mech_reaction(const fcfd_mech *m, int i, double *par, int *nu, int *nk, int *ns, int *nr) {
+    if (!m || i < 0 || i >= m->host.ii) { set_error("fcfd_mech_reaction: invalid argument"); return FCFD_E_ARG; }
+    const Mech &h = m->host;
+    if (par) for (int c = 0; c < 3; ++c) par[c] = h.par[size_t(i) * (h.npar + 1) + c];
+    for (int q = 0; q < 6; ++q) {
+        if (nu) nu[q] = h.nu[size_t(i) * 6 + q];
+        if (nk) nk[q] = h.nk[size_t(i) * 6 + q];
+    }
+    if (ns) *ns = h.ns[i];
+    if (nr) *nr = h.nr[i];
+    return FCFD_OK;
+}
+
+int fcfd_mech_falloff(const fcfd_mech *m, int i, int *ifo, int *kfl, double *fpar) {
+    if (!m || i < 0 || i >= m->host.ii || !ifo) { set_error("fcfd_mech_falloff: invalid argument"); return FCFD_E_ARG; }
+    const Mech &h = m->host;
+    *ifo = 0;
+    for (size_t f = 0; f < h.ifl.size(); ++f)
+        if (h.ifl[f] == i + 1) {
+            *ifo = h.ifo[f];
+            if (kfl) *kfl = h.kfl[f];
+            if (fpar) for (int c = 0; c < 8 && c < h.nfar; ++c) fpar[c] = h.fpar[f * h.nfar + c];
+        }
+    return FCFD_OK;
+}
+
+int fcfd_mech_thirdbody(const fcfd_mech *m, int i, int *ntbs, int *nktb, double *aik) {
+    if (!m || i < 0 || i >= m->host.ii || !ntbs) { set_error("fcfd_mech_thirdbody: invalid argument"); return FCFD_E_ARG; }
+    const Mech &h = m->host;
+    *ntbs = -1;
+    for (size_t t = 0; t < h.itb.size(); ++t)
+        if (h.itb[t] == i + 1) {
+            *ntbs = h.ntbs[t];
+            for (int l = 0; l < h.ntbs[t]; ++l) {
+                if (nktb) nktb[l] = h.nktb[t * h.mxtb + l];
+                if (aik) aik[l] = h.aik[t * h.mxtb + l];
+            }
+        }
+    return FCFD_OK;
+}
+
 void fcfd_mech_free(fcfd_mech *m) {
     if (!m) return;
     (void)hipFree(m->d_thermo);

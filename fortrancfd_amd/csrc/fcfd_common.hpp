@@ -10,12 +10,13 @@
 namespace fcfd {
 
 // Species capacity of the device tables.  Kernels are instantiated per species count
-// (FCFD_FOR_EACH_NSP) so that per-lane species arrays stay in registers.
+// (FCFD_FOR_EACH_NSP: the 53 of GRI-3.0 and the 11 of the Li-Dryer H2/O2 mechanism) so that per-lane
+// species arrays stay in registers.
 constexpr int MAXK = 64;
 constexpr int MAXR = 512;   // reactions
 constexpr int MAXTB = 16;   // enhanced third bodies per reaction
 
-#define FCFD_FOR_EACH_NSP(X) X(53)
+#define FCFD_FOR_EACH_NSP(X) X(53) X(11)
 
 // NASA-7 thermo tables in the order CKHML / CKCPML / CKSMH consume them (cklib.f:3537-3556,
 // 1849-1865, 7540-7560): a[0..4] polynomial, a[5] enthalpy constant, a[6] entropy constant.
@@ -54,6 +55,10 @@ struct Mech {
 };
 
 int load_mech(const char *chem_bin, const char *tran_bin, Mech &m, std::string &err);
+// The same tables from the mechanism text (chem.inp, optionally therm.dat), mech_text.cpp.
+int load_mech_text(const char *chem_inp, const char *therm_dat, Mech &m, std::string &err);
+// TRANLIB fits of a tran.bin link file for the species of m (mech_io.cpp).
+int load_tran_bin(const char *tran_bin, Mech &m, std::string &err);
 void thermo_h298(DevThermo &t);
 
 void set_error(const std::string &msg);

@@ -137,8 +137,12 @@ int load_mech(const char *chem_bin, const char *tran_bin, Mech &m, std::string &
     }
 
     thermo_h298(d);
+    return tran_bin ? load_tran_bin(tran_bin, m, err) : 0;
+}
 
-    if (tran_bin) {
+int load_tran_bin(const char *tran_bin, Mech &m, std::string &err) {
+    const int kk = m.kk;
+    {
         RecordReader t;
         if (!t.open(tran_bin)) { err = std::string("cannot read ") + tran_bin; return -2; }
         if (!t.next()) { err = "tran.bin: missing header"; return -2; }

@@ -14,6 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "_lib", "libfcfd_hip.so")
 DATA_DIR = os.path.join(PKG, "data")
 GRI30 = os.path.join(DATA_DIR, "GRI3.0_CH4")
+LIDRYER_H2 = os.path.join(DATA_DIR, "LiDryer_H2")
 
 _D = C.POINTER(C.c_double)
 _lib = None
@@ -73,12 +74,20 @@ def _f64(a):
 class Mechanism:
     """CHEMKIN-II link files (chem.bin, tran.bin) as read by CKINIT / MCINIT."""
 
-    def __init__(self, mech_dir: str = GRI30, transport: bool = True):
+    def __init__(self, mech_dir: str = GRI30, transport: bool = True, text: bool = False):
+        """text=False: the link file chem.bin (CKINIT's input); text=True: the CHEMKIN text
+        chem.inp + therm.dat read directly.  Transport from tran.bin either way."""
         L = lib()
         self.handle = C.c_void_p()
-        chem = os.path.join(mech_dir, "chem.bin").encode()
         tran = os.path.join(mech_dir, "tran.bin").encode() if transport else None
-        _check(L.fcfd_mech_load(chem, tran, C.byref(self.handle)))
+        if text:
+            therm = os.path.join(mech_dir, "therm.dat")
+            _check(L.fcfd_mech_load_text(os.path.join(mech_dir, "chem.inp").encode(),
+                                         therm.encode() if os.path.exists(therm) else None, tran,
+                                         C.byref(self.handle)))
+        else:
+            chem = os.path.join(mech_dir, "chem.bin").encode()
+            _check(L.fcfd_mech_load(chem, tran, C.byref(self.handle)))
         kk, ii = C.c_int(), C.c_int()
         _check(L.fcfd_mech_dims(self.handle, C.byref(kk), C.byref(ii)))
         self.kk, self.ii, self.nequ = kk.value, ii.value, 3 + kk.value
@@ -102,6 +111,34 @@ class Mechanism:
             lib().fcfd_mech_free(self.handle)
         except Exception:
             pass
+
+    # ---- tables (host) -------------------------------------------------------------------
+    def thermo(self, k):
+        t, a = np.zeros(3), np.zeros(14)
+        _check(lib().fcfd_mech_thermo(self.handle, C.c_int(k), _p(t), _p(a)))
+        return t, a
+
+    def reaction(self, i):
+        par = np.zeros(3)
+        nu, nk = np.zeros(6, np.int32), np.zeros(6, np.int32)
+        ns, nr = C.c_int(), C.c_int()
+        P = C.POINTER(C.c_int)
+        _check(lib().fcfd_mech_reaction(self.handle, C.c_int(i), _p(par), nu.ctypes.data_as(P),
+                                        nk.ctypes.data_as(P), C.byref(ns), C.byref(nr)))
+        return par, nu, nk, ns.value, nr.value
+
+    def falloff(self, i):
+        ifo, kfl = C.c_int(), C.c_int()
+        fp = np.zeros(8)
+        _check(lib().fcfd_mech_falloff(self.handle, C.c_int(i), C.byref(ifo), C.byref(kfl), _p(fp)))
+        return ifo.value, kfl.value, fp
+
+    def thirdbody(self, i):
+        n = C.c_int()
+        nk, a = np.zeros(16, np.int32), np.zeros(16)
+        _check(lib().fcfd_mech_thirdbody(self.handle, C.c_int(i), C.byref(n),
+                                         nk.ctypes.data_as(C.POINTER(C.c_int)), _p(a)))
+        return n.value, nk[:max(n.value, 0)], a[:max(n.value, 0)]
 
     # ---- stateless kernels on host arrays ----------------------------------------------------
     def calc_flux_advection(self, qf, n, lv, dx, **kw):

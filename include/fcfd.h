@@ -53,6 +53,23 @@ typedef struct fcfd_opts {
  * (SRC/thermo.f90:43-52 -> CKINIT cklib.f:3972; SRC/trans.f90:35-42 -> MCINIT tranlib.f:93).
  * tran_bin may be NULL when transport is not used. */
 int fcfd_mech_load(const char *chem_bin, const char *tran_bin, fcfd_mech **out);
+/* The same mechanism from its CHEMKIN-II text instead of the link file: chem.inp (ELEMENTS,
+ * SPECIES, optional THERMO, REACTIONS with LOW / TROE / SRI / DUPLICATE / third-body
+ * efficiencies), therm.dat (may be NULL when chem.inp holds all thermo), tran.bin (may be NULL).
+ * Replaces the interpreter step ckinterp.f -> chem.bin of WORK/KineticModels/Chem_bin/GRI3.0_CH4/
+ * comp.sh:3-4 and gives the arrays CKINIT (cklib.f:3972) would read from its output. */
+int fcfd_mech_load_text(const char *chem_inp, const char *therm_dat, const char *tran_bin,
+                        fcfd_mech **out);
+/* Mechanism tables, per species k / reaction i (0-based), as CKINIT holds them: NASA-7 temperatures
+ * (Tlow, Tmid, Thigh) and coefficients (7 low-range, then 7 high-range); Arrhenius A, b, E/R,
+ * coefficients nu[6] and 1-based species nk[6] (reactants in slots 0-2), NSPEC (negative when
+ * irreversible) and NREAC; fall-off form (0: none, 1 Lindemann, 2 SRI, 3/4 Troe), its [M] species
+ * (1-based, 0 for +M) and 8 parameters; third-body efficiency count (-1: no third body), species
+ * and values (at most 10). */
+int fcfd_mech_thermo(const fcfd_mech *m, int k, double *temps, double *coef);
+int fcfd_mech_reaction(const fcfd_mech *m, int i, double *par, int *nu, int *nk, int *ns, int *nr);
+int fcfd_mech_falloff(const fcfd_mech *m, int i, int *ifo, int *kfl, double *fpar);
+int fcfd_mech_thirdbody(const fcfd_mech *m, int i, int *ntbs, int *nktb, double *aik);
 void fcfd_mech_free(fcfd_mech *m);
 int fcfd_mech_dims(const fcfd_mech *m, int *kk, int *ii);
 /* Species k (0-based): symbol (NUL-terminated, at most len-1 chars), molar mass g/mol and element
