@@ -20,7 +20,7 @@ LIB = os.path.join(OUT_DIR, "libfcfd_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["api.hip", "fluid.hip", "chem.hip", "chem_lane.hip", "tran.hip", "mlp.hip", "surrogate.hip", "mech_io.cpp", "mech_text.cpp", "kin_build.cpp"]
+SOURCES = ["api.hip", "fluid.hip", "chem.hip", "chem_lane.hip", "tran.hip", "aux.hip", "mlp.hip", "surrogate.hip", "mech_io.cpp", "mech_text.cpp", "kin_build.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall",
           f"-I{os.path.join(REPO, 'include')}", f"-I{CSRC}"]

@@ -12,6 +12,7 @@
 #include "chem.hpp"
 #include "surrogate_impl.hpp"
 #include "tran.hpp"
+#include "aux.hpp"
 
 namespace fcfd {
 
@@ -33,7 +34,7 @@ using namespace fcfd;
 
 namespace {
 
-constexpr int NTIMERS = 9;   // 0 flux, 1 stage end, 2 CFL, 3 chemistry SDIRK, 4 source Qc refresh, 5 surrogate, 6 explicit screen, 7 transport flux, 8 explicit retry
+constexpr int NTIMERS = 10;  // 0 flux, 1 stage end, 2 CFL, 3 chemistry SDIRK, 4 source Qc refresh, 5 surrogate, 6 explicit screen, 7 transport flux, 8 explicit retry, 9 auxiliary sources
 
 int lvir_of(const fcfd_opts &o) {                                  // SRC/params.f90:161-173
     if (o.flag_acu == 0) return 1;
@@ -44,8 +45,10 @@ int lvir_of(const fcfd_opts &o) {                                  // SRC/params
 }
 
 int check_opts(const fcfd_opts &o, int kk) {
-    if (o.is_spherical != 0) { set_error("is_spherical = 1 is not supported (Cartesian only)"); return FCFD_E_UNSUPPORTED; }
-    if (o.flag_lbound != 0 || o.flag_rbound != 0) { set_error("only reflective walls (flag_lbound = flag_rbound = 0) are supported"); return FCFD_E_UNSUPPORTED; }
+    if (o.is_spherical != 0 && o.is_spherical != 1) { set_error("is_spherical must be 0 (Cartesian) or 1 (spherical)"); return FCFD_E_ARG; }
+    if (o.is_spherical == 1 && o.select_time == 2) { set_error("is_spherical = 1 is supported with select_time 0 or 1 (the reference's RK4 has no spherical update)"); return FCFD_E_UNSUPPORTED; }
+    if ((o.flag_lbound != 0 && o.flag_lbound != 1) || (o.flag_rbound != 0 && o.flag_rbound != 1)) { set_error("flag_lbound / flag_rbound must be 0 (wall) or 1 (inflow / outflow)"); return FCFD_E_ARG; }
+    if ((o.flag_lbound == 1 || o.flag_rbound == 1) && o.select_time == 2) { set_error("inflow / outflow boundaries are supported with select_time 0 or 1"); return FCFD_E_UNSUPPORTED; }
     if (o.select_time < 0 || o.select_time > 2) { set_error("select_time must be 0 (Euler), 1 (TVD-RK3) or 2 (RK4)"); return FCFD_E_UNSUPPORTED; }
     if (o.select_riemann < 0 || o.select_riemann > 2) { set_error("select_riemann must be 0, 1 or 2"); return FCFD_E_ARG; }
     if (o.flag_acu < 0 || o.flag_acu > 2) { set_error("flag_acu must be 0, 1 or 2"); return FCFD_E_ARG; }
@@ -110,6 +113,13 @@ struct fcfd_ctx {
     DevTran *d_tran = nullptr;
     double *props = nullptr;
     long PP = 0;
+    // open boundaries and auxiliary sources
+    FlowBC *d_bc = nullptr;              // inflow / outflow state (fcfd_set_flow_boundary)
+    bool bc_set = false;
+    AuxParams aux{};                     // host copy of d_aux
+    AuxParams *d_aux = nullptr;
+    double *d_xm = nullptr;              // [nblk * ncell] cell centres, m
+    double *d_qrad = nullptr, *d_qw = nullptr;
 };
 
 namespace fcfd {
@@ -200,6 +210,7 @@ FluxArgs flux_args(fcfd_ctx *c, bool god) {
     a.thermo = c->thermo; a.qf = c->qf; a.F = c->F; a.godunov = god ? c->god : nullptr;
     a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P; a.PF = c->PF;
     a.muscl = c->o.flag_muscl; a.riemann = c->o.select_riemann;
+    a.spherical = c->o.is_spherical;
     return a;
 }
 
@@ -208,14 +219,25 @@ StageArgs stage_args(fcfd_ctx *c, int mode, const double *Q0, const double *Qin,
     StageArgs a{};
     a.thermo = c->thermo; a.Q0 = Q0; a.Qin = Qin; a.F = c->F; a.Qout = Qout; a.qf = c->qf;
     a.dt_dev = dt_dev; a.dt = dt; a.dx = c->dx; a.h1 = h1; a.h2 = h2; a.mode = mode; a.ghosts_only = 0;
+    a.spherical = c->o.is_spherical; a.xm = c->d_xm; a.god = c->god;
     a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P; a.PF = c->PF;
     return a;
+}
+
+// The inflow / outflow ghost cells over the wall ghosts a stage or refresh has just written.
+hipError_t open_bc(fcfd_ctx *c, double *Q) {
+    if (c->o.flag_lbound != 1 && c->o.flag_rbound != 1) return hipSuccess;
+    OpenBCArgs a{};
+    a.thermo = c->thermo; a.bc = c->d_bc; a.Q = Q; a.qf = c->qf;
+    a.lbound = c->o.flag_lbound; a.rbound = c->o.flag_rbound;
+    a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P;
+    return launch_open_bc(a, c->kk, c->stream);
 }
 
 // One solve_fluid on the stream; dt from the host (dt_dev == nullptr) or from device memory.
 hipError_t enqueue_fluid(fcfd_ctx *c, double dt, const double *dt_dev) {
     const int nsp = c->kk;
-    FluxArgs fa = flux_args(c, false);
+    FluxArgs fa = flux_args(c, c->o.is_spherical == 1);   // the spherical update reads the Roe pressures
     fa.accumulate = c->o.do_tran == 1 ? 1 : 0;
     // Fplus of one stage (SRC/timeintegration.f90:138-146): transport from the conserved state
     // Qin, then advection from the primitives on top
@@ -231,7 +253,12 @@ hipError_t enqueue_fluid(fcfd_ctx *c, double dt, const double *dt_dev) {
         if (c->o.do_hydro != 1) return hipSuccess;
         return timed(c, 0, [&]() { return launch_flux(fa, nsp, c->o.flag_acu, c->o.flag_weno, c->stream); });
     };
-    auto stage = [&](const StageArgs &s) { return timed(c, 1, [&]() { return launch_stage_end(s, nsp, c->stream); }); };
+    auto stage = [&](const StageArgs &s) {
+        return timed(c, 1, [&]() {
+            hipError_t e2 = launch_stage_end(s, nsp, c->stream);
+            return e2 != hipSuccess ? e2 : open_bc(c, s.Qout);
+        });
+    };
     hipError_t e;
     const int a = c->cur, b = (c->cur + 1) % 3, d = (c->cur + 2) % 3;
     if (c->o.select_time == 0) {                                   // explicit_euler
@@ -277,7 +304,23 @@ hipError_t enqueue_dt(fcfd_ctx *c, double cfl, double *time_dev) {
 
 hipError_t enqueue_refresh(fcfd_ctx *c) {
     StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
-    return timed(c, 1, [&]() { return launch_stage_end(s, c->kk, c->stream); });
+    return timed(c, 1, [&]() {
+        hipError_t e = launch_stage_end(s, c->kk, c->stream);
+        return e != hipSuccess ? e : open_bc(c, c->Q[c->cur]);
+    });
+}
+
+bool aux_on(const fcfd_ctx *c) { return c->aux.nrad > 0 || c->aux.wall_heat || c->aux.ig_type != 0; }
+
+// The auxiliary sources of solve_source (SRC/update.f90:127-147) on the interior cells.
+hipError_t enqueue_aux(fcfd_ctx *c, double dt, const double *dt_dev, double time, const double *time_dev) {
+    AuxArgs a{};
+    a.thermo = c->thermo; a.tran = c->d_tran; a.par = c->d_aux; a.Q = c->Q[c->cur]; a.qf = c->qf;
+    a.xm = c->d_xm; a.qrad = c->d_qrad; a.qw = c->d_qw;
+    a.dt_dev = dt_dev; a.time_dev = time_dev; a.dt = dt; a.time = time;
+    a.rad = c->aux.nrad > 0; a.wall = c->aux.wall_heat; a.ig = c->aux.ig_type != 0;
+    a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P;
+    return timed(c, 9, [&]() { return launch_aux(a, c->kk, c->stream); });
 }
 
 // reaction_macks (SRC/source.f90:17-84): chemistry on every interior cell, then Qc = qfToQc(qf).
@@ -363,12 +406,12 @@ hipError_t enqueue_step(fcfd_ctx *c, double cfl) {
         if ((e = enqueue_refresh(c)) != hipSuccess) return e;
     }
     if (fluid && (e = enqueue_fluid(c, 0.0, c->d_dt)) != hipSuccess) return e;
-    if (react) {
-        if ((e = enqueue_source(c, 0.0, c->strang ? c->d_dth : c->d_dt, hrr)) != hipSuccess) return e;
-        // without a source term the refresh would recompute identical primitives (QcToqf of an
-        // unchanged Qc), so it is only enqueued when the chemistry has changed the state
-        if ((e = enqueue_refresh(c)) != hipSuccess) return e;
-    }
+    if (react && (e = enqueue_source(c, 0.0, c->strang ? c->d_dth : c->d_dt, hrr)) != hipSuccess) return e;
+    // the auxiliary sources take the full dt after the chemistry, at the time after the step
+    if (aux_on(c) && (e = enqueue_aux(c, 0.0, c->d_dt, 0.0, c->d_time)) != hipSuccess) return e;
+    // without a source term the refresh would recompute identical primitives (QcToqf of an
+    // unchanged Qc), so it is only enqueued when a source has changed the state
+    if ((react || aux_on(c)) && (e = enqueue_refresh(c)) != hipSuccess) return e;
     return hipSuccess;
 }
 
@@ -550,11 +593,28 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
         HIP_TRY(hipMemset(c->acc, 0, sb));
     }
     HIP_TRY(hipMalloc(&c->god, sizeof(double) * 3 * c->PF));
+    HIP_TRY(hipMemset(c->god, 0, sizeof(double) * 3 * c->PF));
     HIP_TRY(hipMalloc(&c->d_dt, sizeof(double)));
     HIP_TRY(hipMalloc(&c->d_time, sizeof(double)));
     HIP_TRY(hipMalloc(&c->d_cvel, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->d_cvel, 0, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->d_dt, 0, sizeof(double)));
+    {
+        // cell centres of the uniform block layout (SRC/block.f90:264-305 without refinement): a
+        // running sum of dx from 0, shifted by the half cell length_smr = -dx/2
+        std::vector<double> xm(size_t(nblk) * ncell);
+        double xl = 0.0;
+        const double length_smr = 0.0 - dx / 2.0;
+        for (size_t g = 0; g < xm.size(); ++g) {
+            if (g > 0) xl = xl + dx;
+            xm[g] = xl - length_smr;
+        }
+        HIP_TRY(hipMalloc(&c->d_xm, sizeof(double) * xm.size()));
+        HIP_TRY(hipMemcpy(c->d_xm, xm.data(), sizeof(double) * xm.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&c->d_aux, sizeof(AuxParams)));
+        HIP_TRY(hipMemcpy(c->d_aux, &c->aux, sizeof(AuxParams), hipMemcpyHostToDevice));
+        if (o->flag_lbound == 1 || o->flag_rbound == 1) HIP_TRY(hipMalloc(&c->d_bc, sizeof(FlowBC)));
+    }
     if (o->do_tran) {
         if (!m->host.has_tran) { set_error("do_tran = 1 needs the transport link file (tran.bin)"); return FCFD_E_ARG; }
         if (m->host.no > 4) { set_error("transport fits of order above 4 are not supported"); return FCFD_E_UNSUPPORTED; }
@@ -604,6 +664,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->d_accepted); (void)hipFree(c->d_list); (void)hipFree(c->d_order); (void)hipFree(c->d_hmem); (void)hipFree(c->d_list_n);
     (void)hipFree(c->d_list2); (void)hipFree(c->d_list2_n); (void)hipFree(c->d_retry); (void)hipFree(c->d_f0);
     (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc); (void)hipFree(c->d_dth);
+    (void)hipFree(c->d_bc); (void)hipFree(c->d_aux); (void)hipFree(c->d_xm); (void)hipFree(c->d_qrad); (void)hipFree(c->d_qw);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -670,7 +731,12 @@ int fcfd_init_cells(fcfd_ctx *c, const double *cells) {
     // ghost fill + QcToqf on ghosts only (SRC/main.f90:96-114)
     StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
     s.ghosts_only = 1;
+    if ((c->o.flag_lbound == 1 || c->o.flag_rbound == 1) && !c->bc_set) {
+        set_error("fcfd_init_cells: inflow / outflow boundaries need fcfd_set_flow_boundary first");
+        return FCFD_E_ARG;
+    }
     HIP_TRY(launch_stage_end(s, c->kk, c->stream));
+    HIP_TRY(open_bc(c, c->Q[c->cur]));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return FCFD_OK;
 }
@@ -734,10 +800,14 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
 }
 
 int fcfd_solve_source(fcfd_ctx *c, double dt, double time, double *hrr, double *hrr_dot) {
-    (void)time;
     if (!c) { set_error("fcfd_solve_source: invalid argument"); return FCFD_E_ARG; }
-    if (c->o.do_reaction != 1) return FCFD_OK;
-    HIP_TRY(enqueue_source(c, dt, nullptr, hrr || hrr_dot));
+    if (c->o.do_reaction == 1) HIP_TRY(enqueue_source(c, dt, nullptr, hrr || hrr_dot));
+    if (aux_on(c)) HIP_TRY(enqueue_aux(c, dt, nullptr, time, nullptr));
+    if (c->o.do_reaction != 1) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(collect_timing(c));
+        return FCFD_OK;
+    }
     const size_t nint = size_t(c->nblk) * c->ncell;
     if (hrr) HIP_TRY(hipMemcpyAsync(hrr, c->d_hrr, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
     if (hrr_dot) HIP_TRY(hipMemcpyAsync(hrr_dot, c->d_hrr_dot, sizeof(double) * nint, hipMemcpyDeviceToHost, c->stream));
@@ -756,6 +826,145 @@ int fcfd_set_step(fcfd_ctx *c, int strang, int hrr_each_step) {
     drop_graphs(c);
     c->strang = strang;
     c->step_hrr = hrr_each_step;
+    return FCFD_OK;
+}
+
+// ---- open boundaries and auxiliary sources -----------------------------------------------------
+
+namespace {
+
+int ensure_tran(fcfd_ctx *c) {
+    if (c->d_tran) return FCFD_OK;
+    const fcfd_mech *m = c->mech;
+    if (!m->host.has_tran) { set_error("the wall heat loss needs the transport link file (tran.bin)"); return FCFD_E_ARG; }
+    if (m->host.no > 4) { set_error("transport fits of order above 4 are not supported"); return FCFD_E_UNSUPPORTED; }
+    auto t = std::make_unique<DevTran>();
+    std::memset(t.get(), 0, sizeof(DevTran));
+    build_dev_tran(m->host, *t);
+    DevBuf b;
+    HIP_TRY(hipMalloc(&b.p, sizeof(DevTran)));
+    HIP_TRY(hipMemcpy(b.p, t.get(), sizeof(DevTran), hipMemcpyHostToDevice));
+    c->d_tran = static_cast<DevTran *>(b.p);
+    b.p = nullptr;
+    return FCFD_OK;
+}
+
+int push_aux(fcfd_ctx *c) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->d_aux, &c->aux, sizeof(AuxParams), hipMemcpyHostToDevice));
+    drop_graphs(c);                      // the captured step enqueues the sources that are on
+    return FCFD_OK;
+}
+
+int ensure_cells_buffer(fcfd_ctx *c, double **p) {
+    if (*p) return FCFD_OK;
+    const size_t n = size_t(c->nblk) * c->ncell;
+    HIP_TRY(hipMalloc(p, sizeof(double) * n));
+    HIP_TRY(hipMemset(*p, 0, sizeof(double) * n));
+    return FCFD_OK;
+}
+
+}  // namespace
+
+int fcfd_set_flow_boundary(fcfd_ctx *c, double T_in, double u_in_cm_s, const double *x_in, double p_out_atm) {
+    if (!c) { set_error("fcfd_set_flow_boundary: invalid argument"); return FCFD_E_ARG; }
+    if (c->o.flag_lbound != 1 && c->o.flag_rbound != 1) {
+        set_error("fcfd_set_flow_boundary: the context has walls on both ends (flag_lbound = flag_rbound = 0)");
+        return FCFD_E_ARG;
+    }
+    if (c->o.flag_lbound == 1 && (!x_in || !(T_in > 0.0))) {
+        set_error("fcfd_set_flow_boundary: an inflow needs T_in > 0 and the inflow mole fractions");
+        return FCFD_E_ARG;
+    }
+    if (c->o.flag_rbound == 1 && !(p_out_atm > 0.0)) { set_error("fcfd_set_flow_boundary: p_out must be positive"); return FCFD_E_ARG; }
+    FlowBC h{};
+    h.lbound = c->o.flag_lbound; h.rbound = c->o.flag_rbound;
+    h.T_in = T_in;
+    h.p_out = p_out_atm * 101325.0;                                // SRC/flow_bound.f90:51
+    if (x_in)
+        for (int k = 0; k < c->kk; ++k) h.x_in[k] = x_in[k];
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->d_bc, &h, sizeof(FlowBC), hipMemcpyHostToDevice));
+    if (c->o.flag_lbound == 1) HIP_TRY(launch_flow_bc_setup(c->thermo, c->d_bc, u_in_cm_s * 1.e-2, c->kk, c->stream));
+    c->bc_set = true;
+    drop_graphs(c);
+    // refill the ghost cells of the current state as the start-up does (SRC/main.f90:96-114)
+    StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
+    s.ghosts_only = 1;
+    HIP_TRY(launch_stage_end(s, c->kk, c->stream));
+    HIP_TRY(open_bc(c, c->Q[c->cur]));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+int fcfd_set_radiation(fcfd_ctx *c, int n, const int *species, const double *pmac) {
+    if (!c || n < 0 || n > MAX_RAD || (n > 0 && (!species || !pmac))) {
+        set_error("fcfd_set_radiation: invalid argument (at most " + std::to_string(MAX_RAD) + " species)");
+        return FCFD_E_ARG;
+    }
+    for (int r = 0; r < n; ++r)
+        if (species[r] < 0 || species[r] >= c->kk) { set_error("fcfd_set_radiation: species index out of range"); return FCFD_E_ARG; }
+    if (n > 0)
+        if (int rc = ensure_cells_buffer(c, &c->d_qrad)) return rc;
+    for (int r = 0; r < n; ++r) {
+        c->aux.rad_idx[r] = species[r];
+        for (int j = 0; j < 6; ++j) c->aux.pmac[r][j] = pmac[6 * r + j];
+    }
+    c->aux.nrad = n;
+    return push_aux(c);
+}
+
+int fcfd_set_wall_heat(fcfd_ctx *c, int on, double Nu, double d_mm, double Tw_min, double Tw_max, double alpha,
+                       double beta) {
+    if (!c || (on != 0 && on != 1) || (on && !(d_mm > 0.0))) { set_error("fcfd_set_wall_heat: invalid argument"); return FCFD_E_ARG; }
+    if (on) {
+        if (int rc = ensure_tran(c)) return rc;
+        if (int rc = ensure_cells_buffer(c, &c->d_qw)) return rc;
+    }
+    c->aux.wall_heat = on;
+    c->aux.Nu = Nu; c->aux.d_m = d_mm * 1.0e-3;                     // SRC/mfr.f90:36
+    c->aux.Tw_min = Tw_min; c->aux.Tw_max = Tw_max; c->aux.alpha = alpha; c->aux.beta = beta;
+    return push_aux(c);
+}
+
+int fcfd_set_ignition(fcfd_ctx *c, int type, int n, const double *E_mJ, const double *r_um, const double *t_us,
+                      const double *t0_us) {
+    if (!c || type < 0 || type > 2 || n < 0 || n > MAX_IG || (type != 0 && (n == 0 || !E_mJ || !r_um || !t_us || !t0_us))) {
+        set_error("fcfd_set_ignition: invalid argument (type 0, 1 or 2; 1 to " + std::to_string(MAX_IG) + " sources)");
+        return FCFD_E_ARG;
+    }
+    const double pi = 3.14159265359;                                // SRC/ignitionsource.f90:27
+    for (int s = 0; s < (type ? n : 0); ++s) {                      // :57-61, :121-125
+        c->aux.E_j[s] = E_mJ[s] * 1.e-3;
+        c->aux.r_m[s] = r_um[s] * 1.e-6;
+        c->aux.t_s[s] = t_us[s] * 1.e-6;
+        c->aux.t0_s[s] = t0_us[s] * 1.e-6;
+        c->aux.vol[s] = 4.0 * pi * (c->aux.r_m[s] * c->aux.r_m[s] * c->aux.r_m[s]) / 3.0;
+    }
+    c->aux.ig_type = type;
+    c->aux.n_ig = type ? n : 0;
+    return push_aux(c);
+}
+
+int fcfd_set_cell_centres(fcfd_ctx *c, const double *xm) {
+    if (!c || !xm) { set_error("fcfd_set_cell_centres: invalid argument"); return FCFD_E_ARG; }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->d_xm, xm, sizeof(double) * size_t(c->nblk) * c->ncell, hipMemcpyHostToDevice));
+    return FCFD_OK;
+}
+
+int fcfd_get_aux(fcfd_ctx *c, double *qrad, double *qw) {
+    if (!c) { set_error("fcfd_get_aux: invalid argument"); return FCFD_E_ARG; }
+    const size_t n = size_t(c->nblk) * c->ncell;
+    if (qrad) {
+        if (c->d_qrad) HIP_TRY(hipMemcpyAsync(qrad, c->d_qrad, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        else std::memset(qrad, 0, sizeof(double) * n);
+    }
+    if (qw) {
+        if (c->d_qw) HIP_TRY(hipMemcpyAsync(qw, c->d_qw, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        else std::memset(qw, 0, sizeof(double) * n);
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return FCFD_OK;
 }
 
@@ -973,7 +1182,7 @@ int fcfd_flux_advection_host(const fcfd_mech *m, const fcfd_opts *o, int n, int 
     a.thermo = static_cast<DevThermo *>(th.p); a.qf = static_cast<double *>(dq.p);
     a.F = static_cast<double *>(dF.p); a.godunov = static_cast<double *>(dG.p);
     a.nblk = 1; a.ncell = n; a.lv = lv; a.S = int(cols); a.P = cols; a.PF = nif;
-    a.muscl = oo.flag_muscl; a.riemann = oo.select_riemann;
+    a.muscl = oo.flag_muscl; a.riemann = oo.select_riemann; a.spherical = oo.is_spherical == 1;
     HIP_TRY(launch_flux(a, m->host.kk, oo.flag_acu, oo.flag_weno, nullptr));
     HIP_TRY(hipDeviceSynchronize());
     std::vector<double> hF(size_t(ne) * nif), hG(size_t(3) * nif);

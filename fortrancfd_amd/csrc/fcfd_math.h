@@ -59,19 +59,8 @@ FM_FN uint64_t fm_to_bits(double d) {
 /* 2^m for -1022 <= m <= 1023 */
 FM_FN double fm_pow2i(int m) { return fm_from_bits((uint64_t)(m + 1023) << 52); }
 
-/* exp(xh + xl), |xl| small against ulp(xh) * 2^10 */
-FM_FN double fm_exp_dd(double xh, double xl) {
-    if (xh != xh) return xh + xh;
-    if (xh > 709.79) return __builtin_inf();
-    if (xh < -745.2) return 0.0;
-    const double kd = (xh * FM_INV_LN2 + FM_SHIFT) - FM_SHIFT;
-    /* |xh| >= 1/2 or kd = 0: xh - kd*LN2_HI is exactly representable, so the fma is exact */
-    const double r0 = __builtin_fma(-kd, FM_LN2_HI, xh);
-    /* r = rh + rl = r0 + xl - kd*LN2_LO with rh the rounded sum and rl its rounding error */
-    const double r1 = xl - kd * FM_LN2_LO;
-    const double rh = r0 + r1;
-    const double rl = r1 - (rh - r0);
-    /* q = rh^2 (1/2! + rh/3! + ... + rh^12/14!) */
+/* 1/2! + r/3! + ... + r^12/14!, the Taylor series of (exp(r) - 1 - r) / r^2 */
+FM_FN double fm_exp_series(double rh) {
     double p = 0x1.93974a8c07c9dp-37;            /* 1/14! */
     p = __builtin_fma(p, rh, 0x1.6124613a86d09p-33);  /* 1/13! */
     p = __builtin_fma(p, rh, 0x1.1eed8eff8d898p-29);  /* 1/12! */
@@ -85,6 +74,23 @@ FM_FN double fm_exp_dd(double xh, double xl) {
     p = __builtin_fma(p, rh, 0x1.5555555555555p-5);   /* 1/4! */
     p = __builtin_fma(p, rh, 0x1.5555555555555p-3);   /* 1/3! */
     p = __builtin_fma(p, rh, 0.5);                    /* 1/2! */
+    return p;
+}
+
+/* exp(xh + xl), |xl| small against ulp(xh) * 2^10 */
+FM_FN double fm_exp_dd(double xh, double xl) {
+    if (xh != xh) return xh + xh;
+    if (xh > 709.79) return __builtin_inf();
+    if (xh < -745.2) return 0.0;
+    const double kd = (xh * FM_INV_LN2 + FM_SHIFT) - FM_SHIFT;
+    /* |xh| >= 1/2 or kd = 0: xh - kd*LN2_HI is exactly representable, so the fma is exact */
+    const double r0 = __builtin_fma(-kd, FM_LN2_HI, xh);
+    /* r = rh + rl = r0 + xl - kd*LN2_LO with rh the rounded sum and rl its rounding error */
+    const double r1 = xl - kd * FM_LN2_LO;
+    const double rh = r0 + r1;
+    const double rl = r1 - (rh - r0);
+    /* q = rh^2 (1/2! + rh/3! + ... + rh^12/14!) */
+    const double p = fm_exp_series(rh);
     const double q = (rh * rh) * p;
     /* exp(r) = (1 + rh + q)(1 + rl) to first order in rl (|rl| <= ulp(rh) / 2); 1 + rh as an exact
      * two-term sum (|rh| < 1) */
@@ -97,6 +103,38 @@ FM_FN double fm_exp_dd(double xh, double xl) {
 }
 
 FM_FN double fm_exp(double x) { return fm_exp_dd(x, 0.0); }
+
+/* exp(x) - 1 without the cancellation of forming exp(x) first: the same reduction x = k ln2 + r,
+ * e^r - 1 = r + r^2 P(r) kept apart from the 1, then 2^k (e^r - 1) + (2^k - 1), the second term
+ * exact for |k| <= 53.  Error below 2 ulp. */
+FM_FN double fm_expm1(double x) {
+    if (x != x) return x + x;
+    if (x > 36.0) return fm_exp(x) - 1.0;      /* exp(x) > 2^51: the -1 is a rounding of it */
+    if (x < -38.0) return -1.0;                /* exp(x) < 2^-54: -1 + exp(x) rounds to -1 */
+    if (x > -0x1.62e42fefa39efp-2 && x < 0x1.62e42fefa39efp-2)   /* |x| < ln2 / 2: k = 0 */
+        return x + (x * x) * fm_exp_series(x);
+    const double kd = (x * FM_INV_LN2 + FM_SHIFT) - FM_SHIFT;
+    const double r0 = __builtin_fma(-kd, FM_LN2_HI, x);
+    const double r1 = -kd * FM_LN2_LO;
+    const double rh = r0 + r1;
+    const double rl = r1 - (rh - r0);
+    const double t = rh + (rl + (rh * rl + (rh * rh) * fm_exp_series(rh)));   /* e^r - 1 */
+    const double sk = fm_pow2i((int)kd);
+    return t * sk + (sk - 1.0);
+}
+
+/* tanh(x) = -expm1(-2|x|) / (2 + expm1(-2|x|)) with the sign of x; 1 for |x| > 22 (the quotient
+ * is then 1 - 2e^-44, which rounds to 1).  Error below 3 ulp. */
+FM_FN double fm_tanh(double x) {
+    if (x != x) return x + x;
+    const double a = x < 0.0 ? -x : x;
+    double r = 1.0;
+    if (a <= 22.0) {
+        const double em = fm_expm1(-2.0 * a);
+        r = -em / (2.0 + em);
+    }
+    return x < 0.0 ? -r : r;
+}
 
 /* log(x) as a two-term value (*lo carries the low part); x finite and > 0 */
 FM_FN double fm_log_dd(double x, double *lo) {

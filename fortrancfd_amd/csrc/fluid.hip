@@ -77,7 +77,7 @@ __device__ __forceinline__ Side pair_other(const Side &me) {
     return o;
 }
 
-template <int NSP, int RK>
+template <int NSP, int RK, bool SPH>
 __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
     const long g2 = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long g = g2 >> 1;                               // interface (pairs never straddle a wave)
@@ -111,7 +111,9 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
 
     // physical fluxes of mass, momentum, energy (SRC/flux.f90:270-280)
     const double fl0 = L.rho * L.u, fr0 = R.rho * R.u;
-    const double fl1 = L.rho * L.u * L.u + L.p, fr1 = R.rho * R.u * R.u + R.p;
+    // spherical: no pressure in the momentum flux; the update adds its gradient (SRC/flux.f90:272-278)
+    const double fl1 = SPH ? L.rho * L.u * L.u : L.rho * L.u * L.u + L.p;
+    const double fr1 = SPH ? R.rho * R.u * R.u : R.rho * R.u * R.u + R.p;
     const double fl2 = (L.E + L.p) * L.u, fr2 = (R.E + R.p) * R.u;
     double *__restrict__ F = a.F + g;
     const long PF = a.PF;
@@ -252,7 +254,19 @@ __global__ void __launch_bounds__(64, 2) k_stage_end(StageArgs a) {
     ghost_source(a, b, i, sb, si, sgn);
     const long sc = long(sb) * a.S + a.lv + si;
     const long fj = long(sb) * (a.ncell + 1) + si;
-    const double dtdx = (a.dt_dev ? *a.dt_dev : a.dt) / a.dx;
+    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const double dtdx = dt / a.dx;
+    // spherical (SRC/timeintegration.f90:58-72, 177-196): face areas (r -+ dx/2)^2 and volume
+    // r^2 dx at the source cell's centre
+    const bool sph = a.spherical && a.mode != 2;
+    double Al = 0.0, Ar = 0.0, dtdV = 0.0;
+    if (sph) {
+        const double rm = a.xm[long(sb) * a.ncell + si];
+        const double tl = rm - (a.dx / 2.0), tr = rm + (a.dx / 2.0);
+        Al = tl * tl;
+        Ar = tr * tr;
+        dtdV = dt / (rm * rm * a.dx);
+    }
     double Qv[NSP + 3];
 #pragma unroll
     for (int v = 0; v < NSP + 3; ++v) {
@@ -260,10 +274,12 @@ __global__ void __launch_bounds__(64, 2) k_stage_end(StageArgs a) {
         if (a.mode == 2) {
             Q = a.Qin[v * a.P + sc];
         } else {
-            const double d = a.F[v * a.PF + fj + 1] - a.F[v * a.PF + fj];
+            const double fr = a.F[v * a.PF + fj + 1], fl = a.F[v * a.PF + fj];
             const double qi = a.Qin[v * a.P + sc];
-            if (a.mode == 0) Q = qi - dtdx * d;
-            else Q = a.h1 * a.Q0[v * a.P + sc] + a.h2 * (qi - dtdx * d);
+            const double base = sph ? qi - dtdV * (Ar * fr - Al * fl) : qi - dtdx * (fr - fl);
+            if (a.mode == 0) Q = base;
+            else Q = a.h1 * a.Q0[v * a.P + sc] + a.h2 * base;
+            if (sph && v == 1) Q = Q - dtdx * (a.god[2 * a.PF + fj + 1] - a.god[2 * a.PF + fj]);
         }
         if (v == 1) Q = sgn < 0.0 ? -Q : Q;
         Qv[v] = Q;
@@ -497,17 +513,17 @@ static int recon_kind(int flag_acu, int flag_weno) {
     }
 }
 
-template <int NSP>
+template <int NSP, bool SPH>
 static hipError_t launch_flux_nsp(const FluxArgs &a, int rk, hipStream_t st) {
     const long n = 2 * long(a.nblk) * (a.ncell + 1);            // two lanes per interface
     const dim3 grid(unsigned((n + 63) / 64)), blk(64);
     switch (rk) {
-        case R_FIRST: hipLaunchKernelGGL((k_flux<NSP, R_FIRST>), grid, blk, 0, st, a); break;
-        case R_MUSCL: hipLaunchKernelGGL((k_flux<NSP, R_MUSCL>), grid, blk, 0, st, a); break;
-        case R_WENO3Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO3Z>), grid, blk, 0, st, a); break;
-        case R_WENO5JS: hipLaunchKernelGGL((k_flux<NSP, R_WENO5JS>), grid, blk, 0, st, a); break;
-        case R_WENO5Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO5Z>), grid, blk, 0, st, a); break;
-        case R_WENO7Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO7Z>), grid, blk, 0, st, a); break;
+        case R_FIRST: hipLaunchKernelGGL((k_flux<NSP, R_FIRST, SPH>), grid, blk, 0, st, a); break;
+        case R_MUSCL: hipLaunchKernelGGL((k_flux<NSP, R_MUSCL, SPH>), grid, blk, 0, st, a); break;
+        case R_WENO3Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO3Z, SPH>), grid, blk, 0, st, a); break;
+        case R_WENO5JS: hipLaunchKernelGGL((k_flux<NSP, R_WENO5JS, SPH>), grid, blk, 0, st, a); break;
+        case R_WENO5Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO5Z, SPH>), grid, blk, 0, st, a); break;
+        case R_WENO7Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO7Z, SPH>), grid, blk, 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -516,7 +532,7 @@ static hipError_t launch_flux_nsp(const FluxArgs &a, int rk, hipStream_t st) {
 hipError_t launch_flux(const FluxArgs &a, int nsp, int flag_acu, int flag_weno, hipStream_t st) {
     const int rk = recon_kind(flag_acu, flag_weno);
     if (rk < 0) return hipErrorInvalidValue;
-#define FCFD_CASE(N) if (nsp == N) return launch_flux_nsp<N>(a, rk, st);
+#define FCFD_CASE(N) if (nsp == N) return a.spherical ? launch_flux_nsp<N, true>(a, rk, st) : launch_flux_nsp<N, false>(a, rk, st);
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;

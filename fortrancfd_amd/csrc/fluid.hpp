@@ -14,6 +14,7 @@ struct FluxArgs {
     long P, PF;
     int muscl, riemann;
     int accumulate;       // 1: F += advection flux (transport flux already in F), 0: F = flux
+    int spherical;        // 1: momentum flux rho u u without p (SRC/flux.f90:272-278)
 };
 
 struct StageArgs {
@@ -25,6 +26,9 @@ struct StageArgs {
     double dt, dx, h1, h2;
     int mode;              // 0 Qin - dt/dx dF, 1 h1*Q0 + h2*(Qin - dt/dx dF), 2 copy; RK4: stage 0..3
     int ghosts_only;
+    int spherical;         // 1: r^2-weighted update with the pressure-gradient term (needs xm, god)
+    const double *xm;      // [nblk * ncell] cell centres (spherical)
+    const double *god;     // [3][PF] Roe states of the interfaces; [2] the pressure (spherical)
     int nblk, ncell, lv, S;
     long P, PF;
 };

@@ -34,9 +34,10 @@ class Opts(C.Structure):
 
 
 def opts(do_hydro=1, do_tran=0, do_reaction=0, select_time=1, select_riemann=2, flag_acu=2,
-         flag_muscl=0, flag_weno=2, atol=1e-13, rtol=1e-5) -> Opts:
+         flag_muscl=0, flag_weno=2, atol=1e-13, rtol=1e-5, is_spherical=0, flag_lbound=0,
+         flag_rbound=0) -> Opts:
     return Opts(do_hydro, do_tran, do_reaction, select_time, select_riemann, flag_acu, flag_muscl,
-                flag_weno, 0, 0, 0, atol, rtol)
+                flag_weno, is_spherical, flag_lbound, flag_rbound, atol, rtol)
 
 
 def lib():
@@ -322,6 +323,48 @@ class Solver:
         t = C.c_double(time)
         _check(lib().fcfd_advance(self.handle, C.c_double(cfl), C.c_int(nsteps), C.byref(t)))
         return t.value
+
+    # ---- open boundaries and auxiliary per-cell sources ----------------------------------------
+    def set_flow_boundary(self, T_in=300.0, u_in_cm_s=0.0, x_in=None, p_out_atm=1.0):
+        """Inflow / outflow state for flag_lbound = 1 / flag_rbound = 1 (the values of the
+        reference's flow-boundary file, fortrancfd_amd.inputs.read_flow_boundary)."""
+        x = None if x_in is None else _f64(x_in)
+        _check(lib().fcfd_set_flow_boundary(self.handle, C.c_double(T_in), C.c_double(u_in_cm_s),
+                                            None if x is None else _p(x), C.c_double(p_out_atm)))
+
+    def set_radiation(self, species, pmac):
+        """Optically thin radiation loss of the given species (0-based) with their six absorption
+        coefficients each; an empty list turns it off."""
+        sp = np.ascontiguousarray(species, np.int32)
+        pm = _f64(pmac).reshape(-1)
+        _check(lib().fcfd_set_radiation(self.handle, C.c_int(sp.size),
+                                        sp.ctypes.data_as(C.POINTER(C.c_int)) if sp.size else None,
+                                        _p(pm) if sp.size else None))
+
+    def set_wall_heat(self, Nu, d_mm, Tw_min, Tw_max, alpha, beta, on=True):
+        """Micro-flow-reactor wall heat loss (the &mfrwalltemp values)."""
+        _check(lib().fcfd_set_wall_heat(self.handle, C.c_int(int(on)), *(C.c_double(v) for v in
+                                        (Nu, d_mm, Tw_min, Tw_max, alpha, beta))))
+
+    def set_ignition(self, kind, E_mJ=(), r_um=(), t_us=(), t0_us=()):
+        """Ignition deposit: kind 1 Frendi, 2 Zhang, 0 off; per source energy (mJ), radius (um),
+        duration and start (us)."""
+        arrs = [_f64(np.atleast_1d(v)) for v in (E_mJ, r_um, t_us, t0_us)]
+        n = arrs[0].size
+        _check(lib().fcfd_set_ignition(self.handle, C.c_int(kind), C.c_int(n),
+                                       *((_p(a) if n else None) for a in arrs)))
+
+    def set_cell_centres(self, xm):
+        xm = _f64(xm)
+        assert xm.size == self.nblk * self.ncell
+        _check(lib().fcfd_set_cell_centres(self.handle, _p(xm)))
+
+    def get_aux(self):
+        """(Qrad, Qw) of the last auxiliary-source call, W/m^3 per interior cell."""
+        n = self.nblk * self.ncell
+        qr, qw = np.zeros(n), np.zeros(n)
+        _check(lib().fcfd_get_aux(self.handle, _p(qr), _p(qw)))
+        return qr, qw
 
     def set_timing(self, on: bool):
         _check(lib().fcfd_set_timing(self.handle, C.c_int(1 if on else 0)))

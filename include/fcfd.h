@@ -43,8 +43,8 @@ typedef struct fcfd_opts {
     int flag_muscl;      /* 1 unlimited, 2 minmod, 3 superbee, 4 MC               */
     int flag_weno;       /* 1 WENO3-Z, 2 WENO5-JS, 3 WENO5-Z, 4 WENO7-Z           */
     int is_spherical;    /* must be 0 (Cartesian)                                 */
-    int flag_lbound;     /* 0 reflective wall (only value supported)              */
-    int flag_rbound;     /* 0 reflective wall (only value supported)              */
+    int flag_lbound;     /* 0 reflective wall, 1 inflow (fcfd_set_flow_boundary)  */
+    int flag_rbound;     /* 0 reflective wall, 1 outflow (fcfd_set_flow_boundary) */
     double atol, rtol;   /* chemistry tolerances (solver.inp atol/rtol)          */
 } fcfd_opts;
 
@@ -109,8 +109,10 @@ int fcfd_solve_fluid(fcfd_ctx *c, double dt);
  * SRC/source.f90:17-84): integrates every interior cell's (Y, T) over dt at fixed density,
  * renormalises Y, writes T and Y into the primitive state and rebuilds the interior conserved
  * state (qfToQc).  hrr / hrr_dot (may be NULL) receive eos_obj(:)%HRR and %HRR_dot, one value per
- * interior cell, blocks in order.  Requires do_reaction = 1.  The time argument is unused (the
- * reference passes it for the ignition source, which is not part of this path).  The context keeps,
+ * interior cell, blocks in order (do_reaction = 1).  Then the auxiliary sources that are on
+ * (fcfd_set_radiation, fcfd_set_wall_heat, fcfd_set_ignition) in the reference's order
+ * (SRC/update.f90:127-147); time is the driver's time after the step, which the ignition source's
+ * window reads.  With do_reaction = 0 and no auxiliary source the call does nothing.  The context keeps,
  * per cell, the implicit integrator's last step-size proposal and starts the next call's first step
  * there (when below dt); fcfd_upload and fcfd_init_cells clear it. */
 int fcfd_solve_source(fcfd_ctx *c, double dt, double time, double *hrr, double *hrr_dot);
@@ -122,7 +124,8 @@ int fcfd_chem_stats(fcfd_ctx *c, int *per_cell);
 int fcfd_refresh(fcfd_ctx *c);
 
 /* Device-resident stepping for production loops: nsteps x {calc_dt, solve_fluid, solve_source
- * (when do_reaction = 1), refresh} with dt kept on the device and the step replayed from a
+ * (when do_reaction = 1 or an auxiliary source is on), refresh} with dt kept on the device and the
+ * step replayed from a
  * captured HIP graph.  *time is advanced by the sum of the steps' dt.  Equivalent to the driver
  * loop SRC/main.f90:130-163 without output.  fcfd_set_step chooses the splitting (Lie by default)
  * and whether the source calls compute HRR / HRR_dot (the reference does on every call,
@@ -136,6 +139,43 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time);
  * every step computes HRR / HRR_dot (SRC/source.f90:70-73), read with fcfd_get_hrr; 0: it skips
  * them (one rate evaluation per cell less). */
 int fcfd_set_step(fcfd_ctx *c, int strang, int hrr_each_step);
+/* ---- open boundaries and auxiliary per-cell sources ---------------------------------------- */
+/* Inflow / outflow state for flag_lbound = 1 / flag_rbound = 1 (update_boundary,
+ * SRC/boundary.f90:12-149), from the values set_flow_boundary reads from its input file
+ * (SRC/flow_bound.f90:15-57): inflow temperature T_in (K), velocity u_in (cm/s) and mole fractions
+ * x_in[KK] (may be NULL without an inflow), outflow pressure p_out (atm).  The inflow ghost cells
+ * take the density of the interior cell they mirror, the mass flux density(p_out, T_in, x_in) * u_in,
+ * T_in and Y_in; the outflow ghost cells take the mirrored cell's velocity, composition and
+ * temperature at p_out.  Refills the ghost cells of the current state; must precede
+ * fcfd_init_cells when either flag is 1.  Supported with select_time 0 and 1. */
+int fcfd_set_flow_boundary(fcfd_ctx *c, double T_in, double u_in_cm_s, const double *x_in, double p_out_atm);
+/* Optically thin radiative heat loss (heat_transfer_radiation, SRC/source.f90:85-132 ->
+ * radiation_heat_loss, SRC/radiation.f90:74-121): n radiating species (0-based indices, at most 16)
+ * with six Planck-mean absorption coefficients a_0..a_5 each (row-major n x 6, the columns of the
+ * reference's radiation parameter file, set_radiation_params :21-71).  n = 0 turns it off. */
+int fcfd_set_radiation(fcfd_ctx *c, int n, const int *species, const double *pmac);
+/* Wall heat loss of the micro flow reactor (heat_transfer_mfr, SRC/source.f90:135-182 ->
+ * heat_transfer, SRC/mfr.f90:41-60): Qw = -4 lambda Nu (T - Tw(x)) / d^2 with the mixture
+ * conductivity lambda (MCACON, needs the transport link file) and the wall temperature
+ * Tw(x) = (Tw_max - Tw_min)/2 tanh(alpha (x - beta)) + (Tw_max + Tw_min)/2 (:63-78); d in mm, the
+ * &mfrwalltemp namelist's values.  on = 0 turns it off. */
+int fcfd_set_wall_heat(fcfd_ctx *c, int on, double Nu, double d_mm, double Tw_min, double Tw_max,
+                       double alpha, double beta);
+/* Ignition energy deposit (energy_source, SRC/source.f90:185-236): type 1 A. Frendi
+ * (add_igsource), 2 W. Zhang (add_igsource_WZhang), SRC/ignitionsource.f90:150-264; n sources
+ * (at most 16; 1 for the single-source input, more for set_multi_igsource) with energy (mJ),
+ * radius (um), duration (us) and start time (us), the units of the reference's input files.  A
+ * source deposits while start < time < start + duration; where several are active, the last one's
+ * power is the cell's (the reference overwrites, :180-195).  type = 0 turns it off. */
+int fcfd_set_ignition(fcfd_ctx *c, int type, int n, const double *E_mJ, const double *r_um,
+                      const double *t_us, const double *t0_us);
+/* Cell centres x_m (m) of the interior cells, blocks in order (b_obj(ib)%xm_b).  fcfd_create sets
+ * the uniform block layout (SRC/block.f90:264-305): a running sum of dx from 0 plus dx/2. */
+int fcfd_set_cell_centres(fcfd_ctx *c, const double *xm);
+/* Qrad and Qw (W/m^3) of the last auxiliary-source call, one value per interior cell, blocks in order
+ * (eos_obj(:)%Qrad, %qTw); zeros for a source that is off.  Either pointer may be NULL. */
+int fcfd_get_aux(fcfd_ctx *c, double *qrad, double *qw);
+
 /* HRR and HRR_dot of the last source call that computed them (one value per interior cell, blocks
  * in order; either pointer may be NULL). */
 int fcfd_get_hrr(fcfd_ctx *c, double *hrr, double *hrr_dot);
@@ -144,7 +184,8 @@ int fcfd_get_hrr(fcfd_ctx *c, double *hrr, double *hrr_dot);
  * one kernel at a time instead of from the captured graph).  Kernel ids: 0 advection flux,
  * 1 stage update + primitive recovery, 2 CFL reduction, 3 chemistry implicit integrator,
  * 4 interior qfToQc after the source, 5 surrogate (features, network, acceptance), 6 chemistry
- * explicit screen, 7 transport flux, 8 chemistry explicit half-step retry of a long stiff list. */
+ * explicit screen, 7 transport flux, 8 chemistry explicit half-step retry of a long stiff list,
+ * 9 auxiliary sources. */
 int fcfd_set_timing(fcfd_ctx *c, int enable);
 int fcfd_get_timing(fcfd_ctx *c, int kernel_id, double *ms_total, long long *launches);
 

@@ -28,6 +28,8 @@
 #define FO_LOG10 log10
 #define FO_POW pow
 #define FO_EXP10(x) pow(10.0, (x))
+#define FO_TANH tanh
+#define FO_EXPM1 expm1
 #else
 #include "../fortrancfd_amd/csrc/fcfd_math.h"
 #define FO_EXP fm_exp
@@ -35,6 +37,8 @@
 #define FO_LOG10 fm_log10
 #define FO_POW fm_pow
 #define FO_EXP10 fm_exp10
+#define FO_TANH fm_tanh
+#define FO_EXPM1 fm_expm1
 #endif
 
 /* ============================================================================================ */
@@ -790,16 +794,47 @@ static double *cellp(fo_domain *d, double *base, int ib, int i) {
     return base + ((size_t)ib * (d->ncell + 2 * d->lv) + (i + d->lv)) * d->nequ;
 }
 
+static void lbound_inlet(fo_domain *d, double *gh, const double *in) { /* SRC/boundary.f90:70-99 */
+    const fo_aux *a = &d->aux;
+    const double rho_in = in[0];
+    const double p_in = fo_pressure(d->m, rho_in, a->T_in, a->x_in);
+    (void)p_in;                                  /* specific_energy's p argument, unused by it */
+    gh[0] = rho_in;
+    gh[1] = a->rho_u_in;
+    const double u_in = a->rho_u_in / rho_in;
+    const double e_in = fo_specific_energy(d->m, a->T_in, a->x_in);
+    gh[2] = rho_in * (e_in + 0.5 * (u_in * u_in));
+    for (int k = 0; k < d->m->kk; ++k) gh[3 + k] = rho_in * a->y_in[k];
+}
+
+static void rbound_outlet(fo_domain *d, double *gh, const double *in) { /* SRC/boundary.f90:102-149 */
+    const int kk = d->m->kk;
+    double y[64] = {0.0}, x[64];
+    const double u = in[1] / in[0];
+    const double e = in[2] / in[0] - 0.5 * (u * u);
+    for (int k = 0; k < kk; ++k) y[k] = in[3 + k] / in[0];
+    fo_ckytx(d->m, y, x);
+    const double T = fo_temperature_con(d->m, e, x);
+    const double rho = fo_density(d->m, d->aux.p_out, T, x);
+    gh[0] = rho;
+    gh[1] = rho * u;
+    const double e2 = fo_specific_energy(d->m, T, x);
+    gh[2] = rho * (e2 + 0.5 * (u * u));
+    for (int k = 0; k < kk; ++k) gh[3 + k] = rho * y[k];
+}
+
 static void update_boundary(fo_domain *d) {                       /* SRC/boundary.f90:12-67 */
     const int lv = d->lv, nequ = d->nequ, nl = d->nblk - 1, nc = d->ncell;
     for (int g = 1; g <= lv; ++g) {                                /* lbound_wall :152-174 */
         double *gh = cellp(d, d->Qc, 0, -g), *in = cellp(d, d->Qc, 0, g - 1);
+        if (d->aux.lbound == 1) { lbound_inlet(d, gh, in); continue; }
         memcpy(gh, in, sizeof(double) * nequ);
         gh[1] = -in[1];
     }
     for (int g = 1; g <= lv; ++g) fo_qc_to_qf(d->m, cellp(d, d->Qc, 0, -g), cellp(d, d->qf, 0, -g));
     for (int g = 1; g <= lv; ++g) {                                /* rbound_wall :177-199 */
         double *gh = cellp(d, d->Qc, nl, nc - 1 + g), *in = cellp(d, d->Qc, nl, nc - g);
+        if (d->aux.rbound == 1) { rbound_outlet(d, gh, in); continue; }
         memcpy(gh, in, sizeof(double) * nequ);
         gh[1] = -in[1];
     }
@@ -827,6 +862,18 @@ void fo_refresh(fo_domain *d) {                                   /* SRC/main.f9
     interpolation_block(d);
     update_boundary(d);
     qf_all(d);
+}
+
+/* start-up ghost fill (SRC/main.f90:96-114): boundaries, block halos, QcToqf on the ghosts */
+static void init_ghosts(fo_domain *d) {
+    update_boundary(d);
+    interpolation_block(d);
+    update_boundary(d);
+    for (int ib = 0; ib < d->nblk; ++ib)
+        for (int g = 1; g <= d->lv; ++g) {
+            fo_qc_to_qf(d->m, cellp(d, d->Qc, ib, -g), cellp(d, d->qf, ib, -g));
+            fo_qc_to_qf(d->m, cellp(d, d->Qc, ib, d->ncell - 1 + g), cellp(d, d->qf, ib, d->ncell - 1 + g));
+        }
 }
 
 fo_domain *fo_domain_create(const fo_mech *m, const fo_opts *o, int nblk, int ncell, double dx,
@@ -859,14 +906,18 @@ fo_domain *fo_domain_create(const fo_mech *m, const fo_opts *o, int nblk, int nc
             Q[0] = rho; Q[1] = rho * c[2]; Q[2] = rho * (e + 0.5 * (c[2] * c[2]));
             for (int k = 0; k < kk; ++k) { q[3 + k] = y[k]; Q[3 + k] = rho * y[k]; }
         }
-    update_boundary(d);                                             /* main.f90:96-114 */
-    interpolation_block(d);
-    update_boundary(d);
-    for (int ib = 0; ib < nblk; ++ib)
-        for (int g = 1; g <= d->lv; ++g) {
-            fo_qc_to_qf(m, cellp(d, d->Qc, ib, -g), cellp(d, d->qf, ib, -g));
-            fo_qc_to_qf(m, cellp(d, d->Qc, ib, ncell - 1 + g), cellp(d, d->qf, ib, ncell - 1 + g));
-        }
+    /* cell centres of the uniform block layout (SRC/block.f90:264-305 without refinement): a
+     * running sum of dx from 0, shifted by the half cell length_smr = -dx/2 */
+    d->xm = calloc((size_t)nblk * ncell, sizeof(double));
+    d->qrad = calloc((size_t)nblk * ncell, sizeof(double));
+    d->qw = calloc((size_t)nblk * ncell, sizeof(double));
+    double xl = 0.0;
+    const double length_smr = 0.0 - dx / 2.0;
+    for (size_t g = 0; g < (size_t)nblk * ncell; ++g) {
+        if (g > 0) xl = xl + dx;
+        d->xm[g] = xl - length_smr;
+    }
+    init_ghosts(d);
     return d;
 }
 
@@ -874,6 +925,7 @@ void fo_domain_free(fo_domain *d) {
     if (!d) return;
     free(d->Qc); free(d->qf); free(d->Fplus); free(d->godunov);
     free(d->Qc0); free(d->k1); free(d->k2); free(d->k3); free(d->k4); free(d->hmem); free(d->cstats);
+    free(d->xm); free(d->qrad); free(d->qw);
     free(d);
 }
 
@@ -917,14 +969,38 @@ static void stage_sync(fo_domain *d) {
     qf_all(d);
 }
 
+/* Spherical update of one interior cell (SRC/timeintegration.f90:58-72 and 177-196): face areas
+ * (r -+ dx/2)^2 and volume r^2 dx at the cell centre r = x_m; the pressure gradient from the Roe
+ * pressures of the two interfaces (godunov_state(iqpres)), which the spherical momentum flux
+ * leaves out (SRC/flux.f90:272-278), subtracted after the combination with its full dt/dx. */
+static void sph_update(fo_domain *d, int ib, int i, double dt, double h1, double h2, const double *Q0,
+                       double *Q) {
+    const int nequ = d->nequ, nc = d->ncell;
+    const double dx = d->dx;
+    const double rm = d->xm[(size_t)ib * nc + i];
+    const double Al = (rm - (dx / 2.0)) * (rm - (dx / 2.0));
+    const double Ar = (rm + (dx / 2.0)) * (rm + (dx / 2.0));
+    const double dV = rm * rm * dx;
+    const double *Fr = fint(d, ib, i + 1), *Fl = fint(d, ib, i);
+    const double *Gr = d->godunov + ((size_t)ib * (nc + 1) + i + 1) * 3;
+    const double *Gl = d->godunov + ((size_t)ib * (nc + 1) + i) * 3;
+    for (int n = 0; n < nequ; ++n) {
+        const double v = Q[n] - (dt / dV) * (Ar * Fr[n] - Al * Fl[n]);
+        Q[n] = Q0 ? h1 * Q0[n] + h2 * v : v;
+    }
+    Q[1] = Q[1] - (dt / dx) * (Gr[2] - Gl[2]);
+}
+
 void fo_solve_fluid(fo_domain *d, double dt) {
     const int nequ = d->nequ, nc = d->ncell;
     const double dx = d->dx;
+    const int sph = d->o.is_spherical == 1;
     if (d->o.select_time == 0) {                                   /* explicit_euler :12-96 */
         compute_fluxes(d);
         for (int ib = 0; ib < d->nblk; ++ib)
             for (int i = 0; i < nc; ++i) {
                 double *Q = cellp(d, d->Qc, ib, i), *Fr = fint(d, ib, i + 1), *Fl = fint(d, ib, i);
+                if (sph) { sph_update(d, ib, i, dt, 1.0, 1.0, NULL, Q); continue; }
                 for (int n = 0; n < nequ; ++n) Q[n] = Q[n] - (dt / dx) * (Fr[n] - Fl[n]);
             }
         stage_sync(d);
@@ -945,6 +1021,7 @@ void fo_solve_fluid(fo_domain *d, double dt) {
                 for (int i = 0; i < nc; ++i) {
                     double *Q = cellp(d, d->Qc, ib, i), *Fr = fint(d, ib, i + 1), *Fl = fint(d, ib, i);
                     const double *Q0 = d->Qc0 + ((size_t)ib * nc + i) * nequ;
+                    if (sph) { sph_update(d, ib, i, dt, h1, h2, Q0, Q); continue; }
                     for (int n = 0; n < nequ; ++n)
                         Q[n] = h1 * Q0[n] + h2 * (Q[n] - (dt / dx) * (Fr[n] - Fl[n]));
                 }
@@ -1444,6 +1521,147 @@ void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *
     if (nfail) *nfail = fails;
 }
 
+/* ============================================================================================ */
+/* Auxiliary per-cell sources and open-boundary parameters                                        */
+/* ============================================================================================ */
+#define FO_PI_IG 3.14159265359             /* SRC/ignitionsource.f90:27 */
+#define FO_PI_IG_15 0x1.645f7c63f2ed6p+2   /* FO_PI_IG**1.5, correctly rounded */
+
+double fo_radiation_heat_loss(const fo_aux *a, double p, double T, const double *X) {
+    const double sigma = 5.6703744191844294e-08;                   /* SRC/radiation.f90:96 */
+    const double T1 = T, T2 = T1 * T1, T3 = T2 * T1, T4 = T3 * T1, T5 = T4 * T1;
+    const double T0 = 300.0, T04 = T0 * T0 * T0 * T0;
+    double Kp = 0.0;
+    for (int r = 0; r < a->nrad; ++r) {                             /* :110-116 */
+        const double *c = a->pmac[r];
+        const double kpi = c[0] + c[1] * T1 + c[2] * T2 + c[3] * T3 + c[4] * T4 + c[5] * T5;
+        Kp = Kp + kpi * X[a->rad_idx[r]] * (p / 101325.0);
+    }
+    return -(4.0 * Kp * sigma * (T4 - T04));                       /* :119 */
+}
+
+double fo_wall_temperature(const fo_aux *a, double x) {          /* SRC/mfr.f90:63-78 */
+    const double A = 0.5 * (a->Tw_max - a->Tw_min);
+    const double B = 0.5 * (a->Tw_max + a->Tw_min);
+    return A * FO_TANH(a->alpha * (x - a->beta)) + B;
+}
+
+double fo_wall_heat(const fo_mech *m, const fo_aux *a, double T, const double *X, double x_m) {
+    const double lambda = fo_mcacon(m, T, X) * 1.e-5;              /* SRC/trans.f90:74-76 */
+    return -(4.0 * lambda * a->Nu * (T - fo_wall_temperature(a, x_m)) / (a->d_m * a->d_m));  /* mfr.f90:57 */
+}
+
+double fo_ignition_power(const fo_aux *a, double t, double x_m) { /* SRC/ignitionsource.f90:150-264 */
+    double Eadd = 0.0;              /* energy_source zeroes it; a later active source overwrites */
+    for (int s = 0; s < a->n_ig; ++s) {
+        if (!(a->t0_s[s] < t && t < a->t0_s[s] + a->t_s[s])) continue;
+        const double q = x_m / a->r_m[s];
+        if (a->ig_type == 1) {                                      /* A. Frendi: :165-167 */
+            const double coef = a->E_j[s] / (a->vol[s] * a->t_s[s]);
+            const double q3 = q * q * q;
+            Eadd = coef * FO_EXP(-(FO_PI_IG / 4.0) * (q3 * q3));
+        } else {                                                    /* W. Zhang: :218-219 */
+            const double r = a->r_m[s];
+            const double coef = a->E_j[s] / (FO_PI_IG_15 * (r * r * r) * a->t_s[s]);
+            Eadd = coef * FO_EXP(-(q * q));
+        }
+    }
+    return Eadd;
+}
+
+int fo_domain_set_flow_boundary(fo_domain *d, int flag_lbound, int flag_rbound, double T_in,
+                                double u_in_cm_s, const double *x_in, double p_out_atm) {
+    if ((flag_lbound != 0 && flag_lbound != 1) || (flag_rbound != 0 && flag_rbound != 1)) return -1;
+    fo_aux *a = &d->aux;
+    const int kk = d->m->kk;
+    a->lbound = flag_lbound;
+    a->rbound = flag_rbound;
+    for (int k = 0; k < kk; ++k) a->x_in[k] = x_in ? x_in[k] : 0.0;
+    a->T_in = T_in;
+    a->u_in = u_in_cm_s * 1.e-2;                                    /* SRC/flow_bound.f90:50-51 */
+    a->p_out = p_out_atm * 101325.0;
+    if (flag_lbound == 1) {
+        fo_ckxty(d->m, a->x_in, a->y_in);                           /* :53-54 */
+        a->rho_u_in = fo_density(d->m, a->p_out, a->T_in, a->x_in) * a->u_in;
+    }
+    init_ghosts(d);
+    return 0;
+}
+
+int fo_domain_set_radiation(fo_domain *d, int n, const int *species, const double *pmac) {
+    if (n < 0 || n > FO_MAX_RAD) return -1;
+    for (int r = 0; r < n; ++r) {
+        if (species[r] < 0 || species[r] >= d->m->kk) return -1;
+        d->aux.rad_idx[r] = species[r];
+        for (int j = 0; j < 6; ++j) d->aux.pmac[r][j] = pmac[6 * r + j];
+    }
+    d->aux.nrad = n;
+    return 0;
+}
+
+void fo_domain_set_wall_heat(fo_domain *d, int on, double Nu, double d_mm, double Tw_min,
+                             double Tw_max, double alpha, double beta) {
+    fo_aux *a = &d->aux;
+    a->wall_heat = on ? 1 : 0;
+    a->Nu = Nu; a->d_m = d_mm * 1.0e-3;                             /* SRC/mfr.f90:36 */
+    a->Tw_min = Tw_min; a->Tw_max = Tw_max; a->alpha = alpha; a->beta = beta;
+}
+
+int fo_domain_set_ignition(fo_domain *d, int type, int n, const double *E_mJ, const double *r_um,
+                           const double *t_us, const double *t0_us) {
+    if (type < 0 || type > 2 || n < 0 || n > FO_MAX_IG || (type != 0 && n == 0)) return -1;
+    fo_aux *a = &d->aux;
+    for (int s = 0; s < n; ++s) {                                   /* SRC/ignitionsource.f90:57-61 */
+        a->E_j[s] = E_mJ[s] * 1.e-3;
+        a->r_m[s] = r_um[s] * 1.e-6;
+        a->t_s[s] = t_us[s] * 1.e-6;
+        a->t0_s[s] = t0_us[s] * 1.e-6;
+        a->vol[s] = 4.0 * FO_PI_IG * (a->r_m[s] * a->r_m[s] * a->r_m[s]) / 3.0;
+    }
+    a->ig_type = type;
+    a->n_ig = type ? n : 0;
+    return 0;
+}
+
+void fo_domain_set_xm(fo_domain *d, const double *xm) {
+    memcpy(d->xm, xm, sizeof(double) * d->nblk * d->ncell);
+}
+
+static int aux_on(const fo_domain *d) {
+    return d->aux.nrad > 0 || d->aux.wall_heat || d->aux.ig_type != 0;
+}
+
+void fo_aux_sources(fo_domain *d, double dt, double time) {
+    const fo_aux *a = &d->aux;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int ib = 0; ib < d->nblk; ++ib)
+        for (int i = 0; i < d->ncell; ++i) {
+            const size_t n = (size_t)ib * d->ncell + i;
+            double *q = cellp(d, d->qf, ib, i), *Q = cellp(d, d->Qc, ib, i);
+            double x[64];
+            if (a->nrad > 0) {                                      /* SRC/source.f90:85-132 */
+                fo_ckytx(d->m, q + 3, x);
+                const double p = fo_pressure(d->m, q[0], q[2], x);
+                const double Qrad = fo_radiation_heat_loss(a, p, q[2], x);
+                d->qrad[n] = Qrad;
+                Q[2] = Q[2] + dt * Qrad;
+                fo_qc_to_qf(d->m, Q, q);
+            }
+            if (a->wall_heat) {                                     /* :135-182 */
+                fo_ckytx(d->m, q + 3, x);
+                const double Qw = fo_wall_heat(d->m, a, q[2], x, d->xm[n]);
+                d->qw[n] = Qw;
+                Q[2] = Q[2] + dt * Qw;
+                fo_qc_to_qf(d->m, Q, q);
+            }
+            if (a->ig_type != 0) {                                  /* :185-236 */
+                const double Eadd = fo_ignition_power(a, time, d->xm[n]);
+                Q[2] = Q[2] + dt * Eadd;
+                fo_qc_to_qf(d->m, Q, q);
+            }
+        }
+}
+
 /* One time step of the driver loop (SRC/main.f90:130-163) with the chemistry split either the way
  * the reference driver does it (strang = 0, Lie: calc_dt, solve_fluid(dt), solve_source(dt),
  * refresh) or symmetrically (strang = 1: calc_dt, solve_source(dt/2), refresh, solve_fluid(dt),
@@ -1453,16 +1671,16 @@ void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *
 double fo_step(fo_domain *d, double cfl, int react, int strang, double rtol, double atol, double *hrr,
                double *hrr_dot, int *nfail) {
     const double dt = fo_calc_dt(d, cfl);
+    d->time = d->time + dt;                                         /* SRC/main.f90:139 */
     int nf = 0, nf2 = 0;
     if (react && strang) {
         fo_solve_source(d, dt * 0.5, rtol, atol, NULL, NULL, &nf);
         fo_refresh(d);
     }
     fo_solve_fluid(d, dt);
-    if (react) {
-        fo_solve_source(d, strang ? dt * 0.5 : dt, rtol, atol, hrr, hrr_dot, &nf2);
-        fo_refresh(d);
-    }
+    if (react) fo_solve_source(d, strang ? dt * 0.5 : dt, rtol, atol, hrr, hrr_dot, &nf2);
+    if (aux_on(d)) fo_aux_sources(d, dt, d->time);                 /* SRC/update.f90:127-147 */
+    if (react || aux_on(d)) fo_refresh(d);
     if (nfail) *nfail = nf + nf2;
     return dt;
 }
@@ -1486,6 +1704,8 @@ void fo_math_eval(int which, int n, const double *x, const double *y, double *ou
         case 1: out[i] = FO_LOG(x[i]); break;
         case 2: out[i] = FO_LOG10(x[i]); break;
         case 4: out[i] = FO_EXP10(x[i]); break;
+        case 5: out[i] = FO_TANH(x[i]); break;
+        case 6: out[i] = FO_EXPM1(x[i]); break;
         default: out[i] = FO_POW(x[i], y[i]); break;
         }
     }

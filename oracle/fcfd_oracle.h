@@ -88,6 +88,37 @@ double fo_mcavis(const fo_mech *m, double T, const double *X);                  
 double fo_mcacon(const fo_mech *m, double T, const double *X);                  /* :827 */
 void fo_mcadif(const fo_mech *m, double p, double T, const double *X, double *D); /* :973 */
 
+/* ---- auxiliary per-cell sources and open boundaries ------------------------------------------ */
+#define FO_MAX_RAD 16
+#define FO_MAX_IG 16
+typedef struct {
+    /* update_boundary (SRC/boundary.f90:12-67): 0 reflective wall, 1 inflow (left, lbound_inlet
+     * :70-99) / outflow (right, rbound_outlet :102-149); inflow state as set_flow_boundary
+     * derives it (SRC/flow_bound.f90:15-57), SI */
+    int lbound, rbound;
+    double T_in, u_in, rho_u_in, p_out;
+    double x_in[64], y_in[64];
+    /* radiation_heat_loss (SRC/radiation.f90:74-121): radiating species (0-based) and their six
+     * Planck-mean absorption coefficients */
+    int nrad;
+    int rad_idx[FO_MAX_RAD];
+    double pmac[FO_MAX_RAD][6];
+    /* heat_transfer (SRC/mfr.f90:41-78): Nusselt number, diameter (m), wall-temperature profile */
+    int wall_heat;
+    double Nu, d_m, Tw_min, Tw_max, alpha, beta;
+    /* add_igsource (1) / add_igsource_WZhang (2) (SRC/ignitionsource.f90:150-264), SI */
+    int ig_type, n_ig;
+    double E_j[FO_MAX_IG], r_m[FO_MAX_IG], t_s[FO_MAX_IG], t0_s[FO_MAX_IG], vol[FO_MAX_IG];
+} fo_aux;
+
+/* radiation_heat_loss: Qrad (W/m^3) at p (Pa), T, mole fractions X. */
+double fo_radiation_heat_loss(const fo_aux *a, double p, double T, const double *X);
+/* Tw (SRC/mfr.f90:63-78) and heat_transfer (:41-60): Qw (W/m^3) of a cell at x_m (m). */
+double fo_wall_temperature(const fo_aux *a, double x);
+double fo_wall_heat(const fo_mech *m, const fo_aux *a, double T, const double *X, double x_m);
+/* add_igsource / add_igsource_WZhang: Eadd (W/m^3) at time t of a cell at x_m. */
+double fo_ignition_power(const fo_aux *a, double t, double x_m);
+
 /* ---- multi-block wall-bounded domain (reference storage: per block (ncell+2lv) x nequ) ------ */
 typedef struct {
     const fo_mech *m;
@@ -100,6 +131,10 @@ typedef struct {
     double *Qc0, *k1, *k2, *k3, *k4;  /* interior work   */
     double *hmem;        /* nblk * ncell: the integrator's last step-size proposal (0: none) */
     int *cstats;         /* nblk * ncell * 4: steps, rejects, newton, failed of the last solve_source */
+    fo_aux aux;          /* open boundaries and auxiliary sources (all off after create)          */
+    double *xm;          /* nblk * ncell cell centres x_m (m)                                      */
+    double *qrad, *qw;   /* nblk * ncell: Qrad and Qw of the last auxiliary-source call            */
+    double time;         /* driver time, advanced by fo_step                                        */
 } fo_domain;
 
 /* Allocates and initialises from (T, p, u, X) per interior cell the way initialize_variables /
@@ -111,6 +146,29 @@ void fo_domain_free(fo_domain *d);
 double fo_calc_dt(fo_domain *d, double cfl);   /* SRC/update.f90:13-59, single rank */
 void fo_solve_fluid(fo_domain *d, double dt);  /* SRC/update.f90:62-97 -> timeintegration.f90 */
 void fo_refresh(fo_domain *d);                 /* post-step BC/halo/QcToqf, SRC/main.f90:149-163 */
+
+/* Inflow / outflow boundaries: flag_lbound 0 wall, 1 inflow at T_in (K), u_in (cm/s) and mole
+ * fractions x_in[KK]; flag_rbound 0 wall, 1 outflow at p_out (atm) -- the inputs of
+ * set_flow_boundary (SRC/flow_bound.f90:34-54).  Re-fills the ghost cells the way the driver does
+ * at start-up.  Returns 0, or -1 for an invalid flag. */
+int fo_domain_set_flow_boundary(fo_domain *d, int flag_lbound, int flag_rbound, double T_in,
+                                double u_in_cm_s, const double *x_in, double p_out_atm);
+/* Radiation (set_radiation_params, SRC/radiation.f90:21-71): n species (0-based) with six
+ * coefficients each (row-major n x 6); n = 0 turns it off.  Returns 0 or -1 (n too large). */
+int fo_domain_set_radiation(fo_domain *d, int n, const int *species, const double *pmac);
+/* Wall heat of the micro flow reactor (set_wallT_boundary, SRC/mfr.f90:21-38): d in mm. */
+void fo_domain_set_wall_heat(fo_domain *d, int on, double Nu, double d_mm, double Tw_min,
+                             double Tw_max, double alpha, double beta);
+/* Ignition sources (set_igsource / set_multi_igsource, SRC/ignitionsource.f90:40-147): type 1
+ * Frendi, 2 Zhang, 0 off; energy mJ, radius um, duration and start us.  Returns 0 or -1. */
+int fo_domain_set_ignition(fo_domain *d, int type, int n, const double *E_mJ, const double *r_um,
+                           const double *t_us, const double *t0_us);
+/* Cell centres (nblk * ncell, m); create sets the uniform block layout (SRC/block.f90:264-305). */
+void fo_domain_set_xm(fo_domain *d, const double *xm);
+/* The auxiliary sources of solve_source (SRC/update.f90:127-147) on the interior cells, in the
+ * reference's order: radiation, wall heat, ignition, each adding dt * source to rho E and
+ * recovering the primitives (SRC/source.f90:85-239).  time: the driver's time after the step. */
+void fo_aux_sources(fo_domain *d, double dt, double time);
 
 /* ---- CKLIB kinetics -------------------------------------------------------------------------- */
 /* CKWYR (cklib.f:9107-9204): rho g/cm^3, T K, Y -> wdot mol/(cm^3 s).  qf_/qr_ (may be NULL):
@@ -135,11 +193,13 @@ int fo_chem_step(const fo_mech *m, double dt, double rho, double *yT, double rto
 void fo_solve_source(fo_domain *d, double dt, double rtol, double atol, double *hrr,
                      double *hrr_dot, int *nfail);
 
-/* exp (0), log (1), log10 (2), pow (3), 10^x (4) of the build's math on n arguments */
+/* exp (0), log (1), log10 (2), pow (3), 10^x (4), tanh (5), expm1 (6) of the build's math on n
+ * arguments */
 void fo_math_eval(int which, int n, const double *x, const double *y, double *out);
 
 /* One driver time step (SRC/main.f90:130-163): Lie (strang = 0, the reference's order) or Strang
- * splitting of the chemistry; returns dt. */
+ * splitting of the chemistry, then the auxiliary sources that are on (with the full dt, after the
+ * chemistry); advances d->time by dt and returns dt. */
 double fo_step(fo_domain *d, double cfl, int react, int strang, double rtol, double atol, double *hrr,
                double *hrr_dot, int *nfail);
 /* OpenMP threads of the parallel loops (1 without OpenMP). */

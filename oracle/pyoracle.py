@@ -33,6 +33,8 @@ def _lib(libm=False):
     lib = C.CDLL(path)
     lib.fo_mech_load.restype = C.c_void_p
     lib.fo_mech_load.argtypes = [C.c_char_p, C.c_char_p]
+    for name in ["fo_radiation_heat_loss", "fo_wall_temperature", "fo_wall_heat", "fo_ignition_power"]:
+        getattr(lib, name).restype = C.c_double
     for name in ["fo_pressure", "fo_density", "fo_specific_energy", "fo_cp", "fo_cv",
                  "fo_sonic_speed", "fo_temperature_con", "fo_heat_release_rate",
                  "fo_heat_release_rate_omegadot", "fo_mcavis", "fo_mcacon", "fo_calc_dt", "fo_step"]:
@@ -57,6 +59,24 @@ class FoDomainStats(C.Structure):
     """fo_domain up to its per-cell integrator counters."""
     _fields_ = FoDomain._fields_ + [("Qc0", _D), ("k1", _D), ("k2", _D), ("k3", _D), ("k4", _D),
                                     ("hmem", _D), ("cstats", _I)]
+
+
+class FoAux(C.Structure):
+    """fo_aux: open boundaries and auxiliary sources of a domain."""
+    _fields_ = [("lbound", C.c_int), ("rbound", C.c_int), ("T_in", C.c_double), ("u_in", C.c_double),
+                ("rho_u_in", C.c_double), ("p_out", C.c_double), ("x_in", C.c_double * 64),
+                ("y_in", C.c_double * 64), ("nrad", C.c_int), ("rad_idx", C.c_int * 16),
+                ("pmac", (C.c_double * 6) * 16), ("wall_heat", C.c_int), ("Nu", C.c_double),
+                ("d_m", C.c_double), ("Tw_min", C.c_double), ("Tw_max", C.c_double),
+                ("alpha", C.c_double), ("beta", C.c_double), ("ig_type", C.c_int), ("n_ig", C.c_int),
+                ("E_j", C.c_double * 16), ("r_m", C.c_double * 16), ("t_s", C.c_double * 16),
+                ("t0_s", C.c_double * 16), ("vol", C.c_double * 16)]
+
+
+class FoDomainAux(C.Structure):
+    """fo_domain in full."""
+    _fields_ = FoDomainStats._fields_ + [("aux", FoAux), ("xm", _D), ("qrad", _D), ("qw", _D),
+                                         ("time", C.c_double)]
 
 
 def _p(a):
@@ -203,9 +223,9 @@ class Oracle:
     # ---- flux -----------------------------------------------------------------------------
     @staticmethod
     def opts(flag_acu=2, flag_weno=2, flag_muscl=0, select_riemann=2, select_time=1, dx_comp=1e-3,
-             do_tran=0):
-        return FoOpts(flag_acu, flag_weno, flag_muscl, select_riemann, select_time, 0, 1, do_tran,
-                      dx_comp)
+             do_tran=0, is_spherical=0):
+        return FoOpts(flag_acu, flag_weno, flag_muscl, select_riemann, select_time, is_spherical, 1,
+                      do_tran, dx_comp)
 
     def flux_advection(self, qf, n, lv, dx, **kw):
         o = self.opts(dx_comp=dx, **kw)
@@ -342,6 +362,62 @@ class Domain:
         if nf.value:
             raise RuntimeError(f"oracle: {nf.value} cells failed to integrate")
         return (dt, hrr, hrd) if diagnostics else dt
+
+    # ---- open boundaries and auxiliary sources ----------------------------------------------
+    @property
+    def full(self):
+        return FoDomainAux.from_address(self.ptr.value)
+
+    def set_flow_boundary(self, flag_lbound, flag_rbound, T_in=300.0, u_in_cm_s=0.0, x_in=None,
+                          p_out_atm=1.0):
+        x = _f64(np.zeros(self.nequ - 3) if x_in is None else x_in)
+        rc = self.orc.lib.fo_domain_set_flow_boundary(self.ptr, C.c_int(flag_lbound), C.c_int(flag_rbound),
+                                                      C.c_double(T_in), C.c_double(u_in_cm_s), _p(x),
+                                                      C.c_double(p_out_atm))
+        if rc:
+            raise ValueError("oracle: invalid boundary flags")
+
+    def set_radiation(self, species, pmac):
+        sp = np.ascontiguousarray(species, np.int32)
+        pm = _f64(pmac).reshape(-1)
+        if self.orc.lib.fo_domain_set_radiation(self.ptr, C.c_int(sp.size), sp.ctypes.data_as(_I), _p(pm)):
+            raise ValueError("oracle: invalid radiation parameters")
+
+    def set_wall_heat(self, Nu, d_mm, Tw_min, Tw_max, alpha, beta, on=True):
+        self.orc.lib.fo_domain_set_wall_heat(self.ptr, C.c_int(int(on)), *(C.c_double(v) for v in
+                                             (Nu, d_mm, Tw_min, Tw_max, alpha, beta)))
+
+    def set_ignition(self, kind, E_mJ, r_um, t_us, t0_us):
+        arrs = [_f64(np.atleast_1d(v)) for v in (E_mJ, r_um, t_us, t0_us)]
+        if self.orc.lib.fo_domain_set_ignition(self.ptr, C.c_int(kind), C.c_int(arrs[0].size),
+                                               *(_p(a) for a in arrs)):
+            raise ValueError("oracle: invalid ignition parameters")
+
+    @property
+    def xm(self):
+        return np.ctypeslib.as_array(self.full.xm, shape=(self.nblk * self.ncell,)).copy()
+
+    def set_xm(self, xm):
+        x = _f64(xm)
+        assert x.size == self.nblk * self.ncell
+        self.orc.lib.fo_domain_set_xm(self.ptr, _p(x))
+
+    @property
+    def time(self):
+        return self.full.time
+
+    def set_time(self, t):
+        self.full.time = t
+
+    def aux_sources(self, dt, time):
+        self.orc.lib.fo_aux_sources(self.ptr, C.c_double(dt), C.c_double(time))
+
+    def aux_outputs(self):
+        """Qrad and Qw (W/m^3) of the last auxiliary-source call, per interior cell."""
+        n = self.nblk * self.ncell
+        f = self.full
+        return (np.ctypeslib.as_array(f.qrad, shape=(n,)).copy(),
+                np.ctypeslib.as_array(f.qw, shape=(n,)).copy())
 
     def chem_stats(self):
         """Per interior cell (block-major): steps, rejects, newton, failed of the last

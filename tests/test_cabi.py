@@ -40,6 +40,10 @@ def test_bad_paths_and_options_fail_loudly():
         Solver(m, 1, 16, 1e-3, flag_acu=2, flag_weno=7)
     with pytest.raises(FcfdError, match="select_time"):
         Solver(m, 1, 16, 1e-3, select_time=3)
+    with pytest.raises(FcfdError, match="flag_lbound"):
+        Solver(m, 1, 16, 1e-3, flag_lbound=2)
+    with pytest.raises(FcfdError, match="inflow / outflow"):
+        Solver(m, 1, 16, 1e-3, select_time=2, flag_rbound=1)
     h = C.c_void_p()
     assert lib().fcfd_mech_load(None, None, C.byref(h)) == -2
 

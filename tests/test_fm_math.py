@@ -135,3 +135,28 @@ def test_special_values(orc):
 def test_libm_build_is_the_c_library(orc_libm):
     x = sample_exp(1000)
     assert np.array_equal(fm(orc_libm, 0, x), [math.exp(v) for v in x])
+
+
+def test_expm1_and_tanh(orc):
+    """expm1 and tanh (the wall-temperature profile of the micro flow reactor) against the C
+    library and against exact values, over the reduction's branches: |x| < ln2/2, the scaled range,
+    the saturation ends; tanh on both signs, tiny arguments and the cut-off at 22."""
+    n = 100000
+    x = RNG.uniform(-40.0, 40.0, n)
+    x[: n // 4] = RNG.uniform(-0.35, 0.35, n // 4)
+    x[n // 4: n // 3] = np.ldexp(RNG.uniform(-1.0, 1.0, n // 3 - n // 4), RNG.integers(-60, -1, n // 3 - n // 4))
+    assert ulps(fm(orc, 6, x), [math.expm1(v) for v in x]).max() <= 2
+    ex = [Decimal(v).exp() - 1 for v in x[:3000]]
+    assert exact_ulp_err(fm(orc, 6, x[:3000]), ex).max() < 1.5
+    t = RNG.uniform(-25.0, 25.0, n)
+    t[: n // 4] = RNG.uniform(-0.5, 0.5, n // 4)
+    t[n // 4: n // 3] = np.ldexp(RNG.uniform(-1.0, 1.0, n // 3 - n // 4), RNG.integers(-60, -1, n // 3 - n // 4))
+    got = fm(orc, 5, t)
+    assert ulps(np.abs(got), [abs(math.tanh(v)) for v in t]).max() <= 3
+    assert (np.sign(got) == np.sign(t)).all()
+    def dtanh(v):
+        e2 = (2 * Decimal(v)).exp()
+        return (e2 - 1) / (e2 + 1)
+    assert exact_ulp_err(fm(orc, 5, t[:3000]), [dtanh(v) for v in t[:3000]]).max() < 2.5
+    assert list(fm(orc, 5, [0.0, 22.5, -30.0, math.inf, -math.inf])) == [0.0, 1.0, -1.0, 1.0, -1.0]
+    assert list(fm(orc, 6, [0.0, -50.0, math.inf])) == [0.0, -1.0, math.inf]
