@@ -120,6 +120,13 @@ struct fcfd_ctx {
     AuxParams *d_aux = nullptr;
     double *d_xm = nullptr;              // [nblk * ncell] cell centres, m
     double *d_qrad = nullptr, *d_qw = nullptr;
+    // stretched mesh (fcfd_create_smr): per-block cell size, coarse block counts at each end, the
+    // left block of every pair of unequal blocks
+    double *d_dxb = nullptr;             // nullptr: dx in every block
+    std::vector<double> h_dxb;
+    int smr_left = 0, smr_right = 0;
+    int *d_smr_ifb = nullptr;
+    int n_smr_if = 0;
 };
 
 namespace fcfd {
@@ -219,9 +226,20 @@ StageArgs stage_args(fcfd_ctx *c, int mode, const double *Q0, const double *Qin,
     StageArgs a{};
     a.thermo = c->thermo; a.Q0 = Q0; a.Qin = Qin; a.F = c->F; a.Qout = Qout; a.qf = c->qf;
     a.dt_dev = dt_dev; a.dt = dt; a.dx = c->dx; a.h1 = h1; a.h2 = h2; a.mode = mode; a.ghosts_only = 0;
-    a.spherical = c->o.is_spherical; a.xm = c->d_xm; a.god = c->god;
+    a.spherical = c->o.is_spherical; a.xm = c->d_xm; a.god = c->god; a.dxb = c->d_dxb;
     a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P; a.PF = c->PF;
     return a;
+}
+
+// The halos between blocks of unequal cell size, over the plain copies a stage or refresh has
+// just written.
+hipError_t smr_halo(fcfd_ctx *c, double *Q) {
+    if (c->n_smr_if == 0) return hipSuccess;
+    SmrArgs a{};
+    a.thermo = c->thermo; a.Q = Q; a.qf = c->qf; a.ifb = c->d_smr_ifb; a.dxb = c->d_dxb;
+    a.nif = c->n_smr_if; a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.nequ = c->nequ;
+    a.P = c->P;
+    return launch_smr_halo(a, c->kk, c->stream);
 }
 
 // The inflow / outflow ghost cells over the wall ghosts a stage or refresh has just written.
@@ -246,16 +264,34 @@ hipError_t enqueue_fluid(fcfd_ctx *c, double dt, const double *dt_dev) {
             TranArgs ta{};
             ta.thermo = c->thermo; ta.tran = c->d_tran; ta.Qc = Qin; ta.props = c->props; ta.F = c->F;
             ta.nblk = c->nblk; ta.ncell = c->ncell; ta.lv = c->lv; ta.S = c->S; ta.P = c->P; ta.PF = c->PF;
-            ta.PP = c->PP; ta.dx = c->dx;
+            ta.PP = c->PP; ta.dx = c->dx; ta.dxb = c->d_dxb;
             hipError_t e2 = timed(c, 7, [&]() { return launch_flux_tran(ta, nsp, c->stream); });
             if (e2 != hipSuccess) return e2;
         }
         if (c->o.do_hydro != 1) return hipSuccess;
-        return timed(c, 0, [&]() { return launch_flux(fa, nsp, c->o.flag_acu, c->o.flag_weno, c->stream); });
+        if (!c->d_dxb) return timed(c, 0, [&]() { return launch_flux(fa, nsp, c->o.flag_acu, c->o.flag_weno, c->stream); });
+        // refined mesh: the blocks at dx_comp with the configured reconstruction, the coarser
+        // blocks at first order (SRC/flux.f90:240)
+        return timed(c, 0, [&]() {
+            const int nl = c->smr_left, nr = c->smr_right, nf = c->nblk - nl - nr;
+            FluxArgs f = fa;
+            f.b0 = nl; f.nblk = nf;
+            hipError_t e2 = nf > 0 ? launch_flux(f, nsp, c->o.flag_acu, c->o.flag_weno, c->stream) : hipSuccess;
+            if (e2 == hipSuccess && nl > 0) {
+                f.b0 = 0; f.nblk = nl;
+                e2 = launch_flux(f, nsp, 0, c->o.flag_weno, c->stream);
+            }
+            if (e2 == hipSuccess && nr > 0) {
+                f.b0 = c->nblk - nr; f.nblk = nr;
+                e2 = launch_flux(f, nsp, 0, c->o.flag_weno, c->stream);
+            }
+            return e2;
+        });
     };
     auto stage = [&](const StageArgs &s) {
         return timed(c, 1, [&]() {
             hipError_t e2 = launch_stage_end(s, nsp, c->stream);
+            if (e2 == hipSuccess) e2 = smr_halo(c, s.Qout);
             return e2 != hipSuccess ? e2 : open_bc(c, s.Qout);
         });
     };
@@ -306,6 +342,7 @@ hipError_t enqueue_refresh(fcfd_ctx *c) {
     StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
     return timed(c, 1, [&]() {
         hipError_t e = launch_stage_end(s, c->kk, c->stream);
+        if (e == hipSuccess) e = smr_halo(c, c->Q[c->cur]);
         return e != hipSuccess ? e : open_bc(c, c->Q[c->cur]);
     });
 }
@@ -567,8 +604,11 @@ int fcfd_mech_dims(const fcfd_mech *m, int *kk, int *ii) {
     return FCFD_OK;
 }
 
-int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, double dx, fcfd_ctx **out) {
-    if (!m || !o || !out || nblk < 1 || ncell < 1 || !(dx > 0.0)) { set_error("fcfd_create: invalid argument"); return FCFD_E_ARG; }
+namespace {
+
+// Shared by fcfd_create (dxb empty: dx in every block) and fcfd_create_smr.
+int create_impl(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, double dx,
+                const std::vector<double> &dxb, const std::vector<double> &xm_in, fcfd_ctx **out) {
     *out = nullptr;
     int rc = check_opts(*o, m->host.kk);
     if (rc) return rc;
@@ -599,6 +639,21 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
     HIP_TRY(hipMalloc(&c->d_cvel, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->d_cvel, 0, sizeof(unsigned long long)));
     HIP_TRY(hipMemset(c->d_dt, 0, sizeof(double)));
+    if (!dxb.empty()) {
+        c->h_dxb = dxb;
+        HIP_TRY(hipMalloc(&c->d_dxb, sizeof(double) * nblk));
+        HIP_TRY(hipMemcpy(c->d_dxb, dxb.data(), sizeof(double) * nblk, hipMemcpyHostToDevice));
+        std::vector<int> ifb;
+        for (int b = 0; b + 1 < nblk; ++b)
+            if (dxb[b] != dxb[b + 1]) ifb.push_back(b);
+        c->n_smr_if = int(ifb.size());
+        if (!ifb.empty()) {
+            HIP_TRY(hipMalloc(&c->d_smr_ifb, sizeof(int) * ifb.size()));
+            HIP_TRY(hipMemcpy(c->d_smr_ifb, ifb.data(), sizeof(int) * ifb.size(), hipMemcpyHostToDevice));
+        }
+        while (c->smr_left < nblk && dxb[c->smr_left] != dx) ++c->smr_left;
+        while (c->smr_right < nblk - c->smr_left && dxb[nblk - 1 - c->smr_right] != dx) ++c->smr_right;
+    }
     {
         // cell centres of the uniform block layout (SRC/block.f90:264-305 without refinement): a
         // running sum of dx from 0, shifted by the half cell length_smr = -dx/2
@@ -609,6 +664,7 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
             if (g > 0) xl = xl + dx;
             xm[g] = xl - length_smr;
         }
+        if (!xm_in.empty()) xm = xm_in;
         HIP_TRY(hipMalloc(&c->d_xm, sizeof(double) * xm.size()));
         HIP_TRY(hipMemcpy(c->d_xm, xm.data(), sizeof(double) * xm.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMalloc(&c->d_aux, sizeof(AuxParams)));
@@ -653,6 +709,76 @@ int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, dou
     return FCFD_OK;
 }
 
+}  // namespace
+
+int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, double dx, fcfd_ctx **out) {
+    if (!m || !o || !out || nblk < 1 || ncell < 1 || !(dx > 0.0)) { set_error("fcfd_create: invalid argument"); return FCFD_E_ARG; }
+    return create_impl(m, o, nblk, ncell, dx, {}, {}, out);
+}
+
+int fcfd_create_smr(const fcfd_mech *m, const fcfd_opts *o, int nblk_comp, int ncell, double dx_comp,
+                    int lev_smr_0, int num_smr_0, int lev_smr_n, int num_smr_n, fcfd_ctx **out) {
+    if (!m || !o || !out || nblk_comp < 1 || ncell < 1 || !(dx_comp > 0.0) || lev_smr_0 < 1 || lev_smr_n < 1 ||
+        num_smr_0 < 0 || num_smr_n < 0 || lev_smr_0 > 40 || lev_smr_n > 40) {
+        set_error("fcfd_create_smr: invalid argument");
+        return FCFD_E_ARG;
+    }
+    if (ncell < 2 * lvir_of(*o)) { set_error("fcfd_create_smr: ncell must be at least twice lvir"); return FCFD_E_ARG; }
+    if (o->select_time == 2) { set_error("fcfd_create_smr: a refined mesh is supported with select_time 0 or 1"); return FCFD_E_UNSUPPORTED; }
+    const int n0 = (lev_smr_0 - 1) * num_smr_0, nn = (lev_smr_n - 1) * num_smr_n;
+    const int nblk = nblk_comp + n0 + nn;                         // SRC/block.f90:136-146
+    // block spacings (:160-218): the left refinement blocks coarsest first, the right ones coarsest
+    // last, num_smr blocks per level, doubling per level away from the fine region
+    std::vector<double> dxb(nblk, dx_comp);
+    double length_smr_0 = 0.0;
+    int b = 0;
+    for (int lev = 2; lev <= lev_smr_0; ++lev)
+        for (int k = 0; k < num_smr_0; ++k, ++b) {
+            dxb[b] = dx_comp * double(1LL << (lev_smr_0 - lev + 1));
+            length_smr_0 = length_smr_0 + dxb[b] * ncell;
+        }
+    b = 0;
+    for (int lev = 2; lev <= lev_smr_n; ++lev)
+        for (int k = 0; k < num_smr_n; ++k, ++b) dxb[nblk - 1 - b] = dx_comp * double(1LL << (lev_smr_n - lev + 1));
+    // cell centres (:264-305): a running position stepping by each cell's dx, half a coarse cell
+    // further on a coarse -> fine step and a quarter back on fine -> coarse, shifted so that the
+    // left refinement region ends at x = 0
+    std::vector<double> xm(size_t(nblk) * ncell);
+    double xl = 0.0, dxl = 0.0;
+    for (int ib = 0; ib < nblk; ++ib)
+        for (int i = 0; i < ncell; ++i) {
+            const double h = dxb[ib];
+            double &x = xm[size_t(ib) * ncell + i];
+            if (ib == 0 && i == 0) {
+                x = 0.0; xl = 0.0; dxl = 0.0;
+            } else {
+                x = xl + h;
+                xl = xl + h;
+                if (dxl > h) { x = x + h / 2.0; xl = xl + h / 2.0; }
+                if (dxl < h) { x = x - h / 4.0; xl = xl - h / 4.0; }
+            }
+            dxl = h;
+        }
+    const double length_smr = length_smr_0 - dxb[0] / 2.0;
+    for (auto &x : xm) x = x - length_smr;
+    return create_impl(m, o, nblk, ncell, dx_comp, dxb, xm, out);
+}
+
+int fcfd_nblocks(const fcfd_ctx *c) { return c ? c->nblk : FCFD_E_ARG; }
+
+int fcfd_block_dx(const fcfd_ctx *c, double *dx_block) {
+    if (!c || !dx_block) { set_error("fcfd_block_dx: invalid argument"); return FCFD_E_ARG; }
+    for (int b = 0; b < c->nblk; ++b) dx_block[b] = c->h_dxb.empty() ? c->dx : c->h_dxb[b];
+    return FCFD_OK;
+}
+
+int fcfd_get_cell_centres(fcfd_ctx *c, double *xm) {
+    if (!c || !xm) { set_error("fcfd_get_cell_centres: invalid argument"); return FCFD_E_ARG; }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(xm, c->d_xm, sizeof(double) * size_t(c->nblk) * c->ncell, hipMemcpyDeviceToHost));
+    return FCFD_OK;
+}
+
 void fcfd_destroy(fcfd_ctx *c) {
     if (!c) return;
     drop_graphs(c);
@@ -665,6 +791,7 @@ void fcfd_destroy(fcfd_ctx *c) {
     (void)hipFree(c->d_list2); (void)hipFree(c->d_list2_n); (void)hipFree(c->d_retry); (void)hipFree(c->d_f0);
     (void)hipFree(c->d_tran); (void)hipFree(c->props); (void)hipFree(c->acc); (void)hipFree(c->d_dth);
     (void)hipFree(c->d_bc); (void)hipFree(c->d_aux); (void)hipFree(c->d_xm); (void)hipFree(c->d_qrad); (void)hipFree(c->d_qw);
+    (void)hipFree(c->d_dxb); (void)hipFree(c->d_smr_ifb);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -736,6 +863,7 @@ int fcfd_init_cells(fcfd_ctx *c, const double *cells) {
         return FCFD_E_ARG;
     }
     HIP_TRY(launch_stage_end(s, c->kk, c->stream));
+    HIP_TRY(smr_halo(c, c->Q[c->cur]));
     HIP_TRY(open_bc(c, c->Q[c->cur]));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return FCFD_OK;
@@ -892,6 +1020,7 @@ int fcfd_set_flow_boundary(fcfd_ctx *c, double T_in, double u_in_cm_s, const dou
     StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
     s.ghosts_only = 1;
     HIP_TRY(launch_stage_end(s, c->kk, c->stream));
+    HIP_TRY(smr_halo(c, c->Q[c->cur]));
     HIP_TRY(open_bc(c, c->Q[c->cur]));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return FCFD_OK;

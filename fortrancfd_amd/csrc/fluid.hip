@@ -84,7 +84,10 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
     const int s = int(threadIdx.x & 1);                   // 0: left state, 1: right state
     const int nif = a.ncell + 1;
     if (g >= long(a.nblk) * nif) return;                  // both lanes of a pair leave together
-    const int b = int(g / nif), j = int(g % nif);
+    // a launch covers blocks b0 .. b0 + nblk - 1 (the coarse blocks of a refined mesh reconstruct
+    // at first order in launches of their own, SRC/flux.f90:240)
+    const int b = a.b0 + int(g / nif), j = int(g % nif);
+    const long gi = long(b) * nif + j;                    // interface index over all blocks
     const long c0 = long(b) * a.S + a.lv + j - 1;      // left cell of the interface
     const DevThermo *__restrict__ m = a.thermo;
     const double *__restrict__ q = a.qf;
@@ -115,7 +118,7 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
     const double fl1 = SPH ? L.rho * L.u * L.u : L.rho * L.u * L.u + L.p;
     const double fr1 = SPH ? R.rho * R.u * R.u : R.rho * R.u * R.u + R.p;
     const double fl2 = (L.E + L.p) * L.u, fr2 = (R.E + R.p) * R.u;
-    double *__restrict__ F = a.F + g;
+    double *__restrict__ F = a.F + gi;
     const long PF = a.PF;
     // Fplus accumulates (SRC/flux.f90:358): on top of the transport flux when there is one
     const bool acc = a.accumulate != 0;
@@ -195,7 +198,7 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
     }
     if (a.godunov && s == 0) {                                    // SRC/flux.f90:360-367
         const double rt = sqrt(R.rho / L.rho);
-        double *G = a.godunov + g;
+        double *G = a.godunov + gi;
         G[0] = sqrt(R.rho * L.rho);
         G[PF] = (L.u + rt * R.u) / (1.0 + rt);
         G[2 * PF] = (L.p + rt * R.p) / (1.0 + rt);
@@ -255,17 +258,18 @@ __global__ void __launch_bounds__(64, 2) k_stage_end(StageArgs a) {
     const long sc = long(sb) * a.S + a.lv + si;
     const long fj = long(sb) * (a.ncell + 1) + si;
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
-    const double dtdx = dt / a.dx;
+    const double dx = a.dxb ? a.dxb[sb] : a.dx;         // the source cell's block spacing
+    const double dtdx = dt / dx;
     // spherical (SRC/timeintegration.f90:58-72, 177-196): face areas (r -+ dx/2)^2 and volume
     // r^2 dx at the source cell's centre
     const bool sph = a.spherical && a.mode != 2;
     double Al = 0.0, Ar = 0.0, dtdV = 0.0;
     if (sph) {
         const double rm = a.xm[long(sb) * a.ncell + si];
-        const double tl = rm - (a.dx / 2.0), tr = rm + (a.dx / 2.0);
+        const double tl = rm - (dx / 2.0), tr = rm + (dx / 2.0);
         Al = tl * tl;
         Ar = tr * tr;
-        dtdV = dt / (rm * rm * a.dx);
+        dtdV = dt / (rm * rm * dx);
     }
     double Qv[NSP + 3];
 #pragma unroll
@@ -318,14 +322,15 @@ __global__ void __launch_bounds__(64) k_rk4_stage(StageArgs a) {
     const long sc = long(sb) * a.S + a.lv + si;
     const long fj = long(sb) * (a.ncell + 1) + si;
     const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const double dx = a.dxb ? a.dxb[sb] : a.dx;
 #pragma unroll
     for (int v = 0; v < NSP + 3; ++v) {
         const double d = a.F[v * a.PF + fj + 1] - a.F[v * a.PF + fj];
-        const double k = -(dt / a.dx) * d;
+        const double k = -(dt / dx) * d;
         const double qi = a.Qin[v * a.P + sc];
         double Q;
-        if (st < 2) Q = qi - 0.5 * (0.5 * dt / a.dx) * d;
-        else if (st == 2) Q = qi - (dt / a.dx) * d;
+        if (st < 2) Q = qi - 0.5 * (0.5 * dt / dx) * d;
+        else if (st == 2) Q = qi - (dt / dx) * d;
         else Q = a.Q0[v * a.P + sc] + (1.0 / 6.0) * (a.acc[v * a.P + sc] + k);
         if (interior && st < 3) {
             const double prev = st == 0 ? 0.0 : a.acc[v * a.P + c];
@@ -336,6 +341,56 @@ __global__ void __launch_bounds__(64) k_rk4_stage(StageArgs a) {
         a.Qout[v * a.P + c] = Q;
     }
     if (st == 3 || lwall) qc_to_qf_store<NSP>(a.thermo, Qv, a.qf, a.P, c);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Halo between blocks of unequal cell size (interpolation_block, SRC/block.f90:499-563), after the
+// kernel that filled every halo as a plain copy.  k_smr_fill: one thread per (block pair,
+// variable): the coarse block's lv ghosts average pairs of fine cells, then the fine block's lv
+// ghosts take the coarse cell they lie in, -+ a quarter of its minmod slope (the slope stencil
+// reaches the coarse ghost just written).  k_smr_qf: QcToqf on those ghost columns.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double smr_slope(double m, double p) {
+    return 0.5 * (copysign(1.0, m) + copysign(1.0, p)) * fmin(fabs(m), fabs(p));
+}
+
+__global__ void __launch_bounds__(64) k_smr_fill(SmrArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nif * a.nequ) return;
+    const int ib = a.ifb[t / a.nequ], v = t % a.nequ;
+    const long L = long(ib) * a.S + a.lv, R = long(ib + 1) * a.S + a.lv;   // interior cell 0 of each
+    const int nc = a.ncell, lv = a.lv;
+    double *__restrict__ Q = a.Q + long(v) * a.P;
+    if (a.dxb[ib] > a.dxb[ib + 1]) {                   // coarse | fine
+        for (int g = 1; g <= lv; ++g)
+            Q[L + nc - 1 + g] = (Q[R + 2 * (g - 1)] + Q[R + 2 * (g - 1) + 1]) / 2.0;
+        for (int g = 1; g <= lv; ++g) {
+            const long c = L + nc - 1 - (g - 1) / 2;
+            const double s = smr_slope(Q[c] - Q[c - 1], Q[c + 1] - Q[c]);
+            Q[R - g] = (g % 2 == 1) ? Q[c] + 0.25 * s : Q[c] - 0.25 * s;
+        }
+    } else {                                            // fine | coarse
+        for (int g = 1; g <= lv; ++g)
+            Q[R - g] = (Q[L + nc - 1 - 2 * (g - 1)] + Q[L + nc - 1 - (2 * (g - 1) + 1)]) / 2.0;
+        for (int g = 1; g <= lv; ++g) {
+            const long c = R + (g - 1) / 2;
+            const double s = smr_slope(Q[c] - Q[c - 1], Q[c + 1] - Q[c]);
+            Q[L + nc - 1 + g] = (g % 2 == 1) ? Q[c] - 0.25 * s : Q[c] + 0.25 * s;
+        }
+    }
+}
+
+template <int NSP>
+__global__ void __launch_bounds__(64) k_smr_qf(SmrArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nif * 2 * a.lv) return;
+    const int ib = a.ifb[t / (2 * a.lv)], g = t % (2 * a.lv);
+    // the left block's right ghosts, then the right block's left ghosts
+    const long c = g < a.lv ? long(ib) * a.S + a.lv + a.ncell + g : long(ib + 1) * a.S + (g - a.lv);
+    double Qv[NSP + 3];
+#pragma unroll
+    for (int v = 0; v < NSP + 3; ++v) Qv[v] = a.Q[v * a.P + c];
+    qc_to_qf_store<NSP>(a.thermo, Qv, a.qf, a.P, c);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -527,6 +582,18 @@ static hipError_t launch_flux_nsp(const FluxArgs &a, int rk, hipStream_t st) {
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_smr_halo(const SmrArgs &a, int nsp, hipStream_t st) {
+    if (a.nif == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_smr_fill, dim3(unsigned((a.nif * a.nequ + 63) / 64)), dim3(64), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 grid(unsigned((a.nif * 2 * a.lv + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_smr_qf<N>), grid, blk, 0, st, a); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_flux(const FluxArgs &a, int nsp, int flag_acu, int flag_weno, hipStream_t st) {

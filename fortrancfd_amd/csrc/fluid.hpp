@@ -15,6 +15,7 @@ struct FluxArgs {
     int muscl, riemann;
     int accumulate;       // 1: F += advection flux (transport flux already in F), 0: F = flux
     int spherical;        // 1: momentum flux rho u u without p (SRC/flux.f90:272-278)
+    int b0;               // first block of this launch (nblk blocks from there)
 };
 
 struct StageArgs {
@@ -29,6 +30,7 @@ struct StageArgs {
     int spherical;         // 1: r^2-weighted update with the pressure-gradient term (needs xm, god)
     const double *xm;      // [nblk * ncell] cell centres (spherical)
     const double *god;     // [3][PF] Roe states of the interfaces; [2] the pressure (spherical)
+    const double *dxb;     // [nblk] per-block cell size (nullptr: dx everywhere)
     int nblk, ncell, lv, S;
     long P, PF;
 };
@@ -52,7 +54,19 @@ struct InitArgs {
     long P;
 };
 
+// Halo of blocks with unequal cell sizes (interpolation_block's refinement branches,
+// SRC/block.f90:499-563): ifb lists the left block of each such block pair.
+struct SmrArgs {
+    const DevThermo *thermo;
+    double *Q, *qf;
+    const int *ifb;
+    const double *dxb;
+    int nif, nblk, ncell, lv, S, nequ;
+    long P;
+};
+
 hipError_t launch_flux(const FluxArgs &a, int nsp, int flag_acu, int flag_weno, hipStream_t st);
+hipError_t launch_smr_halo(const SmrArgs &a, int nsp, hipStream_t st);
 hipError_t launch_stage_end(const StageArgs &a, int nsp, hipStream_t st);
 hipError_t launch_rk4_stage(const StageArgs &a, int nsp, hipStream_t st);
 hipError_t launch_dt(const DtArgs &a, int nsp, hipStream_t st);

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(64) k_flux_tran(TranArgs a) {
     const double rho_a = 0.5 * (rho0 + rho1), u_a = 0.5 * (u0 + u1);
     const double T_a = 0.5 * (T0 + T1), mu_a = 0.5 * (P[3 * PP + c0] + P[3 * PP + c1]);
     const double kap_a = 0.5 * (P[4 * PP + c0] + P[4 * PP + c1]);
-    const double dx = a.dx;
+    const double dx = a.dxb ? a.dxb[b] : a.dx;                // b_obj(ib)%smr_dx
     const double tau = -(2.0 / 3.0 * mu_a * (2.0 * ((u1 - u0) / dx)));
     const DevThermo *__restrict__ th = a.thermo;
     const TPow tp = tpow(T_a, th->ru);

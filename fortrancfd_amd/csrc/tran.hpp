@@ -27,6 +27,7 @@ struct TranArgs {
     int nblk, ncell, lv, S;
     long P, PF, PP;
     double dx;
+    const double *dxb;     // [nblk] per-block cell size (nullptr: dx everywhere)
 };
 
 hipError_t launch_flux_tran(const TranArgs &a, int nsp, hipStream_t st);

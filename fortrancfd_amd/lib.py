@@ -223,12 +223,19 @@ class Solver:
     """A multi-block wall-bounded domain on the GPU: the state b_obj(:)%Qc/%qf of the reference
     (SRC/block.f90:10-23) held on the device, advanced by calc_dt / solve_fluid / refresh."""
 
-    def __init__(self, mech: Mechanism, nblk: int, ncell: int, dx: float, **kw):
+    def __init__(self, mech: Mechanism, nblk: int, ncell: int, dx: float, smr=None, **kw):
+        """smr = (lev_smr_0, num_smr_0, lev_smr_n, num_smr_n): the reference's stretched-mesh
+        layout around the nblk blocks at dx (self.nblk then counts every block)."""
         self.mech = mech
         self.o = opts(**kw)
         self.handle = C.c_void_p()
-        _check(lib().fcfd_create(mech.handle, C.byref(self.o), C.c_int(nblk), C.c_int(ncell),
-                                 C.c_double(dx), C.byref(self.handle)))
+        if smr is None:
+            _check(lib().fcfd_create(mech.handle, C.byref(self.o), C.c_int(nblk), C.c_int(ncell),
+                                     C.c_double(dx), C.byref(self.handle)))
+        else:
+            _check(lib().fcfd_create_smr(mech.handle, C.byref(self.o), C.c_int(nblk), C.c_int(ncell),
+                                         C.c_double(dx), *(C.c_int(v) for v in smr), C.byref(self.handle)))
+            nblk = lib().fcfd_nblocks(self.handle)
         self.nblk, self.ncell, self.dx = nblk, ncell, dx
         self.lv = lib().fcfd_lvir(self.handle)
         self.nequ = mech.nequ
@@ -358,6 +365,16 @@ class Solver:
         xm = _f64(xm)
         assert xm.size == self.nblk * self.ncell
         _check(lib().fcfd_set_cell_centres(self.handle, _p(xm)))
+
+    def block_dx(self):
+        out = np.zeros(self.nblk)
+        _check(lib().fcfd_block_dx(self.handle, _p(out)))
+        return out
+
+    def cell_centres(self):
+        out = np.zeros(self.nblk * self.ncell)
+        _check(lib().fcfd_get_cell_centres(self.handle, _p(out)))
+        return out
 
     def get_aux(self):
         """(Qrad, Qw) of the last auxiliary-source call, W/m^3 per interior cell."""

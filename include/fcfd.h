@@ -84,9 +84,26 @@ int fcfd_mech_elements(const fcfd_mech *m, int *mm);
  * (replaces set_one_block_size / set_block_size + set_block_params, SRC/block.f90:38,91,394). */
 int fcfd_create(const fcfd_mech *m, const fcfd_opts *o, int nblk, int ncell, double dx,
                 fcfd_ctx **out);
+/* The reference's stretched-mesh layout, its &blocks namelist (set_block_size,
+ * SRC/block.f90:91-392 on one rank): nblk_comp blocks of ncell cells at dx_comp, then
+ * (lev_smr_0 - 1) * num_smr_0 coarser blocks on the left and (lev_smr_n - 1) * num_smr_n on the
+ * right, num_smr blocks per level, the cell size doubling per level away from the fine region
+ * (dx_comp * 2^(level - 1)).  Blocks are numbered left to right (fcfd_nblocks of them; host arrays
+ * cover all of them).  Coarse blocks reconstruct at first order (SRC/flux.f90:240); the halo
+ * between unequal blocks averages pairs of fine cells and interpolates coarse cells with a
+ * minmod slope (interpolation_block, SRC/block.f90:499-563); the cell centres follow
+ * set_block_size (x = 0 where the left refinement region ends).  ncell >= 2 lvir; select_time 0
+ * or 1.  calc_dt keeps dx_comp (SRC/main.f90:134). */
+int fcfd_create_smr(const fcfd_mech *m, const fcfd_opts *o, int nblk_comp, int ncell, double dx_comp,
+                    int lev_smr_0, int num_smr_0, int lev_smr_n, int num_smr_n, fcfd_ctx **out);
 void fcfd_destroy(fcfd_ctx *c);
 /* Ghost layers per block end for the configured scheme (SRC/params.f90:161-173). */
 int fcfd_lvir(const fcfd_ctx *c);
+/* Number of blocks; each block's cell size b_obj(ib)%smr_dx (nblocks values); the interior cell
+ * centres b_obj(ib)%xm_b (nblocks * ncell values, m). */
+int fcfd_nblocks(const fcfd_ctx *c);
+int fcfd_block_dx(const fcfd_ctx *c, double *dx_block);
+int fcfd_get_cell_centres(fcfd_ctx *c, double *xm);
 
 /* Host <-> device state of block ib (0-based): (ncell + 2*lvir) cells x nequ, ghosts included,
  * i.e. b_obj(ib)%Qc / %qf (SRC/block.f90:409-413).  qf may be NULL on upload, in which case it

@@ -842,12 +842,54 @@ static void update_boundary(fo_domain *d) {                       /* SRC/boundar
         fo_qc_to_qf(d->m, cellp(d, d->Qc, nl, nc - 1 + g), cellp(d, d->qf, nl, nc - 1 + g));
 }
 
-static void interpolation_block(fo_domain *d) {                   /* SRC/block.f90:475-497 */
-    for (int ib = 0; ib < d->nblk - 1; ++ib)
-        for (int g = 1; g <= d->lv; ++g) {
-            memcpy(cellp(d, d->Qc, ib + 1, -g), cellp(d, d->Qc, ib, d->ncell - g), sizeof(double) * d->nequ);
-            memcpy(cellp(d, d->Qc, ib, d->ncell - 1 + g), cellp(d, d->Qc, ib + 1, g - 1), sizeof(double) * d->nequ);
+/* minmod slope of the coarse/fine fill (SRC/block.f90:522-525) */
+static double smr_slope(double m, double p) {
+    return 0.5 * (copysign(1.0, m) + copysign(1.0, p)) * fmin(fabs(m), fabs(p));
+}
+
+static void interpolation_block(fo_domain *d) {                   /* SRC/block.f90:475-565 */
+    const int lv = d->lv, nc = d->ncell, ne = d->nequ;
+    for (int ib = 0; ib < d->nblk - 1; ++ib) {
+        const double dl = d->dxb[ib], dr = d->dxb[ib + 1];
+        if (dl == dr) {                                            /* :490-497 */
+            for (int g = 1; g <= lv; ++g) {
+                memcpy(cellp(d, d->Qc, ib + 1, -g), cellp(d, d->Qc, ib, nc - g), sizeof(double) * ne);
+                memcpy(cellp(d, d->Qc, ib, nc - 1 + g), cellp(d, d->Qc, ib + 1, g - 1), sizeof(double) * ne);
+            }
+        } else if (dl > dr) {                                      /* coarse | fine, :499-531 */
+            for (int g = 1; g <= lv; ++g) {
+                double *gh = cellp(d, d->Qc, ib, nc - 1 + g);
+                const double *a = cellp(d, d->Qc, ib + 1, 2 * (g - 1)), *b = cellp(d, d->Qc, ib + 1, 2 * (g - 1) + 1);
+                for (int n = 0; n < ne; ++n) gh[n] = (a[n] + b[n]) / 2.0;
+            }
+            for (int g = 1; g <= lv; ++g) {
+                const int ic = -((g - 1) / 2);
+                const double *c0 = cellp(d, d->Qc, ib, nc - 1 + ic), *cm = cellp(d, d->Qc, ib, nc - 2 + ic),
+                             *cp = cellp(d, d->Qc, ib, nc + ic);
+                double *gh = cellp(d, d->Qc, ib + 1, -g);
+                for (int n = 0; n < ne; ++n) {
+                    const double s = smr_slope(c0[n] - cm[n], cp[n] - c0[n]);
+                    gh[n] = (g % 2 == 1) ? c0[n] + 0.25 * s : c0[n] - 0.25 * s;
+                }
+            }
+        } else {                                                   /* fine | coarse, :533-563 */
+            for (int g = 1; g <= lv; ++g) {
+                double *gh = cellp(d, d->Qc, ib + 1, -g);
+                const double *a = cellp(d, d->Qc, ib, nc - 1 - 2 * (g - 1)), *b = cellp(d, d->Qc, ib, nc - 1 - (2 * (g - 1) + 1));
+                for (int n = 0; n < ne; ++n) gh[n] = (a[n] + b[n]) / 2.0;
+            }
+            for (int g = 1; g <= lv; ++g) {
+                const int ic = (g - 1) / 2;
+                const double *c0 = cellp(d, d->Qc, ib + 1, ic), *cm = cellp(d, d->Qc, ib + 1, ic - 1),
+                             *cp = cellp(d, d->Qc, ib + 1, ic + 1);
+                double *gh = cellp(d, d->Qc, ib, nc - 1 + g);
+                for (int n = 0; n < ne; ++n) {
+                    const double s = smr_slope(c0[n] - cm[n], cp[n] - c0[n]);
+                    gh[n] = (g % 2 == 1) ? c0[n] - 0.25 * s : c0[n] + 0.25 * s;
+                }
+            }
         }
+    }
 }
 
 static void qf_all(fo_domain *d) {
@@ -892,6 +934,8 @@ fo_domain *fo_domain_create(const fo_mech *m, const fo_opts *o, int nblk, int nc
     d->k3 = calloc(nint, sizeof(double)); d->k4 = calloc(nint, sizeof(double));
     d->hmem = calloc((size_t)nblk * ncell, sizeof(double));
     d->cstats = calloc((size_t)nblk * ncell * 4, sizeof(int));
+    d->dxb = calloc((size_t)nblk, sizeof(double));
+    for (int ib = 0; ib < nblk; ++ib) d->dxb[ib] = dx;
     for (int ib = 0; ib < nblk; ++ib)
         for (int i = 0; i < ncell; ++i) {
             const double *c = cells + ((size_t)ib * ncell + i) * (3 + kk);
@@ -921,11 +965,55 @@ fo_domain *fo_domain_create(const fo_mech *m, const fo_opts *o, int nblk, int nc
     return d;
 }
 
+fo_domain *fo_domain_create_smr(const fo_mech *m, const fo_opts *o, int nblk_comp, int ncell,
+                                double dx_comp, int lev_smr_0, int num_smr_0, int lev_smr_n,
+                                int num_smr_n, const double *cells) {
+    if (lev_smr_0 < 1 || lev_smr_n < 1 || num_smr_0 < 0 || num_smr_n < 0 || nblk_comp < 1) return NULL;
+    const int n0 = (lev_smr_0 - 1) * num_smr_0, nn = (lev_smr_n - 1) * num_smr_n;
+    const int nblk = nblk_comp + n0 + nn;                          /* SRC/block.f90:136-146 */
+    fo_domain *d = fo_domain_create(m, o, nblk, ncell, dx_comp, cells);
+    /* block spacings (:160-218): the left refinement blocks coarsest first, the right ones
+     * coarsest last, each level num_smr blocks, doubling per level */
+    double length_smr_0 = 0.0;
+    int b = 0;
+    for (int lev = 2; lev <= lev_smr_0; ++lev)
+        for (int k = 0; k < num_smr_0; ++k, ++b) {
+            d->dxb[b] = dx_comp * (double)(1 << (lev_smr_0 - lev + 1));
+            length_smr_0 = length_smr_0 + d->dxb[b] * ncell;
+        }
+    b = 0;
+    for (int lev = 2; lev <= lev_smr_n; ++lev)
+        for (int k = 0; k < num_smr_n; ++k, ++b)
+            d->dxb[nblk - 1 - b] = dx_comp * (double)(1 << (lev_smr_n - lev + 1));
+    /* cell centres (:264-305): a running position that steps by each cell's dx, half a coarse
+     * cell further on a coarse -> fine step and a quarter back on fine -> coarse, then shifted so
+     * that the left refinement region ends at x = 0 */
+    double xl = 0.0, dxl = 0.0;
+    for (int ib = 0; ib < nblk; ++ib)
+        for (int i = 0; i < ncell; ++i) {
+            const double h = d->dxb[ib];
+            double *x = d->xm + (size_t)ib * ncell + i;
+            if (ib == 0 && i == 0) {
+                *x = 0.0; xl = 0.0; dxl = 0.0;
+            } else {
+                *x = xl + h;
+                xl = xl + h;
+                if (dxl > h) { *x = *x + h / 2.0; xl = xl + h / 2.0; }
+                if (dxl < h) { *x = *x - h / 4.0; xl = xl - h / 4.0; }
+            }
+            dxl = h;
+        }
+    const double length_smr = length_smr_0 - d->dxb[0] / 2.0;
+    for (size_t g = 0; g < (size_t)nblk * ncell; ++g) d->xm[g] = d->xm[g] - length_smr;
+    init_ghosts(d);
+    return d;
+}
+
 void fo_domain_free(fo_domain *d) {
     if (!d) return;
     free(d->Qc); free(d->qf); free(d->Fplus); free(d->godunov);
     free(d->Qc0); free(d->k1); free(d->k2); free(d->k3); free(d->k4); free(d->hmem); free(d->cstats);
-    free(d->xm); free(d->qrad); free(d->qw);
+    free(d->xm); free(d->qrad); free(d->qw); free(d->dxb);
     free(d);
 }
 
@@ -951,9 +1039,9 @@ static void compute_fluxes(fo_domain *d) {
     for (int ib = 0; ib < d->nblk; ++ib) {
         double *F = d->Fplus + (size_t)ib * (d->ncell + 1) * nequ;
         if (d->o.do_tran == 1)
-            fo_flux_tran(d->m, d->ncell, d->lv, d->dx, cellp(d, d->Qc, ib, -d->lv), F);
+            fo_flux_tran(d->m, d->ncell, d->lv, d->dxb[ib], cellp(d, d->Qc, ib, -d->lv), F);
         if (d->o.do_hydro == 1)
-            fo_flux_advection(d->m, &d->o, d->ncell, d->lv, d->dx, cellp(d, d->qf, ib, -d->lv), F,
+            fo_flux_advection(d->m, &d->o, d->ncell, d->lv, d->dxb[ib], cellp(d, d->qf, ib, -d->lv), F,
                               d->godunov + (size_t)ib * (d->ncell + 1) * 3);
     }
 }
@@ -976,7 +1064,7 @@ static void stage_sync(fo_domain *d) {
 static void sph_update(fo_domain *d, int ib, int i, double dt, double h1, double h2, const double *Q0,
                        double *Q) {
     const int nequ = d->nequ, nc = d->ncell;
-    const double dx = d->dx;
+    const double dx = d->dxb[ib];
     const double rm = d->xm[(size_t)ib * nc + i];
     const double Al = (rm - (dx / 2.0)) * (rm - (dx / 2.0));
     const double Ar = (rm + (dx / 2.0)) * (rm + (dx / 2.0));
@@ -993,13 +1081,13 @@ static void sph_update(fo_domain *d, int ib, int i, double dt, double h1, double
 
 void fo_solve_fluid(fo_domain *d, double dt) {
     const int nequ = d->nequ, nc = d->ncell;
-    const double dx = d->dx;
     const int sph = d->o.is_spherical == 1;
     if (d->o.select_time == 0) {                                   /* explicit_euler :12-96 */
         compute_fluxes(d);
         for (int ib = 0; ib < d->nblk; ++ib)
             for (int i = 0; i < nc; ++i) {
                 double *Q = cellp(d, d->Qc, ib, i), *Fr = fint(d, ib, i + 1), *Fl = fint(d, ib, i);
+                const double dx = d->dxb[ib];
                 if (sph) { sph_update(d, ib, i, dt, 1.0, 1.0, NULL, Q); continue; }
                 for (int n = 0; n < nequ; ++n) Q[n] = Q[n] - (dt / dx) * (Fr[n] - Fl[n]);
             }
@@ -1021,6 +1109,7 @@ void fo_solve_fluid(fo_domain *d, double dt) {
                 for (int i = 0; i < nc; ++i) {
                     double *Q = cellp(d, d->Qc, ib, i), *Fr = fint(d, ib, i + 1), *Fl = fint(d, ib, i);
                     const double *Q0 = d->Qc0 + ((size_t)ib * nc + i) * nequ;
+                    const double dx = d->dxb[ib];
                     if (sph) { sph_update(d, ib, i, dt, h1, h2, Q0, Q); continue; }
                     for (int n = 0; n < nequ; ++n)
                         Q[n] = h1 * Q0[n] + h2 * (Q[n] - (dt / dx) * (Fr[n] - Fl[n]));
@@ -1038,6 +1127,7 @@ void fo_solve_fluid(fo_domain *d, double dt) {
                 for (int i = 0; i < nc; ++i) {
                     double *Q = cellp(d, d->Qc, ib, i), *Fr = fint(d, ib, i + 1), *Fl = fint(d, ib, i);
                     double *k = ks[st] + ((size_t)ib * nc + i) * nequ;
+                    const double dx = d->dxb[ib];
                     for (int n = 0; n < nequ; ++n) {
                         k[n] = -(dt / dx) * (Fr[n] - Fl[n]);
                         if (st < 2) Q[n] = Q[n] - 0.5 * (0.5 * dt / dx) * (Fr[n] - Fl[n]);

@@ -135,6 +135,7 @@ typedef struct {
     double *xm;          /* nblk * ncell cell centres x_m (m)                                      */
     double *qrad, *qw;   /* nblk * ncell: Qrad and Qw of the last auxiliary-source call            */
     double time;         /* driver time, advanced by fo_step                                        */
+    double *dxb;         /* nblk: each block's cell size (smr_dx)                                  */
 } fo_domain;
 
 /* Allocates and initialises from (T, p, u, X) per interior cell the way initialize_variables /
@@ -142,6 +143,15 @@ typedef struct {
  * SRC/main.f90:94-114). cells: nblk*ncell rows of (3+KK). */
 fo_domain *fo_domain_create(const fo_mech *m, const fo_opts *o, int nblk, int ncell, double dx,
                             const double *cells);
+/* The reference's stretched-mesh layout (&blocks: set_block_size, SRC/block.f90:91-392 on one
+ * rank): nblk_comp blocks at dx_comp, (lev_smr_0 - 1) * num_smr_0 coarser blocks on the left and
+ * (lev_smr_n - 1) * num_smr_n on the right, the spacing doubling per level away from the fine
+ * region; cells covers all nblk_comp + ... blocks in order.  Coarse blocks reconstruct at first
+ * order (SRC/flux.f90:240), block halos between unequal blocks are averaged / minmod-interpolated
+ * (SRC/block.f90:499-563).  NULL for invalid levels. */
+fo_domain *fo_domain_create_smr(const fo_mech *m, const fo_opts *o, int nblk_comp, int ncell,
+                                double dx_comp, int lev_smr_0, int num_smr_0, int lev_smr_n,
+                                int num_smr_n, const double *cells);
 void fo_domain_free(fo_domain *d);
 double fo_calc_dt(fo_domain *d, double cfl);   /* SRC/update.f90:13-59, single rank */
 void fo_solve_fluid(fo_domain *d, double dt);  /* SRC/update.f90:62-97 -> timeintegration.f90 */

@@ -40,6 +40,7 @@ def _lib(libm=False):
                  "fo_heat_release_rate_omegadot", "fo_mcavis", "fo_mcacon", "fo_calc_dt", "fo_step"]:
         getattr(lib, name).restype = C.c_double
     lib.fo_domain_create.restype = C.c_void_p
+    lib.fo_domain_create_smr.restype = C.c_void_p
     return lib
 
 
@@ -76,7 +77,7 @@ class FoAux(C.Structure):
 class FoDomainAux(C.Structure):
     """fo_domain in full."""
     _fields_ = FoDomainStats._fields_ + [("aux", FoAux), ("xm", _D), ("qrad", _D), ("qw", _D),
-                                         ("time", C.c_double)]
+                                         ("time", C.c_double), ("dxb", _D)]
 
 
 def _p(a):
@@ -228,7 +229,9 @@ class Oracle:
                       do_tran, dx_comp)
 
     def flux_advection(self, qf, n, lv, dx, **kw):
-        o = self.opts(dx_comp=dx, **kw)
+        """dx: the block's cell size; dx_comp (default dx) the fine mesh size of the domain."""
+        kw.setdefault("dx_comp", dx)
+        o = self.opts(**kw)
         qf = _f64(qf)
         F = np.zeros((n + 1, self.nequ))
         G = np.zeros((n + 1, 3))
@@ -306,12 +309,25 @@ Oracle.ncf, Oracle.chem_rhs, Oracle.chem_jac, Oracle.chem_step = _chem_methods()
 class Domain:
     """Multi-block wall-bounded domain stepped by the oracle (reference storage layout)."""
 
-    def __init__(self, orc: Oracle, cells, nblk, ncell, dx, **opts):
+    def __init__(self, orc: Oracle, cells, nblk, ncell, dx, smr=None, **opts):
+        """smr = (lev_smr_0, num_smr_0, lev_smr_n, num_smr_n): the reference's stretched-mesh
+        layout around nblk blocks at dx (cells then covers every block)."""
         self.orc = orc
         self.o = orc.opts(dx_comp=dx, **opts)
         cells = _f64(cells)
-        self.ptr = C.c_void_p(orc.lib.fo_domain_create(orc.m, C.byref(self.o), C.c_int(nblk),
-                                                        C.c_int(ncell), C.c_double(dx), _p(cells)))
+        if smr is None:
+            ptr = orc.lib.fo_domain_create(orc.m, C.byref(self.o), C.c_int(nblk), C.c_int(ncell),
+                                           C.c_double(dx), _p(cells))
+        else:
+            l0, n0, ln, nn = smr
+            nblk = nblk + (l0 - 1) * n0 + (ln - 1) * nn
+            assert cells.shape[0] == nblk * ncell
+            ptr = orc.lib.fo_domain_create_smr(orc.m, C.byref(self.o), C.c_int(nblk - (l0 - 1) * n0 - (ln - 1) * nn),
+                                               C.c_int(ncell), C.c_double(dx), C.c_int(l0), C.c_int(n0),
+                                               C.c_int(ln), C.c_int(nn), _p(cells))
+            if not ptr:
+                raise ValueError("oracle: invalid refinement levels")
+        self.ptr = C.c_void_p(ptr)
         self.d = C.cast(self.ptr, C.POINTER(FoDomain)).contents
         self.nblk, self.ncell, self.lv, self.nequ = nblk, ncell, self.d.lv, orc.nequ
 
@@ -401,6 +417,10 @@ class Domain:
         x = _f64(xm)
         assert x.size == self.nblk * self.ncell
         self.orc.lib.fo_domain_set_xm(self.ptr, _p(x))
+
+    @property
+    def dxb(self):
+        return np.ctypeslib.as_array(self.full.dxb, shape=(self.nblk,)).copy()
 
     @property
     def time(self):

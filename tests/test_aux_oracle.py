@@ -295,3 +295,100 @@ def test_spherical_blast_conserves_mass_and_energy(orc):
         m1, e1 = tot()
         assert m1 == pytest.approx(m0, rel=1e-13) and e1 == pytest.approx(e0, rel=1e-13)
         assert d.qf[0, lv:-lv, 1].max() > 1.0            # the blast is moving outward
+
+
+# ---- stretched mesh (SMR) -----------------------------------------------------------------------
+def smr_domain(orc, smr=(3, 1, 3, 2), nfine=4, ncell=8, dx=1e-4, **kw):
+    from oracle.pyoracle import Domain
+    nm = names()
+    kk = len(nm)
+    nblk = nfine + (smr[0] - 1) * smr[1] + (smr[2] - 1) * smr[3]
+    cells = np.zeros((nblk * ncell, 3 + kk))
+    cells[:, 0], cells[:, 1] = 300.0, 101325.0
+    for sp, x in CH4_AIR.items():
+        cells[:, 3 + nm.index(sp)] = x
+    return Domain(orc, cells, nfine, ncell, dx, smr=smr, **kw), cells
+
+
+def test_smr_layout(orc):
+    """Block spacings double per level away from the fine region, num_smr blocks per level; the
+    cell centres are the midpoints of consecutive cells of those widths with x = 0 where the left
+    refinement region ends (SRC/block.f90:160-305)."""
+    d, _ = smr_domain(orc)
+    dx = 1e-4
+    np.testing.assert_array_equal(d.dxb, dx * np.array([4, 2, 1, 1, 1, 1, 2, 2, 4, 4]))
+    widths = np.repeat(d.dxb, d.ncell)
+    edges = np.concatenate([[0.0], np.cumsum(widths)]) - d.ncell * (4 + 2) * dx
+    np.testing.assert_allclose(d.xm, 0.5 * (edges[:-1] + edges[1:]), rtol=0, atol=1e-18)
+    assert d.xm[2 * d.ncell] == pytest.approx(dx / 2, rel=1e-12)
+
+
+def test_smr_halo_reproduces_linear_data_and_limits_extrema(orc):
+    """interpolation_block between unequal blocks (SRC/block.f90:499-563): on data linear in x every
+    halo cell takes the line's value at its centre (pair averages on the coarse side, the coarse
+    cell plus or minus a quarter of its slope on the fine side); at a local extremum of the coarse
+    data the minmod slope vanishes and the fine halo copies the coarse cell."""
+    d, _ = smr_domain(orc)
+    lv, nc = d.lv, d.ncell
+    Q = d._arr(d.d.Qc)                          # live view of the oracle's conserved state
+    base = Q[0, lv].copy()
+    x = d.xm.reshape(d.nblk, nc)
+    for b in range(d.nblk):
+        Q[b, lv:lv + nc, :] = base * (1.0 + 0.3 * x[b][:, None] / 0.01)
+    d.refresh()
+    Q = d._arr(d.d.Qc)
+    h = d.dxb
+    for b in range(d.nblk - 1):
+        if h[b] == h[b + 1]:
+            continue
+        for g in range(1, lv + 1):
+            # right halo of block b (cell size h[b]) and left halo of block b + 1 (h[b + 1])
+            xg_left_block = x[b, -1] + g * h[b]
+            xg_right_block = x[b + 1, 0] - g * h[b + 1]
+            np.testing.assert_allclose(Q[b, lv + nc - 1 + g], base * (1.0 + 0.3 * xg_left_block / 0.01), rtol=1e-13)
+            np.testing.assert_allclose(Q[b + 1, lv - g], base * (1.0 + 0.3 * xg_right_block / 0.01), rtol=1e-13)
+    # an extremum in the coarse block next to a fine one: zero slope, the fine halo copies it
+    d2, _ = smr_domain(orc)
+    Q = d2._arr(d2.d.Qc)
+    b = 1                                       # coarse (2 dx) left of the first fine block
+    Q[b, lv + nc - 1, 0] *= 1.01                # a peak in the coarse block's last cell
+    d2.refresh()
+    Q = d2._arr(d2.d.Qc)
+    assert Q[b + 1, lv - 1, 0] == Q[b, lv + nc - 1, 0] == Q[b + 1, lv - 2, 0]
+
+
+def test_smr_coarse_blocks_are_first_order(orc):
+    """Blocks whose cell size is not dx_comp take the two adjacent cells as the interface states,
+    whatever the scheme (SRC/flux.f90:240-245)."""
+    rng = np.random.default_rng(3)
+    kk = len(names())
+    n, lv = 12, 3
+    qf = np.zeros((n + 2 * lv, 3 + kk))
+    qf[:, 0] = rng.uniform(0.5, 1.5, n + 2 * lv)
+    qf[:, 1] = rng.uniform(-50, 50, n + 2 * lv)
+    qf[:, 2] = rng.uniform(400, 2000, n + 2 * lv)
+    qf[:, 3 + names().index("N2")] = 0.7
+    qf[:, 3 + names().index("O2")] = 0.3
+    F2, _ = orc.flux_advection(qf, n, lv, 2e-4, dx_comp=1e-4, flag_acu=2, flag_weno=2)
+    F0, _ = orc.flux_advection(qf, n, lv, 1e-4, dx_comp=1e-4, flag_acu=0)
+    Ff, _ = orc.flux_advection(qf, n, lv, 1e-4, dx_comp=1e-4, flag_acu=2, flag_weno=2)
+    np.testing.assert_array_equal(F2, F0)
+    assert not np.array_equal(Ff, F0)
+
+
+def test_smr_domain_steps(orc):
+    """An acoustic pulse crossing the refinement boundaries of a walled domain stays finite and
+    positive, and a uniform state at rest stays put."""
+    d, cells = smr_domain(orc, select_time=1)
+    q0 = d.Qc
+    for _ in range(3):
+        d.step(0.5, react=False)
+    lv = d.lv
+    np.testing.assert_allclose(d.Qc[:, lv:-lv], q0[:, lv:-lv], rtol=1e-13, atol=1e-12)
+    from oracle.pyoracle import Domain
+    cells[40:48, 1] *= 1.2
+    d = Domain(orc, cells, 4, 8, 1e-4, smr=(3, 1, 3, 2))
+    for _ in range(10):
+        d.step(0.5, react=False)
+    qf = d.qf[:, lv:-lv]
+    assert np.isfinite(qf).all() and (qf[..., 0] > 0).all() and np.abs(qf[..., 1]).max() > 0.1

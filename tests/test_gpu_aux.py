@@ -9,7 +9,10 @@ reference's own deck files for their parameters:
     Frendi ignition source of igsource.inp, chemistry with Strang splitting;
   * a spherical blast (hot, pressurised centre) with three schemes, inert;
   * W. Zhang ignition with two overlapping sources (the multi-source file format) through the
-    stepwise entry points, fcfd_solve_source taking the time from the caller.
+    stepwise entry points, fcfd_solve_source taking the time from the caller;
+  * stretched meshes (fcfd_create_smr): the micro flow reactor and the spherical flame with
+    refinement levels on both / the right side, and an inert pulse with two schemes; there the
+    whole state, every block halo included, is compared.
 
 Bar: dt of every step within 1e-13 relative; every field within 1e-12 of its natural scale
 (compare_fields); the ghost cells of the open ends, Qrad and Qw within 1e-12 of their largest
@@ -215,3 +218,84 @@ def test_spherical_blast_vs_oracle(mech, orc, scheme):
     compare_fields(s, d, dts, dts, orc)
     _, qf = s.download()
     assert qf[0, d.lv:-d.lv, 1].max() > 1.0
+
+
+def compare_all_columns(s, d, orc):
+    """Every column of Qc (interiors and all halos) within REL of the compare_ghosts scales."""
+    Qc, _ = s.download()
+    Qo, qo = d.Qc, d.qf
+    rho, T, Y = qo[..., 0], qo[..., 2], qo[..., 3:]
+    a = np.sqrt(1.4 * 8314.51 * T / (1.0 / (Y / orc.weights()).sum(axis=-1)))
+    scale = np.abs(Qo).max(axis=(0, 1))
+    scale[1] = (rho * a).max()
+    scale[3:] = np.maximum(scale[3:], Y_FLOOR * rho.max())
+    err = (np.abs(Qc - Qo) / scale).max()
+    assert err <= REL, err
+
+
+SMR_CASES = {
+    "mfr": dict(smr=(3, 1, 3, 1), react=True, strang=False),
+    "flame": dict(smr=(1, 0, 3, 2), react=True, strang=True),
+    "pulse_weno5z_rk3": dict(smr=(3, 2, 2, 1), react=False, strang=False,
+                             scheme=dict(flag_acu=2, flag_weno=3, select_riemann=2, select_time=1)),
+    "pulse_muscl_euler": dict(smr=(2, 1, 3, 1), react=False, strang=False,
+                              scheme=dict(flag_acu=1, flag_muscl=2, select_riemann=1, select_time=0)),
+}
+
+
+@pytest.mark.parametrize("case", list(SMR_CASES))
+def test_smr_vs_oracle(mech, orc, case):
+    from fortrancfd_amd import inputs
+    from fortrancfd_amd.lib import Solver
+    from oracle.pyoracle import Domain
+    cfg = SMR_CASES[case]
+    nm = names()
+    kk = mech.kk
+    nfine, ncell, dx = 2, 16, 2e-5
+    smr = cfg["smr"]
+    nblk = nfine + (smr[0] - 1) * smr[1] + (smr[2] - 1) * smr[3]
+    n = nblk * ncell
+    cells = ch4_air_cells(kk, n, [])
+    kw = cfg.get("scheme", dict(flag_acu=2, flag_weno=2, select_riemann=2, select_time=1))
+    extra = {}
+    if case == "mfr":
+        kw = dict(kw, do_tran=1)
+        extra = dict(flag_lbound=1, flag_rbound=1)
+        cells[:, 0] = np.linspace(450.0, 1350.0, n)
+        cells[:, 2] = 0.3
+    elif case == "flame":
+        kw = dict(kw, is_spherical=1)
+        extra = dict(flag_rbound=1)
+        cells[:20, 0] = 2100.0
+    else:
+        cells[n // 2 - 4: n // 2 + 4, 1] *= 1.3
+    s = Solver(mech, nfine, ncell, dx, smr=smr, do_reaction=int(cfg["react"]), **kw, **extra)
+    assert s.nblk == nblk
+    d = Domain(orc, cells, nfine, ncell, dx, smr=smr, **kw)
+    np.testing.assert_array_equal(s.block_dx(), d.dxb)
+    np.testing.assert_array_equal(s.cell_centres(), d.xm)
+    if case == "mfr":
+        fb = inputs.read_flow_boundary(deck("CH4_GRI3.0_MFR", "bound_flow.txt"), nm)
+        w = inputs.read_wall_temperature(deck("CH4_GRI3.0_MFR", "wallT_373_1300.txt"))
+        s.set_flow_boundary(**fb)
+        d.set_flow_boundary(1, 1, **fb)
+        s.set_wall_heat(**w)
+        d.set_wall_heat(**w)
+    elif case == "flame":
+        base = ("CH4_GRI3.0_SphericalFlame_radiation",)
+        fb = inputs.read_flow_boundary(deck(*base, "bound_flow.txt"), nm)
+        idx, pm = inputs.read_radiation_params(deck(*base, "rad_params_4sp.txt"), nm)
+        ig = inputs.read_igsource(deck(*base, "igsource.inp"))
+        s.set_flow_boundary(**fb)
+        d.set_flow_boundary(0, 1, **fb)
+        for o in (s, d):
+            o.set_radiation(idx, pm)
+            o.set_ignition(1, **ig)
+    if cfg["react"]:
+        s.set_step(strang=cfg["strang"], hrr_each_step=True)
+    s.init_cells(cells)
+    compare_all_columns(s, d, orc)                      # the start-up halos
+    print(f"stretched mesh {case}: {nblk} blocks, dx per block {s.block_dx() / dx}", flush=True)
+    dts = step_pair(s, d, 3, cfg["react"], cfg["strang"], cfl=0.4)
+    compare_fields(s, d, dts, dts, orc)
+    compare_all_columns(s, d, orc)
