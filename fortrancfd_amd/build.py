@@ -53,7 +53,29 @@ def build(verbose: bool = False) -> str:
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
     build_fortran(verbose)
+    build_h5(verbose)
     return LIB
+
+
+HDF5_PREFIX = os.environ.get("FCFD_HDF5", "/opt/conda")
+LIB_H5 = os.path.join(OUT_DIR, "libfcfd_h5.so")
+
+
+def build_h5(verbose: bool = False) -> str | None:
+    """The snapshot writer/reader (csrc/h5io.cpp, host only) over the HDF5 C library of
+    $FCFD_HDF5 (default /opt/conda) -> _lib/libfcfd_h5.so; skipped when hdf5.h is absent."""
+    inc, libdir = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"hdf5.h not found under {inc}: libfcfd_h5.so not built", flush=True)
+        return None
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", f"-I{inc}",
+           os.path.join(CSRC, "h5io.cpp"), "-o", LIB_H5 + ".tmp", f"-L{libdir}", "-lhdf5",
+           f"-Wl,-rpath,{libdir}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(LIB_H5 + ".tmp", LIB_H5)
+    return LIB_H5
 
 
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")

@@ -125,6 +125,7 @@ struct fcfd_ctx {
     double *d_dxb = nullptr;             // nullptr: dx in every block
     std::vector<double> h_dxb;
     int smr_left = 0, smr_right = 0;
+    int idx0 = 1;                        // the reference's index of the first cell (nx_all_0)
     int *d_smr_ifb = nullptr;
     int n_smr_if = 0;
 };
@@ -761,7 +762,9 @@ int fcfd_create_smr(const fcfd_mech *m, const fcfd_opts *o, int nblk_comp, int n
         }
     const double length_smr = length_smr_0 - dxb[0] / 2.0;
     for (auto &x : xm) x = x - length_smr;
-    return create_impl(m, o, nblk, ncell, dx_comp, dxb, xm, out);
+    int rc = create_impl(m, o, nblk, ncell, dx_comp, dxb, xm, out);
+    if (rc == FCFD_OK) (*out)->idx0 = 1 - ncell * n0;            // nx_all_0, SRC/block.f90:128
+    return rc;
 }
 
 int fcfd_nblocks(const fcfd_ctx *c) { return c ? c->nblk : FCFD_E_ARG; }
@@ -1079,6 +1082,47 @@ int fcfd_set_cell_centres(fcfd_ctx *c, const double *xm) {
     if (!c || !xm) { set_error("fcfd_set_cell_centres: invalid argument"); return FCFD_E_ARG; }
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipMemcpy(c->d_xm, xm, sizeof(double) * size_t(c->nblk) * c->ncell, hipMemcpyHostToDevice));
+    return FCFD_OK;
+}
+
+int fcfd_eos_fields(fcfd_ctx *c, double *out) {
+    if (!c || !out) { set_error("fcfd_eos_fields: invalid argument"); return FCFD_E_ARG; }
+    const size_t n = size_t(c->nblk) * c->ncell;
+    DevBuf d;
+    HIP_TRY(hipMalloc(&d.p, sizeof(double) * n * (15 + c->kk)));
+    EosArgs a{};
+    a.thermo = c->thermo; a.par = c->d_aux; a.wall_heat = c->aux.wall_heat; a.Q = c->Q[c->cur]; a.qf = c->qf;
+    a.xm = c->d_xm; a.dxb = c->d_dxb; a.dx = c->dx;
+    a.hrr = c->d_hrr; a.hrd = c->d_hrr_dot; a.qrad = c->d_qrad; a.qw = c->d_qw;
+    a.out = static_cast<double *>(d.p); a.idx0 = c->idx0;
+    a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P;
+    HIP_TRY(launch_eos_fields(a, c->kk, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, d.p, sizeof(double) * n * (15 + c->kk), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return FCFD_OK;
+}
+
+int fcfd_init_restart(fcfd_ctx *c, const double *fields) {
+    if (!c || !fields) { set_error("fcfd_init_restart: invalid argument"); return FCFD_E_ARG; }
+    if ((c->o.flag_lbound == 1 || c->o.flag_rbound == 1) && !c->bc_set) {
+        set_error("fcfd_init_restart: inflow / outflow boundaries need fcfd_set_flow_boundary first");
+        return FCFD_E_ARG;
+    }
+    const size_t n = size_t(c->nblk) * c->ncell;
+    DevBuf d;
+    HIP_TRY(hipMalloc(&d.p, sizeof(double) * n * (4 + c->kk)));
+    HIP_TRY(hipMemcpyAsync(d.p, fields, sizeof(double) * n * (4 + c->kk), hipMemcpyHostToDevice, c->stream));
+    EosArgs a{};
+    a.thermo = c->thermo; a.Q = c->Q[c->cur]; a.qf = c->qf; a.out = static_cast<double *>(d.p);
+    a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P;
+    HIP_TRY(launch_restart(a, c->kk, c->stream));
+    if (c->d_hmem) HIP_TRY(hipMemsetAsync(c->d_hmem, 0, sizeof(double) * n, c->stream));
+    StageArgs s = stage_args(c, 2, nullptr, c->Q[c->cur], c->Q[c->cur], 0.0, nullptr, 1.0, 1.0);
+    s.ghosts_only = 1;
+    HIP_TRY(launch_stage_end(s, c->kk, c->stream));
+    HIP_TRY(smr_halo(c, c->Q[c->cur]));
+    HIP_TRY(open_bc(c, c->Q[c->cur]));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return FCFD_OK;
 }
 

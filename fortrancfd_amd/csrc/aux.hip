@@ -213,6 +213,75 @@ __global__ void __launch_bounds__(64) k_aux(AuxArgs a) {
     a.qf[2 * P + c] = T;
 }
 
+// set_eos_params (SRC/block.f90:433-470) and the dataset rows of write_hdf5_b
+// (SRC/write_hdf5.f90:138-160), one thread per interior cell.
+template <int NSP>
+__global__ void __launch_bounds__(64) k_eos_fields(EosArgs a) {
+    const long n = long(a.nblk) * a.ncell;
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int b = int(t / a.ncell);
+    const long c = long(b) * a.S + a.lv + (t % a.ncell);
+    const DevThermo *__restrict__ m = a.thermo;
+    const long P = a.P;
+    double x[NSP];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) x[k] = a.qf[(3 + k) * P + c];
+    ytx<NSP>(m, x);                                     // Yi2Xi, no adjust_sum
+    const double rho = a.qf[c], u = a.qf[P + c], T = a.qf[2 * P + c];
+    const double wx = mix_w<NSP>(m, x);
+    const MixSums ms = mix_sums<NSP>(m, x, T, true, false, false);
+    double *o = a.out;
+    o[0 * n + t] = rho;
+    o[1 * n + t] = u;
+    o[2 * n + t] = T;
+    o[3 * n + t] = (rho * 1.e-3) * m->ru * T / wx * 0.1;                // pressure (CKPX)
+    o[4 * n + t] = a.Q[2 * P + c] / rho;
+    o[5 * n + t] = 0.5 * (u * u);
+    o[6 * n + t] = ms.ubml * 1.e-4 / wx;                                 // specific_energy
+    o[7 * n + t] = a.hrr ? a.hrr[t] : 0.0;
+    o[8 * n + t] = a.hrd ? a.hrd[t] : 0.0;
+    o[9 * n + t] = a.qrad ? a.qrad[t] : 0.0;
+    double Tw = 0.0;
+    if (a.wall_heat) {                                  // SRC/params.f90:495-498 -> SRC/mfr.f90:63-78
+        const AuxParams *__restrict__ p = a.par;
+        const double A = 0.5 * (p->Tw_max - p->Tw_min), B = 0.5 * (p->Tw_max + p->Tw_min);
+        Tw = A * fm_tanh(p->alpha * (a.xm[t] - p->beta)) + B;
+    }
+    o[10 * n + t] = Tw;
+    o[11 * n + t] = a.qw ? a.qw[t] : 0.0;
+    o[12 * n + t] = a.xm[t];
+    o[13 * n + t] = a.dxb ? a.dxb[b] : a.dx;
+    o[14 * n + t] = double(a.idx0 + t);
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) o[(15 + k) * n + t] = x[k];
+}
+
+// Restart state (read_hdf5_restart's X -> Y, SRC/read_hdf5.f90:415-417, then set_block_params,
+// SRC/block.f90:413-429) from rows rho, u, T, e_total, X_1..X_KK.
+template <int NSP>
+__global__ void __launch_bounds__(64) k_restart(EosArgs a) {
+    const long n = long(a.nblk) * a.ncell;
+    const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const long c = long(t / a.ncell) * a.S + a.lv + (t % a.ncell);
+    const DevThermo *__restrict__ m = a.thermo;
+    const long P = a.P;
+    const double *f = a.out;
+    double s = 0.0;                                     // CKXTY (cklib.f:9461-9468)
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) s = s + f[(4 + k) * n + t] * m->wt[k];
+    const double rho = f[t], u = f[n + t], T = f[2 * n + t], et = f[3 * n + t];
+    a.qf[c] = rho; a.qf[P + c] = u; a.qf[2 * P + c] = T;
+    a.Q[c] = rho; a.Q[P + c] = rho * u; a.Q[2 * P + c] = rho * et;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+        const double y = f[(4 + k) * n + t] * m->wt[k] / s;
+        a.qf[(3 + k) * P + c] = y;
+        a.Q[(3 + k) * P + c] = rho * y;
+    }
+}
+
 template <int NSP>
 __global__ void k_flow_bc_setup(const DevThermo *__restrict__ m, FlowBC *bc, double u_in) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -237,6 +306,24 @@ hipError_t launch_aux(const AuxArgs &a, int nsp, hipStream_t st) {
     const long n = long(a.nblk) * a.ncell;
     const dim3 grid(unsigned((n + 63) / 64)), blk(64);
 #define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_aux<N>), grid, blk, 0, st, a); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_eos_fields(const EosArgs &a, int nsp, hipStream_t st) {
+    const long n = long(a.nblk) * a.ncell;
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_eos_fields<N>), grid, blk, 0, st, a); return hipGetLastError(); }
+    FCFD_FOR_EACH_NSP(FCFD_CASE)
+#undef FCFD_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_restart(const EosArgs &a, int nsp, hipStream_t st) {
+    const long n = long(a.nblk) * a.ncell;
+    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_restart<N>), grid, blk, 0, st, a); return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;

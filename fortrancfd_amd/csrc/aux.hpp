@@ -51,7 +51,24 @@ struct AuxArgs {
     long P;
 };
 
+// set_eos_params + the write_hdf5_b field rows, and the restart state of set_block_params.
+struct EosArgs {
+    const DevThermo *thermo;
+    const AuxParams *par;                  // for Tw (wall_heat)
+    int wall_heat;
+    double *Q, *qf;
+    const double *xm, *dxb;                // dxb nullptr: dx
+    double dx;
+    const double *hrr, *hrd, *qrad, *qw;   // nullptr: zeros
+    double *out;                           // [15 + KK][n]; restart: the input [4 + KK][n]
+    int idx0;
+    int nblk, ncell, lv, S;
+    long P;
+};
+
 hipError_t launch_open_bc(const OpenBCArgs &a, int nsp, hipStream_t st);
+hipError_t launch_eos_fields(const EosArgs &a, int nsp, hipStream_t st);
+hipError_t launch_restart(const EosArgs &a, int nsp, hipStream_t st);
 hipError_t launch_aux(const AuxArgs &a, int nsp, hipStream_t st);
 // y_in = CKXTY(x_in) and rho_u_in = density(p_out, T_in, x_in) * u_in on the device (one thread).
 hipError_t launch_flow_bc_setup(const DevThermo *m, FlowBC *bc, double u_in, int nsp, hipStream_t st);

@@ -193,6 +193,20 @@ int fcfd_set_cell_centres(fcfd_ctx *c, const double *xm);
  * (eos_obj(:)%Qrad, %qTw); zeros for a source that is off.  Either pointer may be NULL. */
 int fcfd_get_aux(fcfd_ctx *c, double *qrad, double *qw);
 
+/* ---- snapshot fields and restart ------------------------------------------------------------
+ * The per-cell output of write_hdf5_b (set_eos_params, SRC/block.f90:433-470, then
+ * SRC/write_hdf5.f90:138-160): out holds 15 + KK rows of nblocks * ncell values, in the reference's
+ * dataset order rho, u_x, T, p, e_total, e_kinetic, e_internal, HRR, HRR_dot, Qrad, Tw, qTw, x_m, dx,
+ * idx (the reference's cell index, 1 at the first fine cell), X_1..X_KK.  HRR / HRR_dot are those of
+ * the last source call that computed them, Qrad / qTw of the last auxiliary-source call, Tw the wall
+ * temperature when the wall heat loss is on (zeros otherwise). */
+int fcfd_eos_fields(fcfd_ctx *c, double *out);
+/* Start from a snapshot (read_hdf5_restart, SRC/read_hdf5.f90:13-423, then set_block_params,
+ * SRC/block.f90:394-431): fields holds 4 + KK rows of nblocks * ncell values, rho, u, T, e_total,
+ * X_1..X_KK; qf = (rho, u, T, Y = CKXTY(X)), Qc = (rho, rho u, rho e_total, rho Y), then the start-up
+ * ghost fill.  Clears the integrator's step-size memory. */
+int fcfd_init_restart(fcfd_ctx *c, const double *fields);
+
 /* HRR and HRR_dot of the last source call that computed them (one value per interior cell, blocks
  * in order; either pointer may be NULL). */
 int fcfd_get_hrr(fcfd_ctx *c, double *hrr, double *hrr_dot);

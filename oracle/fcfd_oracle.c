@@ -1752,6 +1752,54 @@ void fo_aux_sources(fo_domain *d, double dt, double time) {
         }
 }
 
+void fo_eos_fields(const fo_domain *d, const double *hrr, const double *hrr_dot, int idx0, double *out) {
+    const size_t n = (size_t)d->nblk * d->ncell;
+    const int kk = d->m->kk;
+    for (int ib = 0; ib < d->nblk; ++ib)
+        for (int i = 0; i < d->ncell; ++i) {
+            const size_t g = (size_t)ib * d->ncell + i;
+            const double *q = ((const double *)d->qf) + ((size_t)ib * (d->ncell + 2 * d->lv) + (i + d->lv)) * d->nequ;
+            const double *Q = ((const double *)d->Qc) + ((size_t)ib * (d->ncell + 2 * d->lv) + (i + d->lv)) * d->nequ;
+            double x[64];
+            fo_ckytx(d->m, q + 3, x);                              /* SRC/block.f90:458-466 */
+            const double rho = q[0], u = q[1], T = q[2];
+            out[0 * n + g] = rho;
+            out[1 * n + g] = u;
+            out[2 * n + g] = T;
+            out[3 * n + g] = fo_pressure(d->m, rho, T, x);
+            out[4 * n + g] = Q[2] / rho;
+            out[5 * n + g] = 0.5 * (u * u);
+            out[6 * n + g] = fo_specific_energy(d->m, T, x);
+            out[7 * n + g] = hrr ? hrr[g] : 0.0;
+            out[8 * n + g] = hrr_dot ? hrr_dot[g] : 0.0;
+            out[9 * n + g] = d->qrad[g];
+            out[10 * n + g] = d->aux.wall_heat ? fo_wall_temperature(&d->aux, d->xm[g]) : 0.0;  /* params.f90:495-498 */
+            out[11 * n + g] = d->qw[g];
+            out[12 * n + g] = d->xm[g];
+            out[13 * n + g] = d->dxb[ib];
+            out[14 * n + g] = (double)(idx0 + (int)g);
+            for (int k = 0; k < kk; ++k) out[(15 + k) * n + g] = x[k];
+        }
+}
+
+void fo_domain_restart(fo_domain *d, const double *f) {
+    const size_t n = (size_t)d->nblk * d->ncell;
+    const int kk = d->m->kk;
+    for (int ib = 0; ib < d->nblk; ++ib)
+        for (int i = 0; i < d->ncell; ++i) {
+            const size_t g = (size_t)ib * d->ncell + i;
+            double x[64], y[64];
+            for (int k = 0; k < kk; ++k) x[k] = f[(4 + k) * n + g];
+            fo_ckxty(d->m, x, y);                                   /* SRC/read_hdf5.f90:415-417 */
+            const double rho = f[g], u = f[n + g], T = f[2 * n + g], et = f[3 * n + g];
+            double *q = cellp(d, d->qf, ib, i), *Q = cellp(d, d->Qc, ib, i);
+            q[0] = rho; q[1] = u; q[2] = T;                         /* SRC/block.f90:415-424 */
+            Q[0] = rho; Q[1] = rho * u; Q[2] = rho * et;
+            for (int k = 0; k < kk; ++k) { q[3 + k] = y[k]; Q[3 + k] = rho * y[k]; }
+        }
+    init_ghosts(d);
+}
+
 /* One time step of the driver loop (SRC/main.f90:130-163) with the chemistry split either the way
  * the reference driver does it (strang = 0, Lie: calc_dt, solve_fluid(dt), solve_source(dt),
  * refresh) or symmetrically (strang = 1: calc_dt, solve_source(dt/2), refresh, solve_fluid(dt),

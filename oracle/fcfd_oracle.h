@@ -180,6 +180,17 @@ void fo_domain_set_xm(fo_domain *d, const double *xm);
  * recovering the primitives (SRC/source.f90:85-239).  time: the driver's time after the step. */
 void fo_aux_sources(fo_domain *d, double dt, double time);
 
+/* The output fields of set_eos_params + write_hdf5_b (SRC/block.f90:433-470,
+ * SRC/write_hdf5.f90:138-160) per interior cell, blocks in order: rho, u, T, p, e_total, e_kinetic,
+ * e_internal, HRR, HRR_dot, Qrad, Tw, qTw, x_m, dx, idx, X_1..X_KK; out is (15 + KK) rows of
+ * nblk * ncell values.  hrr / hrr_dot (may be NULL: zeros) are the last source call's; idx0 is the
+ * reference's index of the first cell. */
+void fo_eos_fields(const fo_domain *d, const double *hrr, const double *hrr_dot, int idx0, double *out);
+/* Restart from such fields (read_hdf5_restart + set_block_params, SRC/read_hdf5.f90:404-419,
+ * SRC/block.f90:413-429): qf = (rho, u, T, CKXTY(X)), Qc = (rho, rho u, rho e_total, rho Y), then the
+ * start-up ghost fill. */
+void fo_domain_restart(fo_domain *d, const double *fields);
+
 /* ---- CKLIB kinetics -------------------------------------------------------------------------- */
 /* CKWYR (cklib.f:9107-9204): rho g/cm^3, T K, Y -> wdot mol/(cm^3 s).  qf_/qr_ (may be NULL):
  * per-reaction forward / reverse rates of progress as left in RCKWRK(NcI1/NcI2). */

@@ -439,6 +439,20 @@ class Domain:
         return (np.ctypeslib.as_array(f.qrad, shape=(n,)).copy(),
                 np.ctypeslib.as_array(f.qw, shape=(n,)).copy())
 
+    def eos_fields(self, hrr=None, hrr_dot=None, idx0=1):
+        """(15 + KK, n) output rows (fo_eos_fields) with the given last HRR / HRR_dot."""
+        n = self.nblk * self.ncell
+        out = np.zeros((self.nequ + 12, n))
+        h = None if hrr is None else _f64(hrr)
+        hd = None if hrr_dot is None else _f64(hrr_dot)
+        self.orc.lib.fo_eos_fields(self.ptr, None if h is None else _p(h), None if hd is None else _p(hd),
+                                   C.c_int(idx0), _p(out))
+        return out
+
+    def restart(self, rows):
+        """fo_domain_restart from rows rho, u, T, e_total, X_1..X_KK."""
+        self.orc.lib.fo_domain_restart(self.ptr, _p(_f64(rows)))
+
     def chem_stats(self):
         """Per interior cell (block-major): steps, rejects, newton, failed of the last
         solve_source, as the library's fcfd_chem_stats."""

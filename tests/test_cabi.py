@@ -10,8 +10,8 @@ import pytest
 from tests.conftest import REPO, golden
 
 
-def declared_functions():
-    src = open(os.path.join(REPO, "include", "fcfd.h")).read()
+def declared_functions(header="fcfd.h"):
+    src = open(os.path.join(REPO, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(fcfd_[a-z0-9_]+)\s*\(", src)))
 
@@ -23,6 +23,14 @@ def test_exports_every_declared_symbol():
     assert len(names) >= 15
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_h5_library_exports_every_declared_symbol():
+    from fortrancfd_amd import snapshot
+    L = snapshot.h5lib()
+    names = declared_functions("fcfd_h5.h")
+    assert len(names) == 4
+    assert not [n for n in names if not hasattr(L, n)]
 
 
 def test_mechanism_load_and_dims():
