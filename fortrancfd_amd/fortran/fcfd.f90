@@ -22,6 +22,9 @@ module fcfd
     public :: fcfd_chem_stats, fcfd_source_accepted, fcfd_set_chem_workgroups
     public :: fcfd_sur_create, fcfd_sur_free, fcfd_sur_nparams, fcfd_sur_set_params, fcfd_sur_get_params
     public :: fcfd_sur_set_norm, fcfd_sur_set_thresholds, fcfd_set_surrogate
+    public :: fcfd_create_smr, fcfd_nblocks, fcfd_block_dx, fcfd_get_cell_centres, fcfd_set_cell_centres
+    public :: fcfd_set_flow_boundary, fcfd_set_radiation, fcfd_set_wall_heat, fcfd_set_ignition
+    public :: fcfd_get_aux, fcfd_set_step, fcfd_get_hrr, fcfd_eos_fields, fcfd_init_restart
 
     ! Same fields and codes as the reference's &params switches (SRC/params.f90:106-111).
     type, bind(C) :: fcfd_opts
@@ -54,6 +57,91 @@ module fcfd
             integer(c_int), value :: nblk, ncell
             real(c_double), value :: dx
             type(c_ptr), intent(out) :: ctx
+        end function
+        ! &blocks layout with refinement levels (set_block_size, SRC/block.f90:91-392)
+        integer(c_int) function fcfd_create_smr(mech, opts, nblk_comp, ncell, dx_comp, lev_smr_0, num_smr_0, &
+                                                lev_smr_n, num_smr_n, ctx) bind(C, name="fcfd_create_smr")
+            import :: c_int, c_double, c_ptr, fcfd_opts
+            type(c_ptr), value :: mech
+            type(fcfd_opts), intent(in) :: opts
+            integer(c_int), value :: nblk_comp, ncell, lev_smr_0, num_smr_0, lev_smr_n, num_smr_n
+            real(c_double), value :: dx_comp
+            type(c_ptr), intent(out) :: ctx
+        end function
+        integer(c_int) function fcfd_nblocks(ctx) bind(C, name="fcfd_nblocks")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function fcfd_block_dx(ctx, dx_block) bind(C, name="fcfd_block_dx")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: dx_block(*)
+        end function
+        integer(c_int) function fcfd_get_cell_centres(ctx, xm) bind(C, name="fcfd_get_cell_centres")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: xm(*)
+        end function
+        integer(c_int) function fcfd_set_cell_centres(ctx, xm) bind(C, name="fcfd_set_cell_centres")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: xm(*)
+        end function
+        ! set_flow_boundary's inputs (SRC/flow_bound.f90:15-57); x_in: c_null_ptr without an inflow
+        integer(c_int) function fcfd_set_flow_boundary(ctx, T_in, u_in_cm_s, x_in, p_out_atm) &
+                                                       bind(C, name="fcfd_set_flow_boundary")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), value :: T_in, u_in_cm_s, p_out_atm
+            type(c_ptr), value :: x_in
+        end function
+        integer(c_int) function fcfd_set_radiation(ctx, n, species, pmac) bind(C, name="fcfd_set_radiation")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: n
+            integer(c_int), intent(in) :: species(*)     ! 0-based species indices
+            real(c_double), intent(in) :: pmac(*)        ! 6 coefficients per species
+        end function
+        integer(c_int) function fcfd_set_wall_heat(ctx, on, Nu, d_mm, Tw_min, Tw_max, alpha, beta) &
+                                                   bind(C, name="fcfd_set_wall_heat")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: on
+            real(c_double), value :: Nu, d_mm, Tw_min, Tw_max, alpha, beta
+        end function
+        integer(c_int) function fcfd_set_ignition(ctx, kind, n, E_mJ, r_um, t_us, t0_us) &
+                                                  bind(C, name="fcfd_set_ignition")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: kind, n
+            real(c_double), intent(in) :: E_mJ(*), r_um(*), t_us(*), t0_us(*)
+        end function
+        ! eos_obj(:)%Qrad and %qTw of the last auxiliary-source call (c_null_ptr skips either)
+        integer(c_int) function fcfd_get_aux(ctx, qrad, qw) bind(C, name="fcfd_get_aux")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            type(c_ptr), value :: qrad, qw
+        end function
+        integer(c_int) function fcfd_set_step(ctx, strang, hrr_each_step) bind(C, name="fcfd_set_step")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: strang, hrr_each_step
+        end function
+        integer(c_int) function fcfd_get_hrr(ctx, hrr, hrr_dot) bind(C, name="fcfd_get_hrr")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            type(c_ptr), value :: hrr, hrr_dot
+        end function
+        ! write_hdf5_b's rows (15 + KK of nblocks*ncell) and the restart of set_block_params
+        integer(c_int) function fcfd_eos_fields(ctx, out) bind(C, name="fcfd_eos_fields")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out(*)
+        end function
+        integer(c_int) function fcfd_init_restart(ctx, fields) bind(C, name="fcfd_init_restart")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: fields(*)
         end function
         subroutine fcfd_destroy(ctx) bind(C, name="fcfd_destroy")
             import :: c_ptr
