@@ -291,4 +291,56 @@ __device__ __forceinline__ void reconstruct(const double *__restrict__ q, long c
     else { s[7] = s[6]; weno7z(s, l, r); }   // s(8) = qf(i+3) (SRC/flux.f90:232)
 }
 
+// One side of the reconstruction at the interface right of cell c0: side 0 the left state, side 1
+// the right state, with the same operations as the two-sided routines above.  The right-state
+// formulas of WENO3-Z, WENO5 and WENO7-Z are the left-state formulas on the mirrored stencil, so
+// both lanes of an interface pair run the same instructions on mirrored loads; the one place
+// where WENO5's two sides add in a different order (2 s0 - 7 s1 + 11 s2 against
+// 11 s3 - 7 s4 + 2 s5) takes its operands through a select.
+template <int RK>
+__device__ __forceinline__ double reconstruct_side(const double *__restrict__ q, long c0, int muscl_flag, int side) {
+    if (RK == R_FIRST) return q[c0 + side];
+    if (RK == R_MUSCL) {
+        double l, r;
+        reconstruct<R_MUSCL>(q, c0, muscl_flag, l, r);
+        return side ? r : l;
+    }
+    if (RK == R_WENO3Z) {
+        // left: s0 s1 s2 = q[c0-1 .. c0+1]; right: s3 s2 s1 = q[c0+2 .. c0]
+        const double u0 = q[c0 + (side ? 2 : -1)], u1 = q[c0 + (side ? 1 : 0)], u2 = q[c0 + (side ? 0 : 1)];
+        const double eps = 1.0e-6;
+        double b2 = (u0 - u1) * (u0 - u1);
+        double b1 = (u1 - u2) * (u1 - u2);
+        const double tau = fabs(b2 - b1);
+        b2 = 1.0 + (tau / (eps + b2)) * (tau / (eps + b2));
+        b1 = 1.0 + (tau / (eps + b1)) * (tau / (eps + b1));
+        const double a2 = 2.0 * b2, a1 = 1.0 * b1, ai = 1.0 / (a2 + a1);
+        return 0.5 * ai * (a2 * (-u0 + 3.0 * u1) + a1 * (u1 + u2));
+    }
+    if (RK == R_WENO5JS || RK == R_WENO5Z) {
+        // left: s0..s4 = q[c0-2 .. c0+2]; right: s5..s1 = q[c0+3 .. c0-1]
+        const long o = side ? c0 + 3 : c0 - 2;
+        const long d = side ? -1 : 1;
+        const double u0 = q[o], u1 = q[o + d], u2 = q[o + 2 * d], u3 = q[o + 3 * d], u4 = q[o + 4 * d];
+        const double c13 = 13.0 / 12.0;
+        double b3 = c13 * sq(u0 - 2.0 * u1 + u2) + 0.25 * sq(u0 - 4.0 * u1 + 3.0 * u2);
+        double b2 = c13 * sq(u1 - 2.0 * u2 + u3) + 0.25 * (u1 - u3) * (u1 - u3);
+        double b1 = c13 * sq(u2 - 2.0 * u3 + u4) + 0.25 * sq(3.0 * u2 - 4.0 * u3 + u4);
+        w5_weights<RK == R_WENO5JS ? 2 : 3>(b3, b2, b1);
+        const double a3 = b3, a2 = 6.0 * b2, a1 = 3.0 * b1, ai = 1.0 / (a3 + a2 + a1);
+        const double x2 = 2.0 * u0, x11 = 11.0 * u2;
+        const double t3 = ((side ? x11 : x2) - 7.0 * u1) + (side ? x2 : x11);
+        return (1.0 / 6.0) * ai *
+               (a3 * t3 + a2 * (-u1 + 5.0 * u2 + 2.0 * u3) + a1 * (2.0 * u2 + 5.0 * u3 - u4));
+    }
+    // WENO7-Z: left s0..s6 = q[c0-3 .. c0+3]; right s7 s6 .. s1 with s7 = s6 = q[c0+3]
+    // (SRC/flux.f90:232)
+    const long o = side ? c0 + 3 : c0 - 3;
+    const long d = side ? -1 : 1;
+    const double v0 = q[o];
+    const double v1 = side ? v0 : q[o + d];
+    const long o1 = side ? o : o + d;         // element 1 of the side's stencil
+    return w7_side(v0, v1, q[o1 + d], q[o1 + 2 * d], q[o1 + 3 * d], q[o1 + 4 * d], q[o1 + 5 * d]);
+}
+
 }  // namespace fcfd

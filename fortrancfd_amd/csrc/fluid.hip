@@ -92,20 +92,14 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
     const DevThermo *__restrict__ m = a.thermo;
     const double *__restrict__ q = a.qf;
 
-    double l, r;
-    reconstruct<RK>(q + 0 * a.P, c0, a.muscl, l, r);
-    const double rho_s = s ? r : l;
-    reconstruct<RK>(q + 1 * a.P, c0, a.muscl, l, r);
-    const double u_s = s ? r : l;
-    reconstruct<RK>(q + 2 * a.P, c0, a.muscl, l, r);
-    const double T_s = s ? r : l;
+    // each lane reconstructs its own side only
+    const double rho_s = reconstruct_side<RK>(q + 0 * a.P, c0, a.muscl, s);
+    const double u_s = reconstruct_side<RK>(q + 1 * a.P, c0, a.muscl, s);
+    const double T_s = reconstruct_side<RK>(q + 2 * a.P, c0, a.muscl, s);
     // reconstructed mass fractions of this lane's side, species-major in LDS (column = lane)
     __shared__ double ys[NSP * 64];
     double *y = ys + threadIdx.x;
-    for (int k = 0; k < NSP; ++k) {
-        reconstruct<RK>(q + (3 + k) * a.P, c0, a.muscl, l, r);
-        y[k * 64] = s ? r : l;
-    }
+    for (int k = 0; k < NSP; ++k) y[k * 64] = reconstruct_side<RK>(q + (3 + k) * a.P, c0, a.muscl, s);
     const double *yo_col = ys + (threadIdx.x ^ 1);     // the pair partner's side
     const Side me = side_state<NSP, 64>(m, rho_s, u_s, T_s, y);
     const Side ot = pair_other(me);
