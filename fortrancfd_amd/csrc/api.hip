@@ -1,6 +1,11 @@
 // api.hip -- C ABI of libfcfd_hip.so (declared in include/fcfd.h).
+#include <dlfcn.h>
+
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -83,6 +88,7 @@ struct fcfd_ctx {
     unsigned long long *d_cvel = nullptr;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     double graph_cfl = -1.0;
+    int graph_tune_gen = -1;             // tuning table generation the graphs were captured with
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, int>> ev_pending;   // (timer id, event index of the start event)
@@ -165,8 +171,85 @@ int ensure_dev_tables(const fcfd_mech *cm) {
     return FCFD_OK;
 }
 
-// Persistent grid size: four one-wave workgroups per CU (LDS-limited).
-int chem_workgroups(const fcfd_mech *m) { return 4 * (m->ncu > 0 ? m->ncu : 256); }
+// ---- launch shapes chosen by the tuner (fortrancfd_amd/tune.py) ------------------------------
+// A text table of "<arch> <species count> <key> <value>" lines: by default data/tuned_launch.txt
+// next to the library's directory (FCFD_TUNE_FILE overrides), read once per process; entries for
+// other GPU architectures are ignored.  Keys: flux_variant, stage_variant (fluid.hpp),
+// chem_wg_per_cu (integrator workgroups per CU), chem_four_wave_factor (four-wave mode for lists of
+// at most factor x workgroups cells).
+struct TuneTable {
+    std::mutex mu;
+    bool loaded = false;
+    std::map<std::string, int> v;   // "<nsp>:<key>"
+    int gen = 0;                    // bumped by every change, so contexts re-capture their graphs
+    std::string source;
+};
+
+TuneTable &tune_table() {
+    static TuneTable t;
+    return t;
+}
+
+std::string device_arch() {
+    int dev = 0;
+    hipDeviceProp_t p{};
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return "none";
+    std::string a(p.gcnArchName);
+    return a.substr(0, a.find(':'));
+}
+
+std::string default_tune_path() {
+    if (const char *e = std::getenv("FCFD_TUNE_FILE")) return e;
+    Dl_info info{};
+    if (!dladdr(reinterpret_cast<void *>(&chem_coop_max), &info) || !info.dli_fname) return "";
+    std::string lib(info.dli_fname);
+    const size_t slash = lib.rfind('/');
+    const std::string dir = slash == std::string::npos ? "." : lib.substr(0, slash);
+    return dir + "/../data/tuned_launch.txt";
+}
+
+// Replaces the table with the entries of path for this device's architecture; false if unreadable.
+bool load_tuning_locked(TuneTable &t, const std::string &path) {
+    t.loaded = true;
+    t.v.clear();
+    t.source = path;
+    ++t.gen;
+    std::ifstream f(path);
+    if (!f) return false;
+    const std::string arch = device_arch();
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        std::istringstream in(line);
+        std::string a, key;
+        int nsp = 0, val = 0;
+        if (!(in >> a >> nsp >> key >> val)) continue;
+        if (a == arch) t.v[std::to_string(nsp) + ":" + key] = val;
+    }
+    return true;
+}
+
+int tuned(int nsp, const char *key, int dflt) {
+    TuneTable &t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (!t.loaded) load_tuning_locked(t, default_tune_path());
+    auto it = t.v.find(std::to_string(nsp) + ":" + key);
+    return it == t.v.end() ? dflt : it->second;
+}
+
+int tune_gen() {
+    TuneTable &t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    return t.gen;
+}
+
+int chem_coop_max(int n_workgroups, int nsp) { return tuned(nsp, "chem_four_wave_factor", 8) * n_workgroups; }
+
+// Persistent grid size: one-wave workgroups per CU (default four, LDS-limited) times the CUs.
+int chem_workgroups(const fcfd_mech *m) {
+    return tuned(m->host.kk, "chem_wg_per_cu", 4) * (m->ncu > 0 ? m->ncu : 256);
+}
 
 }  // namespace fcfd
 
@@ -219,6 +302,7 @@ FluxArgs flux_args(fcfd_ctx *c, bool god) {
     a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P; a.PF = c->PF;
     a.muscl = c->o.flag_muscl; a.riemann = c->o.select_riemann;
     a.spherical = c->o.is_spherical;
+    a.variant = tuned(c->kk, "flux_variant", 0);
     return a;
 }
 
@@ -228,6 +312,7 @@ StageArgs stage_args(fcfd_ctx *c, int mode, const double *Q0, const double *Qin,
     a.thermo = c->thermo; a.Q0 = Q0; a.Qin = Qin; a.F = c->F; a.Qout = Qout; a.qf = c->qf;
     a.dt_dev = dt_dev; a.dt = dt; a.dx = c->dx; a.h1 = h1; a.h2 = h2; a.mode = mode; a.ghosts_only = 0;
     a.spherical = c->o.is_spherical; a.xm = c->d_xm; a.god = c->god; a.dxb = c->d_dxb;
+    a.variant = tuned(c->kk, "stage_variant", 0);
     a.nblk = c->nblk; a.ncell = c->ncell; a.lv = c->lv; a.S = c->S; a.P = c->P; a.PF = c->PF;
     return a;
 }
@@ -396,7 +481,7 @@ hipError_t enqueue_source(fcfd_ctx *c, double dt, const double *dt_dev, bool wan
         r.stiff = c->d_list2;
         r.stiff_n = c->d_list2_n;
         r.retry_done = c->d_retry;
-        r.coop_max = chem_coop_max(chem_wg);
+        r.coop_max = chem_coop_max(chem_wg, c->kk);
         return launch_chem_retry(r, c->kk, c->stream);
     });
     if (e != hipSuccess) return e;
@@ -906,7 +991,12 @@ int fcfd_advance(fcfd_ctx *c, double cfl, int nsteps, double *time) {
         for (int s = 0; s < nsteps; ++s) HIP_TRY(enqueue_step(c, cfl));
         HIP_TRY(collect_timing(c));
     } else {
-        if (cfl != c->graph_cfl) { drop_graphs(c); c->graph_cfl = cfl; }
+        const int gen = tune_gen();
+        if (cfl != c->graph_cfl || gen != c->graph_tune_gen) {
+            drop_graphs(c);
+            c->graph_cfl = cfl;
+            c->graph_tune_gen = gen;
+        }
         for (int s = 0; s < nsteps; ++s) {
             const int cur = c->cur;
             if (!c->graph[cur]) {
@@ -1240,6 +1330,41 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s) {
     }
     drop_graphs(c);
     c->sur = s;
+    return FCFD_OK;
+}
+
+int fcfd_set_launch(const char *key, int nsp, int value) {
+    static const char *const keys[] = {"flux_variant", "stage_variant", "chem_wg_per_cu", "chem_four_wave_factor"};
+    static const int lo[] = {0, 0, 1, 0}, hi[] = {2, 3, 64, 1 << 20};
+    if (!key) { set_error("fcfd_set_launch: invalid key"); return FCFD_E_ARG; }
+    for (int i = 0; i < 4; ++i)
+        if (std::strcmp(key, keys[i]) == 0) {
+            if (value < lo[i] || value > hi[i]) { set_error(std::string("fcfd_set_launch: ") + key + " out of range"); return FCFD_E_ARG; }
+            tuned(nsp, key, 0);                       // loads the file first, so it does not overwrite this later
+            TuneTable &t = tune_table();
+            std::lock_guard<std::mutex> g(t.mu);
+            t.v[std::to_string(nsp) + ":" + key] = value;
+            ++t.gen;
+            return FCFD_OK;
+        }
+    set_error(std::string("fcfd_set_launch: unknown key ") + key);
+    return FCFD_E_ARG;
+}
+
+int fcfd_get_launch(const char *key, int nsp, int *value) {
+    if (!key || !value) { set_error("fcfd_get_launch: invalid argument"); return FCFD_E_ARG; }
+    const int dflt = std::strcmp(key, "chem_wg_per_cu") == 0 ? 4 : (std::strcmp(key, "chem_four_wave_factor") == 0 ? 8 : 0);
+    *value = tuned(nsp, key, dflt);
+    return FCFD_OK;
+}
+
+int fcfd_load_tuning(const char *path) {
+    TuneTable &t = tune_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    if (!load_tuning_locked(t, path ? std::string(path) : default_tune_path())) {
+        set_error(std::string("fcfd_load_tuning: cannot read ") + (path ? path : t.source.c_str()));
+        return FCFD_E_IO;
+    }
     return FCFD_OK;
 }
 

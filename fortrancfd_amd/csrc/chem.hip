@@ -1001,7 +1001,7 @@ hipError_t launch_chem(const ChemArgs &a0, int nsp, int n_workgroups, hipStream_
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     ChemArgs a = a0;
-    a.coop_max = chem_coop_max(n_workgroups);
+    a.coop_max = chem_coop_max(n_workgroups, nsp);
     if (a.order && a.list && a.list_n && a.stats && a.layout == 0) {
         hipLaunchKernelGGL(k_heavy_first, dim3(1), dim3(1024), 0, st, a.list, a.list_n, a.stats, a.order);
         if ((e = hipGetLastError()) != hipSuccess) return e;

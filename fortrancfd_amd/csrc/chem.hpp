@@ -95,7 +95,9 @@ hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st);
 // the launch) and *a.retry_done = 1; a shorter list is copied to a.stiff unchanged, *a.retry_done = 0.
 hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st);
 // Length up to which a stiff list runs four waves per cell in launch_chem.
-inline int chem_coop_max(int n_workgroups) { return 8 * n_workgroups; }
+// Longest list that the four-wave integrator mode takes: the tuned factor (default 8) times the
+// workgroup count (api.hip).
+int chem_coop_max(int n_workgroups, int nsp);
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
 // *out = f * *in on the device (one thread), e.g. the half step of Strang splitting.
 hipError_t launch_scale_scalar(const double *in, double *out, double f, hipStream_t st);

@@ -77,8 +77,8 @@ __device__ __forceinline__ Side pair_other(const Side &me) {
     return o;
 }
 
-template <int NSP, int RK, bool SPH>
-__global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
+template <int NSP, int RK, bool SPH, int BS = 64, int WPE = 2>
+__global__ void __launch_bounds__(BS, WPE) k_flux(FluxArgs a) {
     const long g2 = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long g = g2 >> 1;                               // interface (pairs never straddle a wave)
     const int s = int(threadIdx.x & 1);                   // 0: left state, 1: right state
@@ -97,11 +97,11 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
     const double u_s = reconstruct_side<RK>(q + 1 * a.P, c0, a.muscl, s);
     const double T_s = reconstruct_side<RK>(q + 2 * a.P, c0, a.muscl, s);
     // reconstructed mass fractions of this lane's side, species-major in LDS (column = lane)
-    __shared__ double ys[NSP * 64];
+    __shared__ double ys[NSP * BS];
     double *y = ys + threadIdx.x;
-    for (int k = 0; k < NSP; ++k) y[k * 64] = reconstruct_side<RK>(q + (3 + k) * a.P, c0, a.muscl, s);
+    for (int k = 0; k < NSP; ++k) y[k * BS] = reconstruct_side<RK>(q + (3 + k) * a.P, c0, a.muscl, s);
     const double *yo_col = ys + (threadIdx.x ^ 1);     // the pair partner's side
-    const Side me = side_state<NSP, 64>(m, rho_s, u_s, T_s, y);
+    const Side me = side_state<NSP, BS>(m, rho_s, u_s, T_s, y);
     const Side ot = pair_other(me);
     const Side L = s ? ot : me;
     const Side R = s ? me : ot;
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
         put(F[2 * PF], 2, 0.5 * (fl2 + fr2 - al * (R.E - L.E)));
 #pragma unroll
         for (int k = 0; k < NSP; ++k) {
-            const double ym = y[k * 64], yo = yo_col[k * 64];
+            const double ym = y[k * BS], yo = yo_col[k * BS];
             const double yl = s ? yo : ym, yr = s ? ym : yo;
             const double QL = L.rho * (fmax(yl, 0.0) / L.sy), QR = R.rho * (fmax(yr, 0.0) / R.sy);
             const double fL = L.rho * yl * L.u, fR = R.rho * yr * R.u;
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
             put(F[2 * PF], 2, hll(fl2, fr2, L.E, R.E));
 #pragma unroll
             for (int k = 0; k < NSP; ++k) {
-                const double ym = y[k * 64], yo = yo_col[k * 64];
+                const double ym = y[k * BS], yo = yo_col[k * BS];
                 const double yl = s ? yo : ym, yr = s ? ym : yo;
                 const double QL = L.rho * (fmax(yl, 0.0) / L.sy), QR = R.rho * (fmax(yr, 0.0) / R.sy);
                 put(F[(3 + k) * PF], 3 + k, hll(L.rho * yl * L.u, R.rho * yr * R.u, QL, QR));
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
                 put(F[2 * PF], 2, f2);
 #pragma unroll
                 for (int k = 0; k < NSP; ++k) {
-                    const double yk = (left == (s == 0)) ? y[k * 64] : yo_col[k * 64];
+                    const double yk = (left == (s == 0)) ? y[k * BS] : yo_col[k * BS];
                     put(F[(3 + k) * PF], 3 + k, S.rho * yk * S.u);
                 }
             } else {                                              // star state of the upwind side
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(64, 2) k_flux(FluxArgs a) {
                 put(F[2 * PF], 2, f2 + sK * (c * v2 - S.E));
 #pragma unroll
                 for (int k = 0; k < NSP; ++k) {
-                    const double yk = (left == (s == 0)) ? y[k * 64] : yo_col[k * 64];
+                    const double yk = (left == (s == 0)) ? y[k * BS] : yo_col[k * BS];
                     const double Qk = S.rho * (fmax(yk, 0.0) / S.sy);
                     const double fk = S.rho * yk * S.u;
                     put(F[(3 + k) * PF], 3 + k, fk + sK * (c * (Qk / S.rho) - Qk));
@@ -239,8 +239,8 @@ __device__ __forceinline__ void ghost_source(const StageArgs &a, int b, int i, i
 // of SRC/timeintegration.f90:150-175 (or :48-56 for Euler), then update_boundary /
 // interpolation_block / QcToqf on every column (:201-215).  mode 0: Q = Qin - dt/dx*dF;
 // mode 1: Q = h1*Q0 + h2*(Qin - dt/dx*dF); mode 2: no update (ghost fill + QcToqf only).
-template <int NSP>
-__global__ void __launch_bounds__(64, 2) k_stage_end(StageArgs a) {
+template <int NSP, int BS = 64, int WPE = 2>
+__global__ void __launch_bounds__(BS, WPE) k_stage_end(StageArgs a) {
     const long c = long(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= long(a.nblk) * a.S) return;
     const int b = int(c / a.S), i = int(c % a.S) - a.lv;
@@ -566,12 +566,21 @@ template <int NSP, bool SPH>
 static hipError_t launch_flux_nsp(const FluxArgs &a, int rk, hipStream_t st) {
     const long n = 2 * long(a.nblk) * (a.ncell + 1);            // two lanes per interface
     const dim3 grid(unsigned((n + 63) / 64)), blk(64);
+    const dim3 grid2(unsigned((n + 127) / 128)), blk2(128);
     switch (rk) {
         case R_FIRST: hipLaunchKernelGGL((k_flux<NSP, R_FIRST, SPH>), grid, blk, 0, st, a); break;
         case R_MUSCL: hipLaunchKernelGGL((k_flux<NSP, R_MUSCL, SPH>), grid, blk, 0, st, a); break;
         case R_WENO3Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO3Z, SPH>), grid, blk, 0, st, a); break;
-        case R_WENO5JS: hipLaunchKernelGGL((k_flux<NSP, R_WENO5JS, SPH>), grid, blk, 0, st, a); break;
-        case R_WENO5Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO5Z, SPH>), grid, blk, 0, st, a); break;
+        case R_WENO5JS:
+            if (a.variant == 1) hipLaunchKernelGGL((k_flux<NSP, R_WENO5JS, SPH, 64, 1>), grid, blk, 0, st, a);
+            else if (a.variant == 2) hipLaunchKernelGGL((k_flux<NSP, R_WENO5JS, SPH, 128, 1>), grid2, blk2, 0, st, a);
+            else hipLaunchKernelGGL((k_flux<NSP, R_WENO5JS, SPH>), grid, blk, 0, st, a);
+            break;
+        case R_WENO5Z:
+            if (a.variant == 1) hipLaunchKernelGGL((k_flux<NSP, R_WENO5Z, SPH, 64, 1>), grid, blk, 0, st, a);
+            else if (a.variant == 2) hipLaunchKernelGGL((k_flux<NSP, R_WENO5Z, SPH, 128, 1>), grid2, blk2, 0, st, a);
+            else hipLaunchKernelGGL((k_flux<NSP, R_WENO5Z, SPH>), grid, blk, 0, st, a);
+            break;
         case R_WENO7Z: hipLaunchKernelGGL((k_flux<NSP, R_WENO7Z, SPH>), grid, blk, 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
@@ -599,10 +608,22 @@ hipError_t launch_flux(const FluxArgs &a, int nsp, int flag_acu, int flag_weno, 
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_stage_end(const StageArgs &a, int nsp, hipStream_t st) {
+template <int NSP>
+static hipError_t launch_stage_end_nsp(const StageArgs &a, hipStream_t st) {
     const long n = long(a.nblk) * a.S;
-    const dim3 grid(unsigned((n + 63) / 64)), blk(64);
-#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_stage_end<N>), grid, blk, 0, st, a); return hipGetLastError(); }
+    const int bs = a.variant == 2 ? 128 : (a.variant == 3 ? 256 : 64);
+    const dim3 grid(unsigned((n + bs - 1) / bs)), blk(bs);
+    switch (a.variant) {
+        case 1: hipLaunchKernelGGL((k_stage_end<NSP, 64, 1>), grid, blk, 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_stage_end<NSP, 128, 1>), grid, blk, 0, st, a); break;
+        case 3: hipLaunchKernelGGL((k_stage_end<NSP, 256, 1>), grid, blk, 0, st, a); break;
+        default: hipLaunchKernelGGL((k_stage_end<NSP>), grid, blk, 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_end(const StageArgs &a, int nsp, hipStream_t st) {
+#define FCFD_CASE(N) if (nsp == N) return launch_stage_end_nsp<N>(a, st);
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;

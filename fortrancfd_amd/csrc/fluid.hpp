@@ -16,6 +16,7 @@ struct FluxArgs {
     int accumulate;       // 1: F += advection flux (transport flux already in F), 0: F = flux
     int spherical;        // 1: momentum flux rho u u without p (SRC/flux.f90:272-278)
     int b0;               // first block of this launch (nblk blocks from there)
+    int variant;          // launch shape (WENO5 only): 0 64 threads, 2 waves/EU; 1 64, 1; 2 128, 1
 };
 
 struct StageArgs {
@@ -31,6 +32,7 @@ struct StageArgs {
     const double *xm;      // [nblk * ncell] cell centres (spherical)
     const double *god;     // [3][PF] Roe states of the interfaces; [2] the pressure (spherical)
     const double *dxb;     // [nblk] per-block cell size (nullptr: dx everywhere)
+    int variant;           // launch shape of k_stage_end: 0 64 threads, 2 waves/EU; 1 64, 1; 2 128, 1; 3 256, 1
     int nblk, ncell, lv, S;
     long P, PF;
 };

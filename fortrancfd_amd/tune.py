@@ -1,0 +1,181 @@
+"""Launch-shape tuner: times the candidate shapes of the hot kernels on this GPU and writes the
+table that the library reads at start-up (fcfd_set_launch / fcfd_load_tuning).
+
+    python -m fortrancfd_amd.tune [--out fortrancfd_amd/data/tuned_launch.txt] [--steps N]
+
+Candidates and the workloads they are timed on (GRI-3.0):
+  * flux_variant (WENO5 advection flux: 64 threads at two or one waves per SIMD, 128 threads) and
+    stage_variant (stage update: 64 / two, 64 / one, 128 / one, 256 / one) on inert hydrodynamics,
+    64 blocks x 4096 cells, WENO5-JS + HLLC + SSP-RK3, kernel times from the context's HIP events;
+  * chem_wg_per_cu (implicit-integrator workgroups per CU) on a long stiff list: 2^16 independent
+    cells of the chemistry batch generator (fortrancfd_amd.workloads.chemistry_batch);
+  * chem_four_wave_factor (the list length below which the integrator gives each cell four waves)
+    on a short stiff list -- the coupled 65536-cell domain after --steps Strang steps, one
+    solve_source of the saved state per candidate -- plus a 4096-cell batch, a list length where the
+    candidates choose different modes.
+A candidate replaces the default only when it is more than 2 % faster.
+Each candidate is timed three times and scored by its median; the table records the winners and,
+as comments, every candidate's time.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import os
+import time
+
+import numpy as np
+
+from .lib import GRI30, Mechanism, Solver, _check, lib
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(PKG, "data", "tuned_launch.txt")
+
+
+def set_launch(key, nsp, value):
+    _check(lib().fcfd_set_launch(key.encode(), C.c_int(nsp), C.c_int(value)))
+
+
+def median(xs):
+    return float(np.median(np.asarray(xs)))
+
+
+def pick(res, default, margin=0.02):
+    """The fastest candidate, or the default unless another beats it by more than margin."""
+    best = min(res, key=res.get)
+    return best if res[best] < (1.0 - margin) * res[default] else default
+
+
+def tune_fluid(mech, log):
+    from .workloads import coupled_cells
+    nblk, ncell = 64, 4096
+    cells = coupled_cells(mech, nblk, ncell, nhot=0)
+    out = {}
+    for key, kid, cands in (("flux_variant", 0, (0, 1, 2)), ("stage_variant", 1, (0, 1, 2, 3))):
+        res = {}
+        for v in cands:
+            set_launch(key, mech.kk, v)
+            s = Solver(mech, nblk, ncell, 25e-6, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1)
+            s.init_cells(cells)
+            s.set_timing(True)
+            s.advance(0.5, 2)                                   # warm-up
+            reps = []
+            for _ in range(3):
+                ms0, n0 = s.timing(kid)
+                s.advance(0.5, 2)
+                ms1, n1 = s.timing(kid)
+                reps.append((ms1 - ms0) / max(n1 - n0, 1))
+            res[v] = median(reps)
+            log(f"{key}={v}: {res[v]:.4f} ms per launch")
+            del s
+        best = pick(res, 0)
+        set_launch(key, mech.kk, best)
+        out[key] = (best, res)
+    return out
+
+
+def time_batch(mech, n, seed, key, cands, log, what):
+    import torch
+    from .workloads import chemistry_batch
+    rho, dt, T, Y = chemistry_batch(mech, n, seed=seed)
+    d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    res = {}
+    for v in cands:
+        set_launch(key, mech.kk, v)
+        reps = []
+        for _ in range(3):
+            tr, tdt, Tf, Yf = d(rho), d(dt), d(T), d(Y.T)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            mech.chem_batch_device(tr, tdt, Tf, Yf)
+            torch.cuda.synchronize()
+            reps.append((time.perf_counter() - t0) * 1e3)
+        res[v] = median(reps)
+        log(f"{key}={v}: {res[v]:.2f} ms for {n} {what}")
+    return res
+
+
+def tune_chem_long(mech, log, n=1 << 16):
+    from .workloads import SEED
+    res = time_batch(mech, n, SEED + 11, "chem_wg_per_cu", (3, 4, 5), log, "batch cells")
+    best = pick(res, 4)
+    set_launch("chem_wg_per_cu", mech.kk, best)
+    return best, res
+
+
+def tune_chem_short(mech, log, steps, wg_per_cu, ncu):
+    from .workloads import coupled_cells
+    cells = coupled_cells(mech)
+    s = Solver(mech, 16, 4096, 25e-6, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1,
+               do_reaction=1)
+    s.set_step(strang=True, hrr_each_step=False)
+    s.init_cells(cells)
+    t = s.advance(0.5, steps)
+    Qc, qf = s.download()
+    dt = s.calc_dt(0.5)
+    s.solve_source(dt * 0.5, t, diagnostics=False)
+    acc, nimp = s.source_accepted()
+    log(f"coupled domain after {steps} steps: {nimp} cells on the implicit integrator")
+    s.set_timing(True)
+    res = {}
+    for v in (0, 1, 2, 4, 8):
+        set_launch("chem_four_wave_factor", mech.kk, v)
+        reps = []
+        for _ in range(3):
+            s.upload(Qc, qf)
+            ms0, _ = s.timing(3)
+            s.solve_source(dt * 0.5, t, diagnostics=False)
+            ms1, _ = s.timing(3)
+            reps.append(ms1 - ms0)
+        res[v] = median(reps)
+        log(f"chem_four_wave_factor={v}: {res[v]:.3f} ms (implicit integrator, {nimp} cells)")
+    # a list between the two modes' ranges: 4096 batch cells, almost all on the integrator
+    from .workloads import SEED
+    nmid = 4096
+    mid = time_batch(mech, nmid, SEED + 13, "chem_four_wave_factor", (0, 8), log, "batch cells")
+    # factor v puts that list on four waves per cell when nmid <= v x workgroups
+    res_all = {v: res[v] + mid[8 if nmid <= v * wg_per_cu * ncu else 0] for v in res}
+    best = pick(res_all, 8)
+    set_launch("chem_four_wave_factor", mech.kk, best)
+    return best, res_all, nimp
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--steps", type=int, default=2000, help="coupled steps before the short-list timing")
+    args = ap.parse_args(argv)
+    import torch
+    assert torch.cuda.is_available(), "the tuner needs a GPU"
+    arch = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    mech = Mechanism(GRI30)
+    lines = []
+
+    def log(msg):
+        print(msg, flush=True)
+        lines.append("# " + msg)
+
+    # start from the defaults, whatever table is installed
+    for key, v in (("flux_variant", 0), ("stage_variant", 0), ("chem_wg_per_cu", 4), ("chem_four_wave_factor", 8)):
+        set_launch(key, mech.kk, v)
+    fl = tune_fluid(mech, log)
+    wg, _ = tune_chem_long(mech, log)
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    fw, _, _ = tune_chem_short(mech, log, args.steps, wg, ncu)
+    table = [f"{arch} {mech.kk} flux_variant {fl['flux_variant'][0]}",
+             f"{arch} {mech.kk} stage_variant {fl['stage_variant'][0]}",
+             f"{arch} {mech.kk} chem_wg_per_cu {wg}",
+             f"{arch} {mech.kk} chem_four_wave_factor {fw}"]
+    with open(args.out, "w") as f:
+        f.write("# Launch shapes chosen by python -m fortrancfd_amd.tune (median of three timings each).\n")
+        f.write("# Lines: <arch> <species count> <key> <value>; read by libfcfd_hip.so at start-up.\n")
+        for ln in lines:
+            f.write(ln + "\n")
+        for ln in table:
+            f.write(ln + "\n")
+    print("\n".join(table), flush=True)
+    return table
+
+
+if __name__ == "__main__":
+    main()

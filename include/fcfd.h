@@ -218,6 +218,20 @@ int fcfd_get_hrr(fcfd_ctx *c, double *hrr, double *hrr_dot);
  * explicit screen, 7 transport flux, 8 chemistry explicit half-step retry of a long stiff list,
  * 9 auxiliary sources. */
 int fcfd_set_timing(fcfd_ctx *c, int enable);
+/* Launch shapes (no counterpart in the reference, whose CPU loops have none).  Read once per process
+ * from a table of "<arch> <species count> <key> <value>" lines written by the tuner
+ * (python -m fortrancfd_amd.tune): data/tuned_launch.txt beside the library directory, or the file
+ * FCFD_TUNE_FILE names; entries of other GPU architectures are ignored ("none" stands for a host
+ * without a GPU), absent keys keep their
+ * defaults.  Keys: "flux_variant" (WENO5 advection flux: 0 = 64 threads and two waves per SIMD,
+ * 1 = 64 / one, 2 = 128 / one), "stage_variant" (stage update: 0 = 64 / two, 1 = 64 / one,
+ * 2 = 128 / one, 3 = 256 / one), "chem_wg_per_cu" (implicit-integrator workgroups per CU, default 4),
+ * "chem_four_wave_factor" (four-wave integrator mode for lists of at most factor x workgroups cells,
+ * default 8).  fcfd_set_launch overrides one entry for the mechanisms of nsp species (every context
+ * re-captures its step graph); fcfd_load_tuning reloads the table (NULL: the default file). */
+int fcfd_set_launch(const char *key, int nsp, int value);
+int fcfd_get_launch(const char *key, int nsp, int *value);
+int fcfd_load_tuning(const char *path);
 int fcfd_get_timing(fcfd_ctx *c, int kernel_id, double *ms_total, long long *launches);
 
 /* ---- stateless entry points (host arrays in, host arrays out; run on the current device) ----

@@ -60,3 +60,23 @@ def test_species_names_from_link_file():
     import bench
     from fortrancfd_amd.lib import GRI30
     assert bench.species_names(GRI30) == list(golden("mech")["names"])
+
+
+def test_launch_table_file_and_overrides(tmp_path):
+    """The launch-shape table: entries of this host's architecture ("none" without a GPU) are read,
+    others ignored; fcfd_set_launch checks its key and range; fcfd_load_tuning(NULL) goes back to
+    the installed default table."""
+    from fortrancfd_amd.lib import lib
+    L = lib()
+    get = lambda key, nsp=53: (lambda v: (L.fcfd_get_launch(key.encode(), nsp, C.byref(v)), v.value)[1])(C.c_int())
+    f = tmp_path / "t.txt"
+    f.write_text("# comment\nnone 53 flux_variant 2\ngfx000 53 stage_variant 3\nnone 11 chem_wg_per_cu 5  # x\n")
+    assert L.fcfd_load_tuning(str(f).encode()) == 0
+    assert get("flux_variant") == 2 and get("stage_variant") == 0
+    assert get("chem_wg_per_cu", 11) == 5 and get("chem_wg_per_cu") == 4 and get("chem_four_wave_factor") == 8
+    assert L.fcfd_set_launch(b"stage_variant", 53, 3) == 0 and get("stage_variant") == 3
+    assert L.fcfd_set_launch(b"stage_variant", 53, 4) != 0
+    assert L.fcfd_set_launch(b"no_such_key", 53, 1) != 0
+    assert L.fcfd_load_tuning(str(tmp_path / "missing.txt").encode()) != 0
+    L.fcfd_load_tuning(None)
+    assert get("stage_variant") in (0, 1, 2, 3)
