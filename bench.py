@@ -155,7 +155,7 @@ def measured_traffic(workload, kernel):
 
 TRAFFIC_PROFILES = {"inert1m": "fluid_inert1m_hbm_traffic.json",
                     "inert65536": "fluid_inert65536_hbm_traffic.json",
-                    "coupled65536": "fluid_inert65536_hbm_traffic.json"}
+                    "coupled65536": "coupled65536_hbm_traffic.json"}
 
 
 def roofline_for(kern, mech, ntot, nblk, ncell, screen_cells, workload="", sdirk_work=None):
@@ -228,6 +228,7 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
     sur_u = load_surrogate(mech, verify_tol=1e30)
 
     strang = not args.lie
+    calib_copy(args)
 
     def run(surrogate):
         s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1,
@@ -345,6 +346,17 @@ def bench_coupled(args, mech, nblk, ncell, dx, nhot, names):
     }
 
 
+def calib_copy(args):
+    """A device-to-device copy of known size: the reference point that scales the hardware byte
+    counters (FETCH_SIZE / WRITE_SIZE) of a profiled run to bytes."""
+    if args.calib_copy_mb:
+        import torch
+        src = torch.empty(args.calib_copy_mb << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+
+
 def bench_inert(args, mech, nblk, ncell, dx, nhot, names):
     import torch
     from fortrancfd_amd.lib import Solver
@@ -353,13 +365,7 @@ def bench_inert(args, mech, nblk, ncell, dx, nhot, names):
     s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1)
     s.init_cells(cells)
     cfl = 0.5
-    if args.calib_copy_mb:
-        # a device-to-device copy of known size: the reference point that scales the hardware
-        # byte counters (FETCH_SIZE / WRITE_SIZE) of a profiled run to bytes
-        src = torch.empty(args.calib_copy_mb << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+    calib_copy(args)
     t = s.advance(cfl, args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -468,7 +474,7 @@ def main():
                     help="coupled workload: the reference driver's Lie splitting instead of Strang")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="inert workloads: one device copy of this many MiB before the steps")
+                    help="one device copy of this many MiB before the steps (byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every block of the domain lives on one device and this "
