@@ -104,7 +104,14 @@ TIMERS = {0: "k_flux", 1: "k_stage_end", 2: "k_wave_speed+k_dt_finalize", 3: "k_
 # constant, equilibrium constant, rates of progress, six production-rate updates), per fall-off
 # reaction 30 more, per third-body efficiency 2 (about 8 per third-body reaction).
 def rhs_flops(kk=53, ii=325, nfall=29, ntb=41):
-    return 40 * kk + 14 * ii + 30 * nfall + 2 * 8 * ntb
+    """FP64 operations of one chemistry right-hand side (CKWYR + the energy equation) with the
+    exponentials at the cost of their evaluation (fm_exp: Cody-Waite reduction, 13-term Horner
+    series, reconstruction ~45 operations; fm_log ~50): per species 60 (NASA-7 thermo, S/R - H/RT,
+    concentrations, production-rate sum), per reaction 14 (concentration products, rate assembly)
+    plus one exponential for the Arrhenius factor and one for the equilibrium constant, per fall-off
+    reaction ~250 (low-pressure Arrhenius exponential, log10 of Pr, the Troe centre's exponentials,
+    its log10 and the blending exp10), per third-body efficiency 2 (8 per third-body row)."""
+    return 60 * kk + 14 * ii + 2 * 45 * ii + 250 * nfall + 2 * 8 * ntb
 
 
 def sdirk_flops(st, kk=53, ii=325):
