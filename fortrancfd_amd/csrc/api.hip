@@ -176,7 +176,7 @@ int ensure_dev_tables(const fcfd_mech *cm) {
 // next to the library's directory (FCFD_TUNE_FILE overrides), read once per process; entries for
 // other GPU architectures are ignored.  Keys: flux_variant, stage_variant (fluid.hpp),
 // chem_wg_per_cu (integrator workgroups per CU), chem_four_wave_factor (four-wave mode for lists of
-// at most factor x workgroups cells).
+// at most factor x workgroups cells), chem_lu (1: the four-wave integrator's blocked matrix-core LU).
 struct TuneTable {
     std::mutex mu;
     bool loaded = false;
@@ -245,6 +245,8 @@ int tune_gen() {
 }
 
 int chem_coop_max(int n_workgroups, int nsp) { return tuned(nsp, "chem_four_wave_factor", 8) * n_workgroups; }
+
+int chem_lu_variant(int nsp) { return tuned(nsp, "chem_lu", 0); }
 
 // Persistent grid size: one-wave workgroups per CU (default four, LDS-limited) times the CUs.
 int chem_workgroups(const fcfd_mech *m) {
@@ -1334,10 +1336,11 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s) {
 }
 
 int fcfd_set_launch(const char *key, int nsp, int value) {
-    static const char *const keys[] = {"flux_variant", "stage_variant", "chem_wg_per_cu", "chem_four_wave_factor"};
-    static const int lo[] = {0, 0, 1, 0}, hi[] = {2, 3, 64, 1 << 20};
+    static const char *const keys[] = {"flux_variant", "stage_variant", "chem_wg_per_cu", "chem_four_wave_factor",
+                                       "chem_lu"};
+    static const int lo[] = {0, 0, 1, 0, 0}, hi[] = {2, 3, 64, 1 << 20, 1};
     if (!key) { set_error("fcfd_set_launch: invalid key"); return FCFD_E_ARG; }
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 5; ++i)
         if (std::strcmp(key, keys[i]) == 0) {
             if (value < lo[i] || value > hi[i]) { set_error(std::string("fcfd_set_launch: ") + key + " out of range"); return FCFD_E_ARG; }
             tuned(nsp, key, 0);                       // loads the file first, so it does not overwrite this later

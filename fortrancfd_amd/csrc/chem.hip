@@ -130,6 +130,8 @@ struct Shared {
     double c[64], smh[64], u[64];
     double rdiag[64];                // 1 / pivot of each elimination step
     int perm[64];
+    int rank[64];                    // elimination step of each row (lu_blocked)
+    double U12[NW == 4 ? 4 : 1][1024];   // per-wave panel rows right of the panel (lu_blocked)
     int cell;
 };
 
@@ -387,6 +389,30 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
     return f0;
 }
 
+// The pivot of an elimination step: the open row with the largest |entry| rk in the pivot column, the
+// lowest such row on ties (NaN entries are passed over).  For |x| >= 0 the upper 32 bits of the
+// double order like the value, so one 32-bit wave max finds the candidates; only when several rows
+// share those bits are their full values compared.  If the column of every open row is NaN, the
+// lowest open row, and ok = false.  Called by every lane of the wave (wave-wide reductions).
+__device__ __forceinline__ int pivot_row(double rk, bool open, bool &ok) {
+    const double av = fabs(rk);
+    const bool num = open && av == av;
+    const unsigned key = num ? (unsigned)__double2hiint(av) : 0u;
+    const unsigned kmax = wave_max_u32(key);
+    const bool top = num && key == kmax;
+    unsigned long long hit = __ballot(top);
+    if (__popcll(hit) > 1) {
+        const double v = top ? av : -1.0;
+        const double vmax = wave_max(v);
+        hit = __ballot(top && v == vmax);
+    }
+    if (hit == 0ull) {
+        hit = __ballot(open);
+        ok = false;
+    }
+    return __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
+}
+
 // LU with partial pivoting of M = I - hg*J, in place in sh.J[0]: lane r owns row r (LDS row stride
 // odd in 64-bit words, so the lanes of one column hit distinct banks).  Rows are not swapped:
 // sh.perm[k] is the pivot row of step k; ties go to the lower row.  Every wave finds the same pivot
@@ -429,27 +455,7 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
         // that column in step k - 1), read before the pivot search so that the read overlaps it
         const int k1 = k + 1 < N ? k + 1 : k;
         const double next_r = row[k1];
-        // pivot: the unpivoted row with the largest |entry| in column k, the lowest such row on ties
-        // (NaN entries are passed over).  For |x| >= 0 the upper 32 bits of the double order like the
-        // value, so one 32-bit wave max finds the candidates; only when several rows share those bits
-        // are their full values compared.
-        const bool open = rank == N;
-        const double av = fabs(rk);
-        const bool num = open && av == av;
-        const unsigned key = num ? (unsigned)__double2hiint(av) : 0u;
-        const unsigned kmax = wave_max_u32(key);   // outside any condition: every lane takes part
-        const bool top = num && key == kmax;
-        unsigned long long hit = __ballot(top);
-        if (__popcll(hit) > 1) {
-            const double v = top ? av : -1.0;
-            const double vmax = wave_max(v);
-            hit = __ballot(top && v == vmax);
-        }
-        if (hit == 0ull) {                      // column k of every open row is NaN
-            hit = __ballot(open);
-            ok = false;
-        }
-        const int p = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
+        const int p = pivot_row(rk, rank == N, ok);
         const double *prow = sh.J[0] + p * LDJ;
         const double piv = bcast(rk, p);
         const double next_p = bcast(next_r, p);   // the pivot row's entry in column k + 1
@@ -472,6 +478,148 @@ __device__ __forceinline__ bool lu(Shared<NSP, NW> &sh, double hg, int wave) {
     // reciprocal pivots, all at once: the pivot of step k is entry k of row perm[k], which no later
     // step changes
     if (wave == 0 && lane < N) sh.rdiag[lane] = 1.0 / sh.J[0][sh.perm[lane] * LDJ + lane];
+    __syncthreads();
+    return ok;
+}
+
+typedef double fcfd_d4 __attribute__((ext_vector_type(4)));
+
+// lu() for the four-wave workgroup, blocked: the same factors, bit for bit, with the elimination of
+// each 16-column panel [c0, c1) in registers and the update of the columns right of it as matrix
+// products on the FP64 matrix cores.  Per panel:
+//  1. every wave factors the panel redundantly: lane r holds row r's panel entries and runs lu()'s
+//     steps c0 .. c1-1 on them (the same pivots, multipliers and fused multiply-adds); wave 0 stores
+//     the panel's final values, sh.perm and each row's step (sh.rank); barrier;
+//  2. every wave forms the panel's pivot rows right of the panel (U12): row perm[k] after the updates
+//     of steps c0 .. k-1, one column per lane, the fused multiply-adds in step order, into its own
+//     block sh.U12[wave];
+//  3. wave w updates the rows 16w .. 16w+15 that are still open, right of the panel, with
+//     v_mfma_f64_16x16x4f64 (D = C + A B, A = minus the multipliers of four steps, B = those steps'
+//     rows of U12), four instructions per 16-column tile in step order; barrier.
+// v_mfma_f64_16x16x4f64 forms each element of D as fused multiply-adds in k order (checked on the
+// GPU against sequential fma for every element of random 16x4x16 products), so each entry of M
+// undergoes the same operations, in the same order, as under lu()'s rank-one updates.  Wave 0 writes
+// a panel's U12 into its pivot rows after that panel's last barrier, when no wave reads them any
+// more.  Columns N .. 63 of the open rows receive values that nothing reads.  N <= 64.
+template <int NSP>
+__device__ __forceinline__ bool lu_blocked(Shared<NSP, 4> &sh, double hg, int wave) {
+    constexpr int N = NSP + 1;
+    static_assert(N <= 64, "one row per lane");
+    const int lane = lane_id();
+    int rank = lane < N ? N : -1;
+    double *const M = sh.J[0];
+    double *row = M + (lane < N ? lane : 0) * LDJ;
+    // M = I - hg*J as lu() forms it (column j by wave j mod 4)
+    if (lane < N) {
+        for (int j = wave; j < N; j += 4) {
+            double m = -hg * row[j];
+            if (j == lane) m = m + 1.0;
+            row[j] = m;
+        }
+    }
+    __syncthreads();
+    bool ok = true;
+    double *const ub = sh.U12[wave];
+    int pend_c0 = -1;                // wave 0: first column of the panel whose U12 (in ub) is still to be
+                                     // written into its pivot rows (-1: none)
+    for (int c0 = 0; c0 < N; c0 += 16) {
+        const int pw = min(16, N - c0), c1 = c0 + pw;
+        if (wave == 0 && pend_c0 >= 0) {           // the previous panel's U12 into its pivot rows
+            const int pc1 = pend_c0 + 16;
+            if (lane < N - pc1) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) M[sh.perm[pend_c0 + k] * LDJ + pc1 + lane] = ub[k * 64 + lane];
+            }
+        }
+        pend_c0 = -1;
+        // ---- 1. the panel, in registers ----
+        double P[16];
+        const bool open0 = rank == N;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) P[j] = (open0 && j < pw) ? row[c0 + j] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < pw) {
+                const int k = c0 + j;
+                const double rk = P[j];
+                const int p = pivot_row(rk, rank == N, ok);
+                const double piv = bcast(rk, p);
+                if (wave == 0 && lane == 0) sh.perm[k] = p;
+                if (!(piv != 0.0)) ok = false;
+                const bool upd = rank == N && lane != p;
+                if (lane == p) rank = k;
+                const double l = rk / piv;
+                if (upd) P[j] = l;
+                // the pivot row's entries (v_readlane reads its lane whatever the branch), and the
+                // updates of the open rows
+#pragma unroll
+                for (int jj = j + 1; jj < 16; ++jj)
+                    if (jj < pw) {
+                        const double pv = bcast(P[jj], p);
+                        if (upd) P[jj] = fma(-l, pv, P[jj]);
+                    }
+            }
+        }
+        if (wave == 0) {
+            if (lane < N && rank >= c0) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (j < pw) row[c0 + j] = P[j];
+            }
+            sh.rank[lane] = rank;
+        }
+        __syncthreads();
+        if (c1 < N) {                              // then pw = 16
+            // ---- 2. U12: the panel's pivot rows right of the panel, column c1 + lane ----
+            const int ncol = N - c1;
+            double uu[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const double *prow = M + sh.perm[c0 + k] * LDJ;     // its multipliers: stored above
+                double v = lane < ncol ? prow[c1 + lane] : 0.0;
+#pragma unroll
+                for (int j = 0; j < k; ++j) v = fma(-prow[c0 + j], uu[j], v);
+                uu[k] = v;
+                ub[k * 64 + lane] = lane < ncol ? v : 0.0;
+            }
+            if (wave == 0) pend_c0 = c0;
+            // ---- 3. the open rows 16 wave .. 16 wave + 15 right of the panel ----
+            const int ra = 16 * wave + (lane & 15);        // row of this lane's A entries
+            const bool a_open = ra < N && sh.rank[ra] >= c1;
+            double A[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) A[q] = a_open ? -M[ra * LDJ + c0 + 4 * q + (lane >> 4)] : 0.0;
+            const int nct = (ncol + 15) / 16;
+            for (int ct = 0; ct < nct; ++ct) {
+                const int col = c1 + 16 * ct + (lane & 15);
+                fcfd_d4 acc;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rr = 16 * wave + (lane >> 4) + 4 * v;
+                    acc[v] = rr < N ? M[rr * LDJ + col] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[q], ub[(4 * q + (lane >> 4)) * 64 + 16 * ct + (lane & 15)],
+                                                              acc, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int rr = 16 * wave + (lane >> 4) + 4 * v;
+                    if (rr < N && sh.rank[rr] >= c1) M[rr * LDJ + col] = acc[v];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (wave == 0 && pend_c0 >= 0) {
+        const int pc1 = pend_c0 + 16;
+        if (lane < N - pc1) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) M[sh.perm[pend_c0 + k] * LDJ + pc1 + lane] = ub[k * 64 + lane];
+        }
+    }
+    __syncthreads();
+    if (wave == 0 && lane < N) sh.rdiag[lane] = 1.0 / M[sh.perm[lane] * LDJ + lane];
     __syncthreads();
     return ok;
 }
@@ -675,7 +823,9 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
                 }
                 have_j = NW > 1;
                 const double hg = h * CH_G;
-                bool ok = lu<NSP, NW>(sh, hg, wave);
+                bool ok;
+                if constexpr (NW == 4) ok = a.lu_blocked ? lu_blocked<NSP>(sh, hg, wave) : lu<NSP, NW>(sh, hg, wave);
+                else ok = lu<NSP, NW>(sh, hg, wave);
                 double k0 = 0.0, k1 = 0.0, k2 = 0.0, z = 0.0;
                 for (int st = 0; st < 3 && ok; ++st) {
                     double base = y;
@@ -1002,6 +1152,7 @@ hipError_t launch_chem(const ChemArgs &a0, int nsp, int n_workgroups, hipStream_
     if (e != hipSuccess) return e;
     ChemArgs a = a0;
     a.coop_max = chem_coop_max(n_workgroups, nsp);
+    a.lu_blocked = chem_lu_variant(nsp);
     if (a.order && a.list && a.list_n && a.stats && a.layout == 0) {
         hipLaunchKernelGGL(k_heavy_first, dim3(1), dim3(1024), 0, st, a.list, a.list_n, a.stats, a.order);
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -7,6 +7,32 @@ namespace fcfd {
 
 constexpr int LANE_NW = 4;       // waves per workgroup of the lane-per-cell kernels
 constexpr int LANE_CHUNK = 8;    // reaction rows per chunk of their scatter (two per wave)
+constexpr int LANE_NC = 56;      // cells per workgroup of the lane-per-cell kernels
+
+// A row of the branch-free range (nplain0 .. ii-1) as the lane-per-cell kernels read it, with one
+// row index for the whole wave: coefficients as doubles, and the species of each present slot as the
+// byte offset k * LANE_NC * 8 of its row in a species-major [k][LANE_NC] LDS block.  Reactant slots
+// are 0 .. nr-1, product slots 3 .. 3+np-1 (build_dev_kin admits only rows stored that way).
+struct LanePlain {
+    double A, B, E;
+    double nud[6];
+    unsigned short off[6];
+    int bits;                    // LP_* fields
+};
+constexpr int LP_NR = 0;         // bits 0-1: reactant slots (1-3)
+constexpr int LP_NP = 2;         // bits 2-3: product slots (0-3)
+constexpr int LP_REV = 1 << 4;   // reverse rate from the equilibrium constant
+constexpr int LP_ARR0 = 1 << 5;  // B = E = 0
+constexpr int LP_NUS = 6;        // bits 6-8: coefficient sum + 3
+constexpr int LP_POW = 1 << 9;   // some slot has |coefficient| > 1
+constexpr int LP_PW = 10;        // bits 10-21: |coefficient| of slot n in bits 10 + 2n .. 11 + 2n
+constexpr int LP_DEAD = 1 << 22; // coefficient of slot 0 is 0: no rate
+// A scatter entry of the lane-per-cell kernels: W[k] += Q[r] * nu with the byte offsets of row k of
+// the production-rate block and row r of the chunk's rate block.
+struct LaneEnt {
+    int offW, offQ;
+    double nu;
+};
 
 // Reaction data in the form CKRATT / CKRATX / CKWYR consume it (cklib.f:6630-6993, 9107-9204),
 // re-indexed 0-based with one row per reaction.
@@ -28,10 +54,10 @@ struct DevKin {
     // scatter lists of the lane-per-cell kernels (chem_lane.hip): rows are taken in chunks of
     // LANE_CHUNK; within chunk c, wave w adds the entries lane_ent[lane_ent0[c * LANE_NW + w] ..
     // lane_ent0[c * LANE_NW + w + 1]) to the production rates of the species it owns (k mod LANE_NW
-    // = w), in row then slot order.  Entry: species k (bits 0-7), row in the chunk (bits 8-15),
-    // stoichiometric coefficient (bits 16-31, signed).
+    // = w), in row then slot order.
     int lane_ent0[(MAXR + LANE_CHUNK - 1) / LANE_CHUNK * LANE_NW + 1];
-    int lane_ent[MAXR * 6];
+    LaneEnt lane_ent[MAXR * 6];
+    LanePlain lane_plain[MAXR];         // rows nplain0 .. ii-1 (other entries unused)
 };
 
 
@@ -82,6 +108,7 @@ struct ChemArgs {
     // optional (domain layout): f at the start state of each cell the explicit screen rejects,
     // [NSP + 1][ncells] (species, then T); k_chem_retry starts from it instead of evaluating it again
     double *f0;
+    int lu_blocked;              // set by launch_chem: 1 = the four-wave mode factors with lu_blocked
 };
 
 // SDIRK integrator over all cells or over a work list (persistent grid): one cell per wave for long
@@ -98,6 +125,9 @@ hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st);
 // Longest list that the four-wave integrator mode takes: the tuned factor (default 8) times the
 // workgroup count (api.hip).
 int chem_coop_max(int n_workgroups, int nsp);
+// 1 when the four-wave integrator factors its Newton matrix with the blocked, matrix-core LU (the
+// tuned "chem_lu" entry, api.hip); the factors are the same either way.
+int chem_lu_variant(int nsp);
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
 // *out = f * *in on the device (one thread), e.g. the half step of Strang splitting.
 hipError_t launch_scale_scalar(const double *in, double *out, double f, hipStream_t st);

@@ -10,14 +10,14 @@
 //
 // Mapping: a workgroup of four waves owns NC = 56 cells (lane l < NC of every wave is cell l; lanes
 // NC..63 repeat cell NC - 1 and store nothing).  Concentrations and S/R - H/RT of the cells live in
-// LDS as species-major [k][NC] blocks.  The reactions are taken in chunks of 16 rows: each wave
-// evaluates four rows of a chunk for all cells and puts their rates of progress in LDS; after a
+// LDS as species-major [k][NC] blocks.  The reactions are taken in chunks of 8 rows: each wave
+// evaluates two rows of a chunk for all cells and puts their rates of progress in LDS; after a
 // barrier every wave adds the chunk's contributions to the species it owns (k mod 4 = wave) from a
 // precomputed list, in row order, into one shared production-rate block whose rows no two waves
 // write.  Each species' production rate is therefore one sequential sum over the rows, independent
 // of the wave count.  Species-wise work (thermo, state update, norms) is split over the waves by
 // k = wave mod 4; cell-wise scalars (T, sums) are combined through LDS in wave order.  The LDS
-// footprint (80 KB) lets two workgroups share a compute unit: two waves per SIMD.  Reaction tables
+// footprint (80,000 B) lets two workgroups share a compute unit: two waves per SIMD.  Reaction tables
 // are read with a wave-uniform row index (scalar loads).
 #include <algorithm>
 
@@ -30,7 +30,7 @@ namespace fcfd {
 namespace {
 
 constexpr int NW = LANE_NW;    // waves per workgroup
-constexpr int NC = 56;         // cells per workgroup
+constexpr int NC = LANE_NC;    // cells per workgroup
 constexpr int CH = LANE_CHUNK; // reaction rows per chunk, CH / NW per wave
 constexpr int RB = CH / NW;
 
@@ -41,7 +41,64 @@ struct LaneShared {
     double W[NSP * NC];        // production rates; row k is written by wave k mod NW only
     double Q[CH * NC];         // rates of progress (forward minus reverse) of the current chunk
     double S[NW * NC];         // per-wave values of a cross-wave sum
+    double PF[7 * NC];         // pfac**m for m = -3 .. 3 (row m + 3), the equilibrium constants' factor
 };
+
+// reaction_plain (chem_device.hpp) for a row the whole wave evaluates (wave-uniform row index), as
+// forward minus reverse rate of progress: the same operations, with the slot counts, coefficient
+// powers and flags of K->lane_plain taken as scalar branches instead of selects, the species rows
+// read at byte offsets from this lane's column of the C and G blocks, and pfac**nus from the lane's
+// PF column.  Multiplications by exactly 1.0 that reaction_plain makes for empty slots are skipped
+// (they do not change a value).
+__device__ __forceinline__ double ipow_u(double x, int m) {      // ipow3 for a wave-uniform m >= 1
+    if (m == 1) return x;
+    const double x2 = x * x;
+    return m == 2 ? x2 : x * x2;
+}
+
+__device__ __forceinline__ double lane_plain_q(const LanePlain *__restrict__ L, double T, double alogt,
+                                               double small, double exparg, const char *Cb, const char *Gb,
+                                               const double *Pf) {
+    const int bits = L->bits;
+    if (bits & LP_DEAD) return 0.0;
+    const int nr = (bits >> LP_NR) & 3, np = (bits >> LP_NP) & 3;
+    auto C = [&](int n) { return *reinterpret_cast<const double *>(Cb + L->off[n]); };
+    auto G = [&](int n) { return *reinterpret_cast<const double *>(Gb + L->off[n]); };
+    double kf = L->A;
+    if (!(bits & LP_ARR0)) kf = kf * fm_exp(L->B * alogt - L->E / T);
+    double kr = 0.0;
+    if (bits & LP_REV) {
+        // reaction_plain starts from 0.0 + t0; the two differ at most in the sign of a zero sum,
+        // which the test s != 0 and exp(+-0) = 1 do not see
+        double s = L->nud[0] * G(0);
+        if (nr >= 2) s = s + L->nud[1] * G(1);
+        if (nr >= 3) s = s + L->nud[2] * G(2);
+        if (np >= 1) s = s + L->nud[3] * G(3);
+        if (np >= 2) s = s + L->nud[4] * G(4);
+        if (np >= 3) s = s + L->nud[5] * G(5);
+        double eqk = s != 0.0 ? fm_exp(fmin(s, exparg)) : 1.0;
+        eqk = eqk * Pf[((bits >> LP_NUS) & 7) * NC];
+        kr = kf / fmax(eqk, small);
+    }
+    double qf, qr = kr;
+    if (!(bits & LP_POW)) {
+        qf = kf * C(0);
+        if (nr >= 2) qf = qf * C(1);
+        if (nr >= 3) qf = qf * C(2);
+        if (np >= 1) qr = kr * C(3);
+        if (np >= 2) qr = qr * C(4);
+        if (np >= 3) qr = qr * C(5);
+    } else {
+        const int pw = bits >> LP_PW;
+        qf = kf * ipow_u(C(0), pw & 3);
+        if (nr >= 2) qf = qf * ipow_u(C(1), (pw >> 2) & 3);
+        if (nr >= 3) qf = qf * ipow_u(C(2), (pw >> 4) & 3);
+        if (np >= 1) qr = kr * ipow_u(C(3), (pw >> 6) & 3);
+        if (np >= 2) qr = qr * ipow_u(C(4), (pw >> 8) & 3);
+        if (np >= 3) qr = qr * ipow_u(C(5), (pw >> 10) & 3);
+    }
+    return qf - qr;
+}
 
 // Species slots of wave w: k = w + NW * j.
 template <int NSP>
@@ -90,12 +147,25 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
             sh.G[k * NC + cl] = s + a.a6 - a.a5 / T;
         }
     }
-    const double ctot = sum4(sh.S, w, cl, ctot_w);    // its barrier also publishes C and G
-    const double cvm = sum4(sh.S, w, cl, cvm_w);
     const double pfac = th->pa / (th->ru * T);
+    {
+        // pfac**m as reaction_plain forms it (x**3 = x * (x * x), x**-m = 1 / x**m); row m + 3 of PF,
+        // rows w and w + 4 written by wave w
+        const double p2 = pfac * pfac, p3 = pfac * p2;
+        const double pm[7] = {1.0 / p3, 1.0 / p2, 1.0 / pfac, 1.0, pfac, p2, p3};
+#pragma unroll
+        for (int r = 0; r < 7; ++r)
+            if (r % NW == w) sh.PF[r * NC + cl] = pm[r];
+    }
+    const double ctot = sum4(sh.S, w, cl, ctot_w);    // its barrier also publishes C, G and PF
+    const double cvm = sum4(sh.S, w, cl, cvm_w);
     const double *Cp = sh.C + cl, *Gp = sh.G + cl;
+    const char *Cb = reinterpret_cast<const char *>(Cp), *Gb = reinterpret_cast<const char *>(Gp);
+    const double *Pf = sh.PF + cl;
     double *Wp = sh.W + cl;
+    char *Wb = reinterpret_cast<char *>(Wp);
     const double *Qp = sh.Q + cl;
+    const char *Qb = reinterpret_cast<const char *>(Qp);
     for (int c0 = 0, chunk = 0; c0 < K->ii; c0 += CH, ++chunk) {
         // this wave's RB rows of the chunk: independent rate chains, so their table loads, LDS
         // reads and transcendental evaluations overlap
@@ -105,8 +175,7 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
 #pragma unroll
             for (int u = 0; u < RB; ++u) {
                 const int i = i0 + NW * u;
-                const Rop r = reaction_plain<NC>(K, i < K->ii ? i : K->ii - 1, T, lnT, pfac, Cp, Gp);
-                q[u] = i < K->ii ? r.qf - r.qr : 0.0;
+                q[u] = i < K->ii ? lane_plain_q(&K->lane_plain[i], T, lnT, K->small, K->exparg, Cb, Gb, Pf) : 0.0;
             }
         } else {
 #pragma unroll
@@ -125,9 +194,9 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
         // the chunk's contributions to this wave's species, in row then slot order
         const int e1 = K->lane_ent0[chunk * NW + w + 1];
         for (int e = K->lane_ent0[chunk * NW + w]; e < e1; ++e) {
-            const int pk = K->lane_ent[e];
-            const int k = pk & 0xff, r = (pk >> 8) & 0xff;
-            Wp[k * NC] = Wp[k * NC] + Qp[r * NC] * (double)(pk >> 16);
+            const LaneEnt &en = K->lane_ent[e];
+            double *wk = reinterpret_cast<double *>(Wb + en.offW);
+            *wk = *wk + *reinterpret_cast<const double *>(Qb + en.offQ) * en.nu;
         }
         __syncthreads();                   // Q is rewritten by the next chunk
     }

@@ -77,14 +77,15 @@ FM_FN double fm_exp_series(double rh) {
     return p;
 }
 
-/* exp(xh + xl), |xl| small against ulp(xh) * 2^10 */
+/* exp(xh + xl), |xl| small against ulp(xh) * 2^10.  Branch-free: the reduction and series run on
+ * the argument clamped to [-745.2, 709.79] (unchanged inside that range) and the results for
+ * arguments outside it (0, +inf, NaN) are selected at the end, so that on the GPU the lanes of a
+ * wave never split over the special cases. */
 FM_FN double fm_exp_dd(double xh, double xl) {
-    if (xh != xh) return xh + xh;
-    if (xh > 709.79) return __builtin_inf();
-    if (xh < -745.2) return 0.0;
-    const double kd = (xh * FM_INV_LN2 + FM_SHIFT) - FM_SHIFT;
-    /* |xh| >= 1/2 or kd = 0: xh - kd*LN2_HI is exactly representable, so the fma is exact */
-    const double r0 = __builtin_fma(-kd, FM_LN2_HI, xh);
+    const double xc = __builtin_fmin(__builtin_fmax(xh, -745.2), 709.79);
+    const double kd = (xc * FM_INV_LN2 + FM_SHIFT) - FM_SHIFT;
+    /* |xc| >= 1/2 or kd = 0: xc - kd*LN2_HI is exactly representable, so the fma is exact */
+    const double r0 = __builtin_fma(-kd, FM_LN2_HI, xc);
     /* r = rh + rl = r0 + xl - kd*LN2_LO with rh the rounded sum and rl its rounding error */
     const double r1 = xl - kd * FM_LN2_LO;
     const double rh = r0 + r1;
@@ -97,9 +98,12 @@ FM_FN double fm_exp_dd(double xh, double xl) {
     const double s = 1.0 + rh;
     const double e = rh - (s - 1.0);
     const double v = s + (e + (rl + (rh * rl + q)));
-    const int k = (int)kd;
+    const int k = (int)kd;                /* -1075 <= k <= 1024: both factors below are normal */
     const int k1 = k / 2;
-    return (v * fm_pow2i(k1)) * fm_pow2i(k - k1);
+    double res = (v * fm_pow2i(k1)) * fm_pow2i(k - k1);
+    res = xh > 709.79 ? __builtin_inf() : res;
+    res = xh < -745.2 ? 0.0 : res;
+    return xh != xh ? xh + xh : res;
 }
 
 FM_FN double fm_exp(double x) { return fm_exp_dd(x, 0.0); }

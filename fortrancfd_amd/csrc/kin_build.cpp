@@ -10,9 +10,10 @@ namespace fcfd {
 
 // Device rows are the mechanism's reactions re-ordered by kind -- fall-off first, then plain
 // third-body, then the remaining reactions whose coefficients exceed 3 in magnitude (or whose
-// product coefficients are negative), then the rest -- so a wave, which evaluates 64 consecutive
-// rows at once, runs the fall-off and third-body code paths in as few row chunks as possible, and
-// the last group (from row nplain0) can take the branch-free evaluation.  Every per-reaction result
+// product coefficients are negative), then the rest, by falling count of exponentials -- so a wave,
+// which evaluates 64 consecutive rows at once, runs the fall-off and third-body code paths in as
+// few row chunks as possible, and the last group (from row nplain0) can take the branch-free
+// evaluation.  Every per-reaction result
 // is independent of the row order; only the order of the species-sum additions changes.
 void build_dev_kin(const Mech &m, DevKin &k) {
     std::memset(&k, 0, sizeof(DevKin));
@@ -25,6 +26,7 @@ void build_dev_kin(const Mech &m, DevKin &k) {
     std::vector<int> tb_of(m.ii, -1), fo_of(m.ii, -1), order;
     for (int t = 0; t < k.nthb; ++t) tb_of[m.itb[t] - 1] = t;
     for (int f = 0; f < k.nfal; ++f) fo_of[m.ifl[f] - 1] = f;
+    // the branch-free rows also store their reactants in slots 0 .. nr-1 and products in 3 .. 3+np-1
     auto small = [&](int i) {
         int sum = 0;
         for (int n = 0; n < 6; ++n) {
@@ -32,16 +34,27 @@ void build_dev_kin(const Mech &m, DevKin &k) {
             if (v > 3 || v < -3 || (n >= 3 && v < 0)) return false;
             sum += v;
         }
-        return sum >= -3 && sum <= 3;
+        for (int h = 0; h < 2; ++h)
+            for (int n = 1; n < 3; ++n)
+                if (m.nk[size_t(i) * 6 + 3 * h + n] > 0 && m.nk[size_t(i) * 6 + 3 * h + n - 1] <= 0) return false;
+        return sum >= -3 && sum <= 3 && m.nk[size_t(i) * 6] > 0;
     };
-    for (int pass = 0; pass < 4; ++pass) {
+    // exponentials a branch-free row evaluates: the Arrhenius factor unless B = E = 0, the
+    // equilibrium constant if reversible
+    const int np1 = m.npar + 1;
+    auto n_exp = [&](int i) {
+        const bool arr0 = m.par[size_t(i) * np1 + 1] == 0.0 && m.par[size_t(i) * np1 + 2] == 0.0;
+        return (arr0 ? 0 : 1) + (m.ns[i] > 0 ? 1 : 0);
+    };
+    for (int pass = 0; pass < 6; ++pass) {
         if (pass == 3) k.nplain0 = int(order.size());
         for (int i = 0; i < m.ii; ++i) {
             const int kind = fo_of[i] >= 0 ? 0 : (tb_of[i] >= 0 ? 1 : (small(i) ? 3 : 2));
-            if (kind == pass) order.push_back(i);
+            // the branch-free rows by falling cost, so that the waves that share a chunk of rows
+            // (and its barriers) get rows of equal cost
+            if (kind == 3 ? pass == 5 - n_exp(i) : kind == pass) order.push_back(i);
         }
     }
-    const int np1 = m.npar + 1;
     for (int r = 0; r < m.ii; ++r) {
         const int i = order[r];
         k.orig[r] = i;
@@ -74,11 +87,29 @@ void build_dev_kin(const Mech &m, DevKin &k) {
                 for (int n = 0; n < 6; ++n) {
                     const int sp = k.nk[row][n];
                     if (sp >= 0 && sp % LANE_NW == w)
-                        k.lane_ent[ne++] = int(uint32_t(sp) | (uint32_t(r) << 8) | (uint32_t(uint16_t(int16_t(k.nu[row][n]))) << 16));
+                        k.lane_ent[ne++] = LaneEnt{sp * LANE_NC * 8, r * LANE_NC * 8, double(k.nu[row][n])};
                 }
             }
         }
     k.lane_ent0[nchunk * LANE_NW] = ne;
+    for (int r = k.nplain0; r < m.ii; ++r) {
+        LanePlain &p = k.lane_plain[r];
+        p.A = k.A[r];
+        p.B = k.B[r];
+        p.E = k.E[r];
+        int nr = 0, np = 0, pw = 0, big = 0;
+        for (int n = 0; n < 6; ++n) {
+            const int sp = k.nk[r][n];
+            p.nud[n] = double(k.nu[r][n]);
+            p.off[n] = (unsigned short)((sp >= 0 ? sp : 0) * LANE_NC * 8);
+            if (sp >= 0) (n < 3 ? nr : np) += 1;
+            const int a = k.nu[r][n] < 0 ? -k.nu[r][n] : k.nu[r][n];
+            pw |= (sp >= 0 ? a : 0) << (2 * n);
+            if (sp >= 0 && a > 1) big = 1;
+        }
+        p.bits = (nr << LP_NR) | (np << LP_NP) | (k.rev[r] ? LP_REV : 0) | (k.arr0[r] ? LP_ARR0 : 0) |
+                 ((k.nus[r] + 3) << LP_NUS) | (big ? LP_POW : 0) | (pw << LP_PW) | (k.nu[r][0] == 0 ? LP_DEAD : 0);
+    }
     for (int t = 0; t < k.nthb; ++t) {
         k.ntbs[t] = m.ntbs[t];
         for (int l = 0; l < m.mxtb && l < MAXTB; ++l) {

@@ -12,7 +12,9 @@ Candidates and the workloads they are timed on (GRI-3.0):
   * chem_four_wave_factor (the list length below which the integrator gives each cell four waves)
     on a short stiff list -- the coupled 65536-cell domain after --steps Strang steps, one
     solve_source of the saved state per candidate -- plus a 4096-cell batch, a list length where the
-    candidates choose different modes.
+    candidates choose different modes;
+  * chem_lu (the four-wave mode's LU: step by step, or in 16-column panels with the FP64 matrix
+    cores; the factors are bitwise the same) on the same short list.
 A candidate replaces the default only when it is more than 2 % faster.
 Each candidate is timed three times and scored by its median; the table records the winners and,
 as comments, every candidate's time.
@@ -137,7 +139,23 @@ def tune_chem_short(mech, log, steps, wg_per_cu, ncu):
     res_all = {v: res[v] + mid[8 if nmid <= v * wg_per_cu * ncu else 0] for v in res}
     best = pick(res_all, 8)
     set_launch("chem_four_wave_factor", mech.kk, best)
-    return best, res_all, nimp
+    # the four-wave mode's factorisation (step by step or blocked on the matrix cores: the same
+    # factors) on the same list
+    lu = {}
+    for v in (0, 1):
+        set_launch("chem_lu", mech.kk, v)
+        reps = []
+        for _ in range(3):
+            s.upload(Qc, qf)
+            ms0, _ = s.timing(3)
+            s.solve_source(dt * 0.5, t, diagnostics=False)
+            ms1, _ = s.timing(3)
+            reps.append(ms1 - ms0)
+        lu[v] = median(reps)
+        log(f"chem_lu={v}: {lu[v]:.3f} ms (implicit integrator, {nimp} cells, four-wave factor {best})")
+    best_lu = pick(lu, 0)
+    set_launch("chem_lu", mech.kk, best_lu)
+    return best, res_all, nimp, best_lu
 
 
 def main(argv=None):
@@ -156,16 +174,18 @@ def main(argv=None):
         lines.append("# " + msg)
 
     # start from the defaults, whatever table is installed
-    for key, v in (("flux_variant", 0), ("stage_variant", 0), ("chem_wg_per_cu", 4), ("chem_four_wave_factor", 8)):
+    for key, v in (("flux_variant", 0), ("stage_variant", 0), ("chem_wg_per_cu", 4), ("chem_four_wave_factor", 8),
+                   ("chem_lu", 0)):
         set_launch(key, mech.kk, v)
     fl = tune_fluid(mech, log)
     wg, _ = tune_chem_long(mech, log)
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
-    fw, _, _ = tune_chem_short(mech, log, args.steps, wg, ncu)
+    fw, _, _, lu = tune_chem_short(mech, log, args.steps, wg, ncu)
     table = [f"{arch} {mech.kk} flux_variant {fl['flux_variant'][0]}",
              f"{arch} {mech.kk} stage_variant {fl['stage_variant'][0]}",
              f"{arch} {mech.kk} chem_wg_per_cu {wg}",
-             f"{arch} {mech.kk} chem_four_wave_factor {fw}"]
+             f"{arch} {mech.kk} chem_four_wave_factor {fw}",
+             f"{arch} {mech.kk} chem_lu {lu}"]
     with open(args.out, "w") as f:
         f.write("# Launch shapes chosen by python -m fortrancfd_amd.tune (median of three timings each).\n")
         f.write("# Lines: <arch> <species count> <key> <value>; read by libfcfd_hip.so at start-up.\n")

@@ -227,7 +227,9 @@ int fcfd_set_timing(fcfd_ctx *c, int enable);
  * 1 = 64 / one, 2 = 128 / one), "stage_variant" (stage update: 0 = 64 / two, 1 = 64 / one,
  * 2 = 128 / one, 3 = 256 / one), "chem_wg_per_cu" (implicit-integrator workgroups per CU, default 4),
  * "chem_four_wave_factor" (four-wave integrator mode for lists of at most factor x workgroups cells,
- * default 8).  fcfd_set_launch overrides one entry for the mechanisms of nsp species (every context
+ * default 8), "chem_lu" (0 = the four-wave integrator factors its Newton matrix one elimination step
+ * at a time, 1 = in 16-column panels with the FP64 matrix cores updating the rest; the factors are
+ * bitwise the same, default 0).  fcfd_set_launch overrides one entry for the mechanisms of nsp species (every context
  * re-captures its step graph); fcfd_load_tuning reloads the table (NULL: the default file). */
 int fcfd_set_launch(const char *key, int nsp, int value);
 int fcfd_get_launch(const char *key, int nsp, int *value);

@@ -1,7 +1,7 @@
 """Every launch shape the tuner may pick gives the same bits: the reference's WENO5-JS + HLLC golden
 run (tests/golden/run_sod_weno5js_hllc.npz) through each advection-flux and stage-update variant
 (fcfd_set_launch); each chemistry integrator mode on the coupled hot-spot domain against the
-oracle."""
+oracle; the blocked LU against the step-by-step one, bitwise."""
 import ctypes as C
 
 import numpy as np
@@ -49,16 +49,54 @@ def test_fluid_variants_bit_exact(mech, key, value):
     assert t > 0
 
 
-@pytest.mark.parametrize("wg,fw", [(4, 8), (4, 0), (2, 1000), (5, 0)])
-def test_integrator_modes_vs_oracle(mech, orc, wg, fw):
+@pytest.mark.parametrize("wg,fw,lu", [(4, 8, 0), (4, 0, 0), (2, 1000, 0), (5, 0, 0), (4, 1000, 1)])
+def test_integrator_modes_vs_oracle(mech, orc, wg, fw, lu):
     """Each integrator mode (four waves per cell for short lists, one wave per cell, other grid
     sizes) against the oracle on the coupled hot-spot domain: the modes sum the Jacobian's per-wave
     partials in different orders, so each meets the chemistry parity bar on its own."""
     from tests.test_gpu_configs import ch4_air_cells, compare_fields, run_pair
     set_launch("chem_wg_per_cu", wg)
     set_launch("chem_four_wave_factor", fw)
+    set_launch("chem_lu", lu)
     cells = ch4_air_cells(mech.kk, 64, [(0, 12, 2100.0), (12, 4, 1600.0)])
-    print(f"integrator mode wg/CU={wg} four-wave factor={fw}", flush=True)
+    print(f"integrator mode wg/CU={wg} four-wave factor={fw} blocked LU={lu}", flush=True)
     s, d, dts_g, dts_o, h = run_pair(mech, orc, cells, 2, 32, 25e-6, 3, False)
     np.testing.assert_allclose(dts_g, dts_o, rtol=1e-13)
     compare_fields(s, d, dts_o, dts_o, orc)
+
+
+def test_blocked_lu_bitwise(mech):
+    """The four-wave integrator's blocked LU (16-column panels, FP64 matrix-core updates) gives the
+    factors of the step-by-step LU bit for bit, so whole reacting runs agree exactly: the coupled
+    hot-spot domain for three steps and a batch of stiff cells, each with both factorizations."""
+    import torch
+    from fortrancfd_amd.lib import Solver
+    from fortrancfd_amd.workloads import SEED, chemistry_batch
+    from tests.test_gpu_configs import ch4_air_cells
+    cells = ch4_air_cells(mech.kk, 64, [(0, 12, 2100.0), (12, 4, 1600.0)])
+    set_launch("chem_four_wave_factor", 1000)           # every list on four waves per cell
+    out = []
+    for lu in (0, 1):
+        set_launch("chem_lu", lu)
+        print(f"coupled hot-spot domain, chem_lu={lu}", flush=True)
+        s = Solver(mech, 2, 32, 25e-6, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1,
+                   do_reaction=1)
+        s.init_cells(cells)
+        s.advance(0.5, 3)
+        out.append(s.download())
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    rho, dt, T, Y = chemistry_batch(mech, 256, seed=SEED + 29)
+    res = []
+    for lu in (0, 1):
+        set_launch("chem_lu", lu)
+        print(f"256 batch cells, chem_lu={lu}", flush=True)
+        d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+        tr, tdt, Tf, Yf = d(rho), d(dt), d(T), d(Y.T)
+        stats = torch.zeros((256, 4), dtype=torch.int32, device="cuda")
+        mech.chem_batch_device(tr, tdt, Tf, Yf, stats=stats)
+        torch.cuda.synchronize()
+        res.append((Tf.cpu().numpy(), Yf.cpu().numpy(), stats.cpu().numpy()))
+    assert res[0][2][:, 0].sum() > 0 and (res[0][2][:, 1] + res[0][2][:, 2]).sum() > 0
+    for a, b in zip(res[0], res[1]):
+        np.testing.assert_array_equal(a, b)
