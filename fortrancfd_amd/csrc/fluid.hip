@@ -17,7 +17,9 @@ struct Side {
     double rho, u, T, sy, p, a, E;
 };
 
-// y[k * stride] holds the reconstructed Y_k (a register array with stride 1 or an LDS column).
+// y[k * stride] holds the reconstructed Y_k (a register array with stride 1 or an LDS column).  Over
+// an LDS column the species loops unroll by four only: fully unrolled, the advection-flux kernel
+// needs more than 256 registers and spills to scratch.
 template <int NSP, int STRIDE = 1>
 __device__ __forceinline__ Side side_state(const DevThermo *__restrict__ m, double rho, double u,
                                            double T, const double *y) {
@@ -25,7 +27,7 @@ __device__ __forceinline__ Side side_state(const DevThermo *__restrict__ m, doub
     s.rho = rho; s.u = u; s.T = T;
     // adjust_sum(Y) denominator (Y itself is renormalised lazily where QcL/R needs it).
     double sy = 0.0, syw = 0.0;
-#pragma unroll
+#pragma unroll (STRIDE == 1 ? NSP : 4)
     for (int k = 0; k < NSP; ++k) {
         sy = sy + fmax(y[k * STRIDE], 0.0);
         syw = syw + y[k * STRIDE] / m->wt[k];          // CKYTX on the raw reconstructed Y
@@ -35,12 +37,12 @@ __device__ __forceinline__ Side side_state(const DevThermo *__restrict__ m, doub
     // mixture sums instead of being held in an array: each sum keeps its species order, and the
     // per-species quotient is the same expression every time it is formed
     double sx = 0.0;
-#pragma unroll
+#pragma unroll (STRIDE == 1 ? NSP : 4)
     for (int k = 0; k < NSP; ++k) sx = sx + fmax(y[k * STRIDE] / (syw * m->wt[k]), 0.0);
     const TPow tp = tpow(T, m->ru);
     double wtm = 0.0;
     MixSums ms{0.0, 0.0, 0.0};
-#pragma unroll
+#pragma unroll (STRIDE == 1 ? NSP : 4)
     for (int k = 0; k < NSP; ++k) {
         const double xk = fmax(y[k * STRIDE] / (syw * m->wt[k]), 0.0) / sx;
         wtm = wtm + xk * m->wt[k];                                   // mix_w
@@ -128,7 +130,7 @@ __global__ void __launch_bounds__(BS, WPE) k_flux(FluxArgs a) {
         put(F[0], 0, 0.5 * (fl0 + fr0 - al * (R.rho - L.rho)));
         put(F[PF], 1, 0.5 * (fl1 + fr1 - al * (R.rho * R.u - L.rho * L.u)));
         put(F[2 * PF], 2, 0.5 * (fl2 + fr2 - al * (R.E - L.E)));
-#pragma unroll
+#pragma unroll 4
         for (int k = 0; k < NSP; ++k) {
             const double ym = y[k * BS], yo = yo_col[k * BS];
             const double yl = s ? yo : ym, yr = s ? ym : yo;
@@ -149,7 +151,7 @@ __global__ void __launch_bounds__(BS, WPE) k_flux(FluxArgs a) {
             put(F[0], 0, hll(fl0, fr0, L.rho, R.rho));
             put(F[PF], 1, hll(fl1, fr1, L.rho * L.u, R.rho * R.u));
             put(F[2 * PF], 2, hll(fl2, fr2, L.E, R.E));
-#pragma unroll
+#pragma unroll 4
             for (int k = 0; k < NSP; ++k) {
                 const double ym = y[k * BS], yo = yo_col[k * BS];
                 const double yl = s ? yo : ym, yr = s ? ym : yo;
@@ -168,7 +170,7 @@ __global__ void __launch_bounds__(BS, WPE) k_flux(FluxArgs a) {
                 put(F[0], 0, f0);
                 put(F[PF], 1, f1);
                 put(F[2 * PF], 2, f2);
-#pragma unroll
+#pragma unroll 4
                 for (int k = 0; k < NSP; ++k) {
                     const double yk = (left == (s == 0)) ? y[k * BS] : yo_col[k * BS];
                     put(F[(3 + k) * PF], 3 + k, S.rho * yk * S.u);
@@ -180,7 +182,7 @@ __global__ void __launch_bounds__(BS, WPE) k_flux(FluxArgs a) {
                 put(F[0], 0, f0 + sK * (c * 1.0 - S.rho));
                 put(F[PF], 1, f1 + sK * (c * sM - S.rho * S.u));
                 put(F[2 * PF], 2, f2 + sK * (c * v2 - S.E));
-#pragma unroll
+#pragma unroll 4
                 for (int k = 0; k < NSP; ++k) {
                     const double yk = (left == (s == 0)) ? y[k * BS] : yo_col[k * BS];
                     const double Qk = S.rho * (fmax(yk, 0.0) / S.sy);
