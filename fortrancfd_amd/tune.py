@@ -10,11 +10,9 @@ Candidates and the workloads they are timed on (GRI-3.0):
   * chem_wg_per_cu (implicit-integrator workgroups per CU) on a long stiff list: 2^16 independent
     cells of the chemistry batch generator (fortrancfd_amd.workloads.chemistry_batch);
   * chem_four_wave_factor (the list length below which the integrator gives each cell four waves)
-    on a short stiff list -- the coupled 65536-cell domain after --steps Strang steps, one
-    solve_source of the saved state per candidate -- plus a 4096-cell batch, a list length where the
-    candidates choose different modes;
-  * chem_lu (the four-wave mode's LU: step by step, or in 16-column panels with the FP64 matrix
-    cores; the factors are bitwise the same) on the same short list.
+    and chem_lu (the four-wave mode's LU: step by step, or in 16-column panels with the FP64 matrix
+    cores; the factors are bitwise the same) on whole steps of the coupled 65536-cell domain: the
+    run continued for 200 steps from its state after --steps Strang steps, per candidate.
 A candidate replaces the default only when it is more than 2 % faster.
 Each candidate is timed three times and scored by its median; the table records the winners and,
 as comments, every candidate's time.
@@ -105,63 +103,53 @@ def tune_chem_long(mech, log, n=1 << 16):
     return best, res
 
 
-def tune_chem_short(mech, log, steps, wg_per_cu, ncu):
+def tune_chem_short(mech, log, steps, window=200):
+    """chem_four_wave_factor and chem_lu on whole coupled steps: the coupled 65536-cell domain
+    (Strang, HRR every step) run to `steps`, then `window` steps from that saved state per candidate
+    (ms per step, median of three).  The window takes in the stiff lists of every length the run
+    produces -- the short lists of the hot spots and the periodic bursts of a thousand or more cold
+    cells -- so the factor is chosen for the mixture the production step sees."""
     from .workloads import coupled_cells
     cells = coupled_cells(mech)
     s = Solver(mech, 16, 4096, 25e-6, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1,
                do_reaction=1)
-    s.set_step(strang=True, hrr_each_step=False)
+    s.set_step(strang=True, hrr_each_step=True)
     s.init_cells(cells)
     t = s.advance(0.5, steps)
     Qc, qf = s.download()
-    dt = s.calc_dt(0.5)
-    s.solve_source(dt * 0.5, t, diagnostics=False)
-    acc, nimp = s.source_accepted()
-    log(f"coupled domain after {steps} steps: {nimp} cells on the implicit integrator")
-    s.set_timing(True)
+
+    def per_step():
+        reps = []
+        for _ in range(3):
+            s.upload(Qc, qf)                 # the same start, step-size memory cleared
+            t0 = time.perf_counter()
+            s.advance(0.5, window, t)
+            reps.append((time.perf_counter() - t0) * 1e3 / window)
+        return median(reps)
+
     res = {}
     for v in (0, 1, 2, 4, 8):
         set_launch("chem_four_wave_factor", mech.kk, v)
-        reps = []
-        for _ in range(3):
-            s.upload(Qc, qf)
-            ms0, _ = s.timing(3)
-            s.solve_source(dt * 0.5, t, diagnostics=False)
-            ms1, _ = s.timing(3)
-            reps.append(ms1 - ms0)
-        res[v] = median(reps)
-        log(f"chem_four_wave_factor={v}: {res[v]:.3f} ms (implicit integrator, {nimp} cells)")
-    # a list between the two modes' ranges: 4096 batch cells, almost all on the integrator
-    from .workloads import SEED
-    nmid = 4096
-    mid = time_batch(mech, nmid, SEED + 13, "chem_four_wave_factor", (0, 8), log, "batch cells")
-    # factor v puts that list on four waves per cell when nmid <= v x workgroups
-    res_all = {v: res[v] + mid[8 if nmid <= v * wg_per_cu * ncu else 0] for v in res}
-    best = pick(res_all, 8)
+        res[v] = per_step()
+        log(f"chem_four_wave_factor={v}: {res[v]:.3f} ms per coupled step (steps {steps}-{steps + window})")
+    best = pick(res, 8)
     set_launch("chem_four_wave_factor", mech.kk, best)
-    # the four-wave mode's factorisation (step by step or blocked on the matrix cores: the same
-    # factors) on the same list
+    # the four-wave mode's factorisation (step by step, or blocked on the matrix cores: the same
+    # factors)
     lu = {}
     for v in (0, 1):
         set_launch("chem_lu", mech.kk, v)
-        reps = []
-        for _ in range(3):
-            s.upload(Qc, qf)
-            ms0, _ = s.timing(3)
-            s.solve_source(dt * 0.5, t, diagnostics=False)
-            ms1, _ = s.timing(3)
-            reps.append(ms1 - ms0)
-        lu[v] = median(reps)
-        log(f"chem_lu={v}: {lu[v]:.3f} ms (implicit integrator, {nimp} cells, four-wave factor {best})")
+        lu[v] = per_step()
+        log(f"chem_lu={v}: {lu[v]:.3f} ms per coupled step (four-wave factor {best})")
     best_lu = pick(lu, 0)
     set_launch("chem_lu", mech.kk, best_lu)
-    return best, res_all, nimp, best_lu
+    return best, res, best_lu
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--out", default=DEFAULT_OUT)
-    ap.add_argument("--steps", type=int, default=2000, help="coupled steps before the short-list timing")
+    ap.add_argument("--steps", type=int, default=2000, help="coupled steps before the timed window")
     args = ap.parse_args(argv)
     import torch
     assert torch.cuda.is_available(), "the tuner needs a GPU"
@@ -179,8 +167,7 @@ def main(argv=None):
         set_launch(key, mech.kk, v)
     fl = tune_fluid(mech, log)
     wg, _ = tune_chem_long(mech, log)
-    ncu = torch.cuda.get_device_properties(0).multi_processor_count
-    fw, _, _, lu = tune_chem_short(mech, log, args.steps, wg, ncu)
+    fw, _, lu = tune_chem_short(mech, log, args.steps)
     table = [f"{arch} {mech.kk} flux_variant {fl['flux_variant'][0]}",
              f"{arch} {mech.kk} stage_variant {fl['stage_variant'][0]}",
              f"{arch} {mech.kk} chem_wg_per_cu {wg}",
