@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
     const int lane = min(int(threadIdx.x & 63), NC - 1);   // the LDS column (lanes >= NC repeat NC - 1)
     const DevThermo *__restrict__ th = a.thermo;
     const DevKin *__restrict__ K = a.kin;
-    const int cell0 = blockIdx.x * NC;
+    const int cell0 = int(xcd_block()) * NC;   // neighbouring cell ranges share their XCD's L2
     const int cell = cell0 + lane;
     // every lane of a column takes the column's decisions (lanes >= NC share column NC - 1 and must
     // write the same LDS values as its owner); only the owner lane stores to global memory

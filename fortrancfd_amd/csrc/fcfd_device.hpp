@@ -21,6 +21,16 @@ struct TPow {
     double rtn0, rtn1, rtn2, rtn3, rtn4;     // RU*T^n for CKCPML
 };
 
+// Workgroup index that gives each of the GPU's eight XCDs one contiguous range of the grid: the
+// dispatcher places workgroup b on XCD b mod 8, so consecutive workgroups -- neighbouring cells,
+// whose stencils share cache lines -- would otherwise land in eight different L2 caches.  A
+// bijection of [0, gridDim.x): XCD x takes ranks x*q + min(x, r) .. (q or q + 1 of them).
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned n = gridDim.x, b = blockIdx.x;
+    const unsigned q = n / 8, r = n % 8, x = b % 8;
+    return x * q + (x < r ? x : r) + b / 8;
+}
+
 __device__ __forceinline__ TPow tpow(double T, double ru) {
     TPow p;
     const double t2 = T * T, t3 = T * t2, t4 = t2 * t2;

@@ -81,7 +81,7 @@ __device__ __forceinline__ Side pair_other(const Side &me) {
 
 template <int NSP, int RK, bool SPH, int BS = 64, int WPE = 2>
 __global__ void __launch_bounds__(BS, WPE) k_flux(FluxArgs a) {
-    const long g2 = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long g2 = long(xcd_block()) * blockDim.x + threadIdx.x;
     const long g = g2 >> 1;                               // interface (pairs never straddle a wave)
     const int s = int(threadIdx.x & 1);                   // 0: left state, 1: right state
     const int nif = a.ncell + 1;
@@ -243,7 +243,7 @@ __device__ __forceinline__ void ghost_source(const StageArgs &a, int b, int i, i
 // mode 1: Q = h1*Q0 + h2*(Qin - dt/dx*dF); mode 2: no update (ghost fill + QcToqf only).
 template <int NSP, int BS = 64, int WPE = 2>
 __global__ void __launch_bounds__(BS, WPE) k_stage_end(StageArgs a) {
-    const long c = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long c = long(xcd_block()) * blockDim.x + threadIdx.x;
     if (c >= long(a.nblk) * a.S) return;
     const int b = int(c / a.S), i = int(c % a.S) - a.lv;
     const bool ghost = (i < 0 || i >= a.ncell);
