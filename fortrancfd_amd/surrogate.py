@@ -7,7 +7,9 @@ Training pairs come from the batched-chemistry generator (workloads.chemistry_ba
 2600) K, p ~ log-U(1, 50) atm, dt ~ log-U(1e-8, 1e-6) s, mixing-line compositions) solved by this
 package's FP64 integrator on the GPU, optionally joined by the stiff-cell (state, result) pairs of
 coupled reacting runs with other hot-spot seeds than the benchmark's (ignition kernels of pristine
-fresh mixture, which the mixing line never produces).  The network is trained on the GPU by the library's own
+fresh mixture, which the mixing line never produces); --coupled-strang records those runs' source
+calls over dt/2 as the benchmark's Strang splitting makes them (below the batch generator's smallest
+dt), instead of over dt (Lie).  The network is trained on the GPU by the library's own
 forward / backward / Adam kernels (fcfd_sur_train_step): Adam, learning rate 1e-3 on a cosine
 schedule, batch 8192.  Species enter and leave the network through the Box-Cox transform
 b(Y) = 10 (Y^0.1 - 1); inputs are z-scored with the training set's mean and standard deviation;
@@ -71,11 +73,13 @@ def solve_pairs(mech, rho, dt, T, Y, chunk=131072, log=print):
     return T1, Y1
 
 
-def coupled_pairs(mech, seeds, steps, every=2, nblk=16, ncell=4096, dx=25e-6, cfl=0.5, log=print):
+def coupled_pairs(mech, seeds, steps, every=2, nblk=16, ncell=4096, dx=25e-6, cfl=0.5, log=print,
+                  strang=False):
     """(state, FP64 result) pairs of the cells the explicit screen sends to the implicit integrator
     in coupled reacting runs: the 65536-cell CH4/air domain with three 32-cell hot spots of the
-    given seeds, stepped with the FP64 chemistry; every `every`-th step the stiff cells' states
-    before and after solve_source are recorded (the steps in between replay the captured step).
+    given seeds, stepped with the FP64 chemistry, split Lie-style (fluid, then a dt source) or
+    (strang) as the benchmark splits (dt/2 source, fluid, dt/2 source); every `every`-th step the stiff cells' states
+    before and after each source call are recorded (the steps in between replay the captured step).
     Returns host arrays (rho, dt, T0, Y0, T1, Y1)."""
     from .lib import Solver
     out = {k: [] for k in ("rho", "dt", "T0", "Y0", "T1", "Y1")}
@@ -83,31 +87,43 @@ def coupled_pairs(mech, seeds, steps, every=2, nblk=16, ncell=4096, dx=25e-6, cf
     for seed in seeds:
         s = Solver(mech, nblk, ncell, dx, flag_acu=2, flag_weno=2, select_riemann=2,
                    select_time=1, do_reaction=1)
+        s.set_step(strang=strang, hrr_each_step=False)
         s.init_cells(workloads.coupled_cells(mech, nblk, ncell, seed=seed))
         lv = s.lv
         inner = (slice(None), slice(lv, lv + ncell))
         n0 = sum(len(v) for v in out["T0"])
         t = 0.0
-        for step in range(0, steps, every):
-            if every > 1:
-                t = s.advance(cfl, every - 1, t)         # device-resident steps in between
-            dt = s.calc_dt(cfl)
-            s.solve_fluid(dt)
+
+        def source(h, tnow):
             q0 = s.download()[1][inner].reshape(-1, mech.nequ)
-            s.solve_source(dt, diagnostics=False)
+            s.solve_source(h, tnow, diagnostics=False)
             route, _ = s.source_accepted()
             stiff = route == 0
             if stiff.any():
                 q1 = s.download()[1][inner].reshape(-1, mech.nequ)
                 y0 = np.maximum(q0[stiff, 3:], 0.0)
                 out["rho"].append(q0[stiff, 0])
-                out["dt"].append(np.full(stiff.sum(), dt))
+                out["dt"].append(np.full(stiff.sum(), h))
                 out["T0"].append(q0[stiff, 2])
                 out["Y0"].append(y0 / y0.sum(axis=1, keepdims=True))
                 out["T1"].append(q1[stiff, 2])
                 out["Y1"].append(q1[stiff, 3:])
             s.refresh()
+
+        for step in range(0, steps, every):
+            if every > 1:
+                t = s.advance(cfl, every - 1, t)         # device-resident steps in between
+            dt = s.calc_dt(cfl)
+            if strang:
+                source(0.5 * dt, t)
+                s.solve_fluid(dt)
+                source(0.5 * dt, t + dt)
+            else:
+                s.solve_fluid(dt)
+                source(dt, t + dt)
             t += dt
+            if (step + every) % 1000 < every:
+                log(f"    seed {seed}: step {step + every} ({time.time() - t0:.1f} s)")
         n1 = sum(len(v) for v in out["T0"])
         log(f"  coupled run seed {seed}: {steps} steps, {n1 - n0} stiff-cell pairs "
             f"({time.time() - t0:.1f} s)")
@@ -159,7 +175,7 @@ def load_surrogate(mech, path=DEFAULT_WEIGHTS, resid_tol=1e-5, t_tol=2e-3, verif
 
 
 def train(mech, n_pairs, epochs, batch=8192, lr=1e-3, seed=workloads.SEED + 4, log=print,
-          coupled_seeds=(), coupled_steps=5000, coupled_every=2):
+          coupled_seeds=(), coupled_steps=5000, coupled_every=2, coupled_strang=False):
     """Generate n_pairs batched-chemistry training pairs (plus, with coupled_seeds, the stiff-cell
     pairs of coupled runs with those hot-spot seeds), fit the scaling and train for `epochs`
     passes.  Returns (Surrogate, params, norm, meta)."""
@@ -170,7 +186,7 @@ def train(mech, n_pairs, epochs, batch=8192, lr=1e-3, seed=workloads.SEED + 4, l
     T1, Y1 = solve_pairs(mech, rho, dt, T0, Y0, log=log)
     n_batch = n_pairs
     if coupled_seeds:
-        c = coupled_pairs(mech, coupled_seeds, coupled_steps, coupled_every, log=log)
+        c = coupled_pairs(mech, coupled_seeds, coupled_steps, coupled_every, log=log, strang=coupled_strang)
         rho, dt, T0, Y0, T1, Y1 = (np.concatenate([a, b]) for a, b in zip((rho, dt, T0, Y0, T1, Y1), c))
         n_pairs = T0.size
     t_solve = time.time() - t0
@@ -206,6 +222,7 @@ def train(mech, n_pairs, epochs, batch=8192, lr=1e-3, seed=workloads.SEED + 4, l
     t_train = time.time() - t1
     meta = {"pairs": n_pairs, "batched_pairs": n_batch, "coupled_pairs": n_pairs - n_batch,
             "coupled_seeds": list(coupled_seeds), "coupled_steps": coupled_steps,
+            "coupled_every": coupled_every, "coupled_split": "strang" if coupled_strang else "lie",
             "epochs": epochs, "batch": batch, "lr": lr, "seed": seed,
             "steps": total, "final_loss": float(hist[-1]), "loss_history": [float(v) for v in hist],
             "solve_s": t_solve, "train_s": t_train,
@@ -282,6 +299,8 @@ def main(argv=None):
                    help="hot-spot seeds of coupled runs whose stiff cells join the training set")
     t.add_argument("--coupled-steps", type=int, default=5000)
     t.add_argument("--coupled-every", type=int, default=2)
+    t.add_argument("--coupled-strang", action="store_true",
+                   help="record the coupled runs' source calls over dt/2 (Strang) instead of dt (Lie)")
     e = sub.add_parser("eval")
     e.add_argument("--cells", type=int, default=65536)
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -291,7 +310,7 @@ def main(argv=None):
     if a.cmd == "train":
         s, params, norm, meta = train(mech, a.pairs, a.epochs, a.batch, a.lr, log=log,
                                       coupled_seeds=a.coupled_seeds, coupled_steps=a.coupled_steps,
-                                      coupled_every=a.coupled_every)
+                                      coupled_every=a.coupled_every, coupled_strang=a.coupled_strang)
         s.set_thresholds()
         if a.eval_cells:
             meta["eval"] = evaluate(mech, s, a.eval_cells, log=log)
