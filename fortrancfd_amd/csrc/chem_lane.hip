@@ -113,6 +113,18 @@ __device__ __forceinline__ double sum4(double *slot, int w, int cl, double v) {
     return s;
 }
 
+// The same for two values at once through a block of 2 * NW * NC doubles (the rate block Q, free
+// outside the reaction loop): one pair of barriers instead of two.
+__device__ __forceinline__ void sum4x2(double *pair, int w, int cl, double va, double vb, double &ra, double &rb) {
+    pair[w * NC + cl] = va;
+    pair[(NW + w) * NC + cl] = vb;
+    __syncthreads();
+    ra = ((pair[cl] + pair[NC + cl]) + pair[2 * NC + cl]) + pair[3 * NC + cl];
+    const double *q = pair + NW * NC;
+    rb = ((q[cl] + q[NC + cl]) + q[2 * NC + cl]) + q[3 * NC + cl];
+    __syncthreads();
+}
+
 // f = dy/dt at (y + s d, T) for this wave's species slots (f[j] of species w + NW*j) and fT for
 // all waves (SHIFT = false: at y).  rho in g/cm^3.  cl: this lane's LDS column.
 template <int NSP, bool SHIFT>
@@ -157,8 +169,9 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
         for (int r = 0; r < 7; ++r)
             if (r % NW == w) sh.PF[r * NC + cl] = pm[r];
     }
-    const double ctot = sum4(sh.S, w, cl, ctot_w);    // its barrier also publishes C, G and PF
-    const double cvm = sum4(sh.S, w, cl, cvm_w);
+    static_assert(CH >= 2 * NW, "the rate block holds a pair of cross-wave sums");
+    double ctot, cvm;
+    sum4x2(sh.Q, w, cl, ctot_w, cvm_w, ctot, cvm);     // its barrier also publishes C, G and PF
     const double *Cp = sh.C + cl, *Gp = sh.G + cl;
     const char *Cb = reinterpret_cast<const char *>(Cp), *Gb = reinterpret_cast<const char *>(Gp);
     const double *Pf = sh.PF + cl;
@@ -217,7 +230,7 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
 
 // Mixture e = sum Y u / W and cv (erg/g) over all species at T for this cell.
 template <int NSP>
-__device__ __forceinline__ void energy_lane(const DevThermo *__restrict__ th, double *slot, int w, int lane,
+__device__ __forceinline__ void energy_lane(const DevThermo *__restrict__ th, double *pair, int w, int lane,
                             const double (&y)[slots<NSP>()], double T, double &e, double &cv) {
     constexpr int NS = slots<NSP>();
     const TPow p = tpow(T, th->ru);
@@ -231,8 +244,7 @@ __device__ __forceinline__ void energy_lane(const DevThermo *__restrict__ th, do
             ec = ec + y[j] * (cp_mol(a, p) - th->ru) / th->wt[k];
         }
     }
-    e = sum4(slot, w, lane, eu);
-    cv = sum4(slot, w, lane, ec);
+    sum4x2(pair, w, lane, eu, ec, e, cv);
 }
 
 // adjust_sum (SRC/thermo.f90:136-149) of this wave's slots: clip at 0, divide by the cell's sum.
@@ -302,7 +314,7 @@ __global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) ya[j] = y[j];
         adjust_lane<NSP>(slot, w, lane, ya);
-        energy_lane<NSP>(th, slot, w, lane, ya, T0, e0, cv0);
+        energy_lane<NSP>(th, sh.Q, w, lane, ya, T0, e0, cv0);
     }
 
     // ---- Heun over dt with the embedded Euler error estimate ----
@@ -356,7 +368,7 @@ __global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
     for (int it = 0; it < 20; ++it) {
         if (!__syncthreads_or(done ? 0 : 1)) break;
         double ek, cvk;
-        energy_lane<NSP>(th, slot, w, lane, y, T, ek, cvk);
+        energy_lane<NSP>(th, sh.Q, w, lane, y, T, ek, cvk);
         if (!done) {
             const double d = (e0 - ek) / cvk;
             T = T + d;
@@ -380,8 +392,7 @@ __global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
                 s2 = s2 + th->h298[k] * fn[j] * rho;     // W_k h_k wdot_k = h_k (W_k wdot_k / rho) rho
             }
         }
-        hrr_s = sum4(slot, w, lane, s1);
-        hrr_d = sum4(slot, w, lane, s2);
+        sum4x2(sh.Q, w, lane, s1, s2, hrr_s, hrr_d);
     }
     if (!ok || !owner) return;
     if (a.layout == 0) {
@@ -468,7 +479,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
 #pragma unroll
         for (int j = 0; j < NS; ++j) ya[j] = y[j];
         adjust_lane<NSP>(slot, w, lane, ya);
-        energy_lane<NSP>(th, slot, w, lane, ya, T0, e0, cv0);
+        energy_lane<NSP>(th, sh.Q, w, lane, ya, T0, e0, cv0);
     }
 
     // ---- two Heun steps of dt/2; a lane keeps going only while its half steps are accepted ----
@@ -531,7 +542,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
     for (int it = 0; it < 20; ++it) {
         if (!__syncthreads_or(done ? 0 : 1)) break;
         double ek, cvk;
-        energy_lane<NSP>(th, slot, w, lane, y, T, ek, cvk);
+        energy_lane<NSP>(th, sh.Q, w, lane, y, T, ek, cvk);
         if (!done) {
             const double d = (e0 - ek) / cvk;
             T = T + d;
@@ -554,8 +565,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
                 s2 = s2 + th->h298[k] * fn[j] * rho;
             }
         }
-        hrr_s = sum4(slot, w, lane, s1);
-        hrr_d = sum4(slot, w, lane, s2);
+        sum4x2(sh.Q, w, lane, s1, s2, hrr_s, hrr_d);
     }
     if (!ok || !owner) return;
 #pragma unroll
