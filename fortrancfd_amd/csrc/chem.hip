@@ -262,7 +262,7 @@ __device__ __forceinline__ RhsOut rhs(const DevThermo *__restrict__ th, const De
         if (i >= K->ii) continue;
         // blocks from K->nplain0 on take the branch-free evaluation (a uniform test)
         const Rop r = b * 64 >= K->nplain0 ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
-                                           : reaction<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
+                                           : reaction_ilp<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
         rhs_scatter<NSP, NW>(K, sh, i, r, split, wave);
     }
     return rhs_end<NSP, NW>(th, sh, rho, y_l, pre, split);
@@ -327,7 +327,7 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
             const int *nu = K->nu[i], *nk = K->nk[i];
             const Rop r = bu * 64 >= K->nplain0
                               ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
-                              : reaction<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
+                              : reaction_ilp<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
             rr[u] = r;
             for (int s = 0; s < 6; ++s) {
                 dq[u][s] = 0.0;
@@ -1020,7 +1020,7 @@ __global__ void __launch_bounds__(64) k_rates(const DevThermo *__restrict__ th, 
                 const int i = b * 64 + lane;
                 if (i >= K->ii) continue;
                 const Rop q = b * 64 >= K->nplain0 ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
-                                                   : reaction<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
+                                                   : reaction_ilp<1>(K, i, T, alogt, pfac, pre.ctot, sh.c, sh.smh);
                 rhs_scatter<NSP, 1>(K, sh, i, q, split == 1, 0);
                 if (split == 0) {
                     qf[long(c) * K->ii + K->orig[i]] = q.qf;

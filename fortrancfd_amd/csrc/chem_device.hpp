@@ -135,6 +135,109 @@ __device__ __forceinline__ Rop reaction(const DevKin *__restrict__ K, int i, dou
     return r;
 }
 
+// reaction() for kernels whose lanes evaluate different rows (the integrator, the rates kernel): the
+// same operations with the same operands, but every exponential a row can need is evaluated before
+// the branches on the row's kind -- the Arrhenius factor, the equilibrium constant, and for fall-off
+// rows the low-pressure factor and the Troe terms -- so their dependent chains run side by side
+// instead of one after another down the lanes' divergent paths.  Values a row does not use are
+// discarded; a row with B = E = 0 multiplies A by exp(0) = 1 (exact).  Results are bitwise those of
+// reaction().
+template <int STRIDE>
+__device__ __forceinline__ Rop reaction_ilp(const DevKin *__restrict__ K, int i, double T, double alogt, double pfac,
+                                            double ctot, const double *__restrict__ c, const double *__restrict__ smh) {
+    const double ekf = fm_exp(K->B[i] * alogt - K->E[i] / T);
+    double s = 0.0;
+    for (int n = 0; n < 6; ++n) {
+        const int k = K->nk[i][n];
+        if (k >= 0) s = s + (double)K->nu[i][n] * smh[k * STRIDE];
+    }
+    const double eks = fm_exp(fmin(s, K->exparg));
+    const int f = K->fo[i];
+    const double *fp = K->fpar[f >= 0 ? f : 0];
+    const bool troe = f >= 0 && K->ifop[f] > 2;
+    double erk = 0.0, e4 = 0.0, e5 = 0.0, e6 = 0.0;
+    if (f >= 0) {
+        erk = fm_exp(fp[1] * alogt - fp[2] / T);
+        if (troe) {
+            e4 = fm_exp(-T / fp[4]);
+            e5 = fm_exp(-T / fp[5]);
+            e6 = fm_exp(-fp[6] / T);
+        }
+    }
+    double kf = K->A[i];
+    if (!K->arr0[i]) kf = kf * ekf;
+    double kr = 0.0;
+    if (K->rev[i]) {
+        double eqk = s != 0.0 ? eks : 1.0;
+        eqk = eqk * ipow(pfac, K->nus[i]);
+        kr = kf / fmax(eqk, K->small);
+    }
+    double ctb = 1.0;
+    const int t = K->tb[i];
+    if (t >= 0) {
+        ctb = ctot;
+        const int nl = K->ntbs[t];
+        constexpr int TB = 8;
+        for (int l0 = 0; l0 < nl; l0 += TB) {
+            double ta[TB];
+            int tk[TB];
+#pragma unroll
+            for (int u = 0; u < TB; ++u) {
+                const int l = min(l0 + u, MAXTB - 1);
+                ta[u] = K->tba[t][l];
+                tk[u] = K->tbk[t][l];
+            }
+#pragma unroll
+            for (int u = 0; u < TB; ++u)
+                if (l0 + u < nl) ctb = ctb + (ta[u] - 1.0) * c[(tk[u] >= 0 ? tk[u] : 0) * STRIDE];
+        }
+    }
+    if (f >= 0) {
+        const double rklow = fp[0] * erk;
+        double pr;
+        if (K->kfal[f] < 0) { pr = rklow * ctb / kf; ctb = 1.0; }
+        else pr = rklow * c[K->kfal[f] * STRIDE] / kf;
+        double pcor = pr / (1.0 + pr);
+        if (K->ifop[f] > 1) {
+            const double prlog = fm_log10(fmax(pr, K->small));
+            double fc;
+            if (K->ifop[f] == 2) {
+                const double xp = 1.0 / (1.0 + prlog * prlog);
+                fc = fm_pow(fp[3] * fm_exp(-fp[4] / T) + fm_exp(-T / fp[5]), xp) * fp[6] * fm_pow(T, fp[7]);
+            } else {
+                double fcent = (1.0 - fp[3]) * e4 + fp[3] * e5;
+                if (K->ifop[f] == 4) fcent = fcent + e6;
+                const double fclog = fm_log10(fmax(fcent, K->small));
+                // single-precision literals of cklib.f:6915-6917
+                const double xn = 0.75 - (double)1.27f * fclog;
+                const double cprlog = prlog - ((double)0.4f + (double)0.67f * fclog);
+                const double q = cprlog / (xn - (double)0.14f * cprlog);
+                fc = fm_exp10(fclog / (1.0 + q * q));
+            }
+            pcor = fc * pcor;
+        }
+        kf = kf * pcor;
+        kr = kr * pcor;
+    }
+    kf = kf * ctb * 1.0;
+    kr = kr * ctb * 1.0;
+    Rop r{kf, kr, 0.0, 0.0};
+    const int *nu = K->nu[i], *nk = K->nk[i];
+    if (nu[0] != 0) {
+        r.qf = kf * ipow(c[nk[0] * STRIDE], abs(nu[0]));
+        r.qr = kr * ipow(c[nk[3] * STRIDE], nu[3]);
+        if (nk[1] >= 0) {
+            r.qf = r.qf * ipow(c[nk[1] * STRIDE], abs(nu[1]));
+            if (nk[2] >= 0) r.qf = r.qf * ipow(c[nk[2] * STRIDE], abs(nu[2]));
+        }
+        if (nk[4] >= 0) {
+            r.qr = r.qr * ipow(c[nk[4] * STRIDE], nu[4]);
+            if (nk[5] >= 0) r.qr = r.qr * ipow(c[nk[5] * STRIDE], nu[5]);
+        }
+    }
+    return r;
+}
+
 // x**m for 0 <= m <= 3 without branches, with ipow's product order (x**3 = x * (x * x)).
 __device__ __forceinline__ double ipow3(double x, int m) {
     const double x2 = x * x;
