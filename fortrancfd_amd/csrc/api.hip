@@ -176,7 +176,8 @@ int ensure_dev_tables(const fcfd_mech *cm) {
 // next to the library's directory (FCFD_TUNE_FILE overrides), read once per process; entries for
 // other GPU architectures are ignored.  Keys: flux_variant, stage_variant (fluid.hpp),
 // chem_wg_per_cu (integrator workgroups per CU), chem_four_wave_factor (four-wave mode for lists of
-// at most factor x workgroups cells), chem_lu (1: the four-wave integrator's blocked matrix-core LU).
+// at most factor x workgroups cells), chem_lu (the four-wave integrator's LU: 0 step by step through
+// LDS, 1 blocked with matrix-core updates, 2 with the matrix in registers).
 struct TuneTable {
     std::mutex mu;
     bool loaded = false;
@@ -244,13 +245,24 @@ int tune_gen() {
     return t.gen;
 }
 
-int chem_coop_max(int n_workgroups, int nsp) { return tuned(nsp, "chem_four_wave_factor", 8) * n_workgroups; }
+// Value of a tuned entry that the table does not hold.
+static int launch_default(const char *key) {
+    if (std::strcmp(key, "chem_wg_per_cu") == 0) return 5;
+    if (std::strcmp(key, "chem_four_wave_factor") == 0) return 8;
+    if (std::strcmp(key, "chem_lu") == 0) return 2;
+    return 0;
+}
 
-int chem_lu_variant(int nsp) { return tuned(nsp, "chem_lu", 0); }
+int chem_coop_max(int n_workgroups, int nsp) {
+    return tuned(nsp, "chem_four_wave_factor", launch_default("chem_four_wave_factor")) * n_workgroups;
+}
 
-// Persistent grid size: one-wave workgroups per CU (default four, LDS-limited) times the CUs.
+int chem_lu_variant(int nsp) { return tuned(nsp, "chem_lu", launch_default("chem_lu")); }
+
+// Persistent grid size: one-wave workgroups per CU (default five, as many as the LDS of a CU holds)
+// times the CUs.
 int chem_workgroups(const fcfd_mech *m) {
-    return tuned(m->host.kk, "chem_wg_per_cu", 4) * (m->ncu > 0 ? m->ncu : 256);
+    return tuned(m->host.kk, "chem_wg_per_cu", launch_default("chem_wg_per_cu")) * (m->ncu > 0 ? m->ncu : 256);
 }
 
 }  // namespace fcfd
@@ -1338,7 +1350,7 @@ int fcfd_set_surrogate(fcfd_ctx *c, fcfd_surrogate *s) {
 int fcfd_set_launch(const char *key, int nsp, int value) {
     static const char *const keys[] = {"flux_variant", "stage_variant", "chem_wg_per_cu", "chem_four_wave_factor",
                                        "chem_lu"};
-    static const int lo[] = {0, 0, 1, 0, 0}, hi[] = {2, 3, 64, 1 << 20, 1};
+    static const int lo[] = {0, 0, 1, 0, 0}, hi[] = {2, 3, 64, 1 << 20, 2};
     if (!key) { set_error("fcfd_set_launch: invalid key"); return FCFD_E_ARG; }
     for (int i = 0; i < 5; ++i)
         if (std::strcmp(key, keys[i]) == 0) {
@@ -1356,8 +1368,7 @@ int fcfd_set_launch(const char *key, int nsp, int value) {
 
 int fcfd_get_launch(const char *key, int nsp, int *value) {
     if (!key || !value) { set_error("fcfd_get_launch: invalid argument"); return FCFD_E_ARG; }
-    const int dflt = std::strcmp(key, "chem_wg_per_cu") == 0 ? 4 : (std::strcmp(key, "chem_four_wave_factor") == 0 ? 8 : 0);
-    *value = tuned(nsp, key, dflt);
+    *value = tuned(nsp, key, launch_default(key));
     return FCFD_OK;
 }
 

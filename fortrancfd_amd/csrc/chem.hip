@@ -130,8 +130,12 @@ struct Shared {
     double c[64], smh[64], u[64];
     double rdiag[64];                // 1 / pivot of each elimination step
     int perm[64];
-    int rank[64];                    // elimination step of each row (lu_blocked)
-    double U12[NW == 4 ? 4 : 1][1024];   // per-wave panel rows right of the panel (lu_blocked)
+    // the four-wave layout's alternative factorizations; a placeholder element with one wave, so
+    // that the one-wave workgroup's LDS stays small enough for four workgroups per CU
+    int rank[NW == 4 ? 64 : 1];                    // elimination step of each row (lu_blocked)
+    double U12[NW == 4 ? 4 : 1][NW == 4 ? 1024 : 1];  // per-wave panel rows right of the panel (lu_blocked)
+    double lmul[2][NW == 4 ? 64 : 1];              // multipliers of the last two elimination steps (lu_regs)
+    int lu_ok;                                     // 0 once a step had no usable pivot (lu_regs)
     int cell;
 };
 
@@ -624,6 +628,89 @@ __device__ __forceinline__ bool lu_blocked(Shared<NSP, 4> &sh, double hg, int wa
     return ok;
 }
 
+// lu() for the four-wave workgroup with the matrix in registers: wave w holds columns
+// w*NCW .. w*NCW + NCW - 1 of every row (lane r: row r), so an elimination step moves no matrix data
+// through LDS.  At step k the wave owning column k finds the pivot (pivot_row, as lu()), forms the
+// open rows' multipliers l = rk / piv and publishes them with the pivot row (sh.lmul[k % 2],
+// sh.perm[k]); after one barrier every wave applies fma(-l, u_j, m_j) to its columns right of k, u_j
+// being the pivot row's entry (read from that lane).  Each entry sees the operations of lu() in the
+// same order, so the factors are bitwise lu()'s; they are written back to sh.J[0] for solve().  The
+// multipliers alternate between two buffers: a wave writes step k + 2's only after step k + 1's
+// barrier, which every wave passes after reading step k's.
+template <int NSP>
+__device__ __forceinline__ bool lu_regs(Shared<NSP, 4> &sh, double hg, int wave) {
+    constexpr int N = NSP + 1;
+    constexpr int NCW = (N + 3) / 4;      // columns per wave
+    static_assert(N <= 64, "one row per lane");
+    const int lane = lane_id();
+    int rank = lane < N ? N : -1;
+    double *const M = sh.J[0];
+    const double *row = M + (lane < N ? lane : 0) * LDJ;
+    const int j0 = wave * NCW;
+    double col[NCW];
+#pragma unroll
+    for (int jj = 0; jj < NCW; ++jj) {
+        const int j = j0 + jj;
+        double m = 0.0;
+        if (lane < N && j < N) {
+            m = -hg * row[j];
+            if (j == lane) m = m + 1.0;
+        }
+        col[jj] = m;
+    }
+    if (threadIdx.x == 0) sh.lu_ok = 1;
+    __syncthreads();
+    for (int blk = 0; blk < 4; ++blk) {
+#pragma unroll
+        for (int jj = 0; jj < NCW; ++jj) {
+            const int k = blk * NCW + jj;
+            if (k >= N) continue;
+            double *lk = sh.lmul[k & 1];
+            if (wave == blk) {
+                bool ok = true;
+                const double rk = col[jj];
+                const int p = pivot_row(rk, rank == N, ok);
+                const double piv = bcast(rk, p);
+                if (!(piv != 0.0)) ok = false;
+                if (!ok && lane == 0) sh.lu_ok = 0;
+                if (lane == 0) sh.perm[k] = p;
+                const bool upd = rank == N && lane != p;
+                const double l = rk / piv;
+                lk[lane] = upd ? l : 0.0;
+                if (upd) col[jj] = l;
+#pragma unroll
+                for (int jn = jj + 1; jn < NCW; ++jn) {
+                    const double u = bcast(col[jn], p);
+                    if (upd) col[jn] = fma(-l, u, col[jn]);
+                }
+            }
+            __syncthreads();
+            const int p = __builtin_amdgcn_readfirstlane(sh.perm[k]);
+            if (wave > blk) {
+                const bool upd = rank == N && lane != p;
+                const double l = lk[lane];
+#pragma unroll
+                for (int jn = 0; jn < NCW; ++jn) {
+                    const double u = bcast(col[jn], p);
+                    if (upd) col[jn] = fma(-l, u, col[jn]);
+                }
+            }
+            if (lane == p) rank = k;
+        }
+    }
+    if (lane < N) {
+        double *w = M + lane * LDJ;
+#pragma unroll
+        for (int jj = 0; jj < NCW; ++jj)
+            if (j0 + jj < N) w[j0 + jj] = col[jj];
+    }
+    __syncthreads();
+    const bool ok = sh.lu_ok != 0;
+    if (wave == 0 && lane < N) sh.rdiag[lane] = 1.0 / M[sh.perm[lane] * LDJ + lane];
+    __syncthreads();
+    return ok;
+}
+
 // dst[0 .. n) = src[0 .. n), by the whole workgroup.  The leading barrier makes every wave finish
 // its reads of dst first (a rejected attempt restores J over the factors its last solve read).
 __device__ __forceinline__ void copy_j(double *dst, const double *src, int n) {
@@ -711,9 +798,11 @@ __device__ __forceinline__ double adjusted(double y_l, int lane, int nsp) {
 // waves of a workgroup split every reaction loop and the LU updates of one cell and run the rest of
 // the integrator redundantly on identical values, so they take identical decisions and meet at the
 // same barriers; for short lists, where the launch lasts as long as its slowest cell.  A launch
-// runs only if its mode matches the list length (a.coop_max), read on the device.
+// runs only if its mode matches the list length (a.coop_max), read on the device.  The one-wave
+// mode is held to two waves per SIMD (256 registers a lane, a few spilled): its LDS admits five
+// workgroups per CU, which one wave per SIMD would cap at four.
 template <int NSP, int NW>
-__global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW == 1 ? 2 : 1))) k_chem(ChemArgs a) {
     __shared__ Shared<NSP, NW> sh;
     constexpr int N = NSP + 1;
     const int lane = lane_id();
@@ -824,7 +913,9 @@ __global__ void __launch_bounds__(64 * NW) k_chem(ChemArgs a) {
                 have_j = NW > 1;
                 const double hg = h * CH_G;
                 bool ok;
-                if constexpr (NW == 4) ok = a.lu_blocked ? lu_blocked<NSP>(sh, hg, wave) : lu<NSP, NW>(sh, hg, wave);
+                if constexpr (NW == 4)
+                    ok = a.lu_blocked == 1 ? lu_blocked<NSP>(sh, hg, wave)
+                                           : (a.lu_blocked == 2 ? lu_regs<NSP>(sh, hg, wave) : lu<NSP, NW>(sh, hg, wave));
                 else ok = lu<NSP, NW>(sh, hg, wave);
                 double k0 = 0.0, k1 = 0.0, k2 = 0.0, z = 0.0;
                 for (int st = 0; st < 3 && ok; ++st) {
@@ -930,8 +1021,9 @@ struct VerifyArgs {
     double tol;
 };
 
+// (two waves per SIMD like k_chem's one-wave mode, whose RHS function it shares)
 template <int NSP>
-__global__ void __launch_bounds__(64) k_chem_verify(ChemArgs a, VerifyArgs v) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_chem_verify(ChemArgs a, VerifyArgs v) {
     __shared__ Shared<NSP, 1> sh;
     constexpr int N = NSP + 1;
     const int lane = lane_id();

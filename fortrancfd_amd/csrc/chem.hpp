@@ -108,7 +108,7 @@ struct ChemArgs {
     // optional (domain layout): f at the start state of each cell the explicit screen rejects,
     // [NSP + 1][ncells] (species, then T); k_chem_retry starts from it instead of evaluating it again
     double *f0;
-    int lu_blocked;              // set by launch_chem: 1 = the four-wave mode factors with lu_blocked
+    int lu_blocked;              // set by launch_chem: the four-wave mode's LU (0 lu, 1 lu_blocked, 2 lu_regs)
 };
 
 // SDIRK integrator over all cells or over a work list (persistent grid): one cell per wave for long
@@ -125,8 +125,8 @@ hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st);
 // Longest list that the four-wave integrator mode takes: the tuned factor (default 8) times the
 // workgroup count (api.hip).
 int chem_coop_max(int n_workgroups, int nsp);
-// 1 when the four-wave integrator factors its Newton matrix with the blocked, matrix-core LU (the
-// tuned "chem_lu" entry, api.hip); the factors are the same either way.
+// The four-wave integrator's LU (the tuned "chem_lu" entry, api.hip): 0 step by step through LDS,
+// 1 blocked with matrix-core trailing updates, 2 with the matrix in registers; the same factors.
 int chem_lu_variant(int nsp);
 hipError_t launch_zero_u32(unsigned *p, hipStream_t st);
 // *out = f * *in on the device (one thread), e.g. the half step of Strang splitting.

@@ -225,11 +225,11 @@ int fcfd_set_timing(fcfd_ctx *c, int enable);
  * without a GPU), absent keys keep their
  * defaults.  Keys: "flux_variant" (WENO5 advection flux: 0 = 64 threads and two waves per SIMD,
  * 1 = 64 / one, 2 = 128 / one), "stage_variant" (stage update: 0 = 64 / two, 1 = 64 / one,
- * 2 = 128 / one, 3 = 256 / one), "chem_wg_per_cu" (implicit-integrator workgroups per CU, default 4),
+ * 2 = 128 / one, 3 = 256 / one), "chem_wg_per_cu" (implicit-integrator workgroups per CU, default 5),
  * "chem_four_wave_factor" (four-wave integrator mode for lists of at most factor x workgroups cells,
- * default 8), "chem_lu" (0 = the four-wave integrator factors its Newton matrix one elimination step
- * at a time, 1 = in 16-column panels with the FP64 matrix cores updating the rest; the factors are
- * bitwise the same, default 0).  fcfd_set_launch overrides one entry for the mechanisms of nsp species (every context
+ * default 8), "chem_lu" (how the four-wave integrator factors its Newton matrix: 0 = one elimination
+ * step at a time through LDS, 1 = in 16-column panels with the FP64 matrix cores updating the rest,
+ * 2 = with each wave's columns held in registers; the factors are bitwise the same, default 2).  fcfd_set_launch overrides one entry for the mechanisms of nsp species (every context
  * re-captures its step graph); fcfd_load_tuning reloads the table (NULL: the default file). */
 int fcfd_set_launch(const char *key, int nsp, int value);
 int fcfd_get_launch(const char *key, int nsp, int *value);

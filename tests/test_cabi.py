@@ -70,10 +70,11 @@ def test_launch_table_file_and_overrides(tmp_path):
     L = lib()
     get = lambda key, nsp=53: (lambda v: (L.fcfd_get_launch(key.encode(), nsp, C.byref(v)), v.value)[1])(C.c_int())
     f = tmp_path / "t.txt"
-    f.write_text("# comment\nnone 53 flux_variant 2\ngfx000 53 stage_variant 3\nnone 11 chem_wg_per_cu 5  # x\n")
+    f.write_text("# comment\nnone 53 flux_variant 2\ngfx000 53 stage_variant 3\nnone 11 chem_wg_per_cu 3  # x\n")
     assert L.fcfd_load_tuning(str(f).encode()) == 0
     assert get("flux_variant") == 2 and get("stage_variant") == 0
-    assert get("chem_wg_per_cu", 11) == 5 and get("chem_wg_per_cu") == 4 and get("chem_four_wave_factor") == 8
+    assert get("chem_wg_per_cu", 11) == 3 and get("chem_wg_per_cu") == 5 and get("chem_four_wave_factor") == 8
+    assert get("chem_lu") == 2
     assert L.fcfd_set_launch(b"stage_variant", 53, 3) == 0 and get("stage_variant") == 3
     assert L.fcfd_set_launch(b"stage_variant", 53, 4) != 0
     assert L.fcfd_set_launch(b"no_such_key", 53, 1) != 0

@@ -49,7 +49,7 @@ def test_fluid_variants_bit_exact(mech, key, value):
     assert t > 0
 
 
-@pytest.mark.parametrize("wg,fw,lu", [(4, 8, 0), (4, 0, 0), (2, 1000, 0), (5, 0, 0), (4, 1000, 1)])
+@pytest.mark.parametrize("wg,fw,lu", [(4, 8, 0), (4, 0, 0), (2, 1000, 0), (5, 0, 0), (4, 1000, 1), (4, 1000, 2)])
 def test_integrator_modes_vs_oracle(mech, orc, wg, fw, lu):
     """Each integrator mode (four waves per cell for short lists, one wave per cell, other grid
     sizes) against the oracle on the coupled hot-spot domain: the modes sum the Jacobian's per-wave
@@ -66,9 +66,10 @@ def test_integrator_modes_vs_oracle(mech, orc, wg, fw, lu):
 
 
 def test_blocked_lu_bitwise(mech):
-    """The four-wave integrator's blocked LU (16-column panels, FP64 matrix-core updates) gives the
-    factors of the step-by-step LU bit for bit, so whole reacting runs agree exactly: the coupled
-    hot-spot domain for three steps and a batch of stiff cells, each with both factorizations."""
+    """The four-wave integrator's blocked LU (16-column panels, FP64 matrix-core updates) and its
+    register LU (each wave's columns in registers) give the factors of the step-by-step LU bit for
+    bit, so whole reacting runs agree exactly: the coupled hot-spot domain for three steps and a
+    batch of stiff cells, with each factorization."""
     import torch
     from fortrancfd_amd.lib import Solver
     from fortrancfd_amd.workloads import SEED, chemistry_batch
@@ -76,7 +77,7 @@ def test_blocked_lu_bitwise(mech):
     cells = ch4_air_cells(mech.kk, 64, [(0, 12, 2100.0), (12, 4, 1600.0)])
     set_launch("chem_four_wave_factor", 1000)           # every list on four waves per cell
     out = []
-    for lu in (0, 1):
+    for lu in (0, 1, 2):
         set_launch("chem_lu", lu)
         print(f"coupled hot-spot domain, chem_lu={lu}", flush=True)
         s = Solver(mech, 2, 32, 25e-6, flag_acu=2, flag_weno=2, select_riemann=2, select_time=1,
@@ -84,11 +85,12 @@ def test_blocked_lu_bitwise(mech):
         s.init_cells(cells)
         s.advance(0.5, 3)
         out.append(s.download())
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
+    for o in out[1:]:
+        np.testing.assert_array_equal(out[0][0], o[0])
+        np.testing.assert_array_equal(out[0][1], o[1])
     rho, dt, T, Y = chemistry_batch(mech, 256, seed=SEED + 29)
     res = []
-    for lu in (0, 1):
+    for lu in (0, 1, 2):
         set_launch("chem_lu", lu)
         print(f"256 batch cells, chem_lu={lu}", flush=True)
         d = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
@@ -98,5 +100,6 @@ def test_blocked_lu_bitwise(mech):
         torch.cuda.synchronize()
         res.append((Tf.cpu().numpy(), Yf.cpu().numpy(), stats.cpu().numpy()))
     assert res[0][2][:, 0].sum() > 0 and (res[0][2][:, 1] + res[0][2][:, 2]).sum() > 0
-    for a, b in zip(res[0], res[1]):
-        np.testing.assert_array_equal(a, b)
+    for r in res[1:]:
+        for a, b in zip(res[0], r):
+            np.testing.assert_array_equal(a, b)
