@@ -10,14 +10,14 @@
 //
 // Mapping: a workgroup of four waves owns NC = 56 cells (lane l < NC of every wave is cell l; lanes
 // NC..63 repeat cell NC - 1 and store nothing).  Concentrations and S/R - H/RT of the cells live in
-// LDS as species-major [k][NC] blocks.  The reactions are taken in chunks of 8 rows: each wave
-// evaluates two rows of a chunk for all cells and puts their rates of progress in LDS; after a
+// LDS as species-major [k][NC] blocks.  The reactions are taken in chunks of 16 rows: each wave
+// evaluates four rows of a chunk for all cells and puts their rates of progress in LDS; after a
 // barrier every wave adds the chunk's contributions to the species it owns (k mod 4 = wave) from a
 // precomputed list, in row order, into one shared production-rate block whose rows no two waves
 // write.  Each species' production rate is therefore one sequential sum over the rows, independent
 // of the wave count.  Species-wise work (thermo, state update, norms) is split over the waves by
 // k = wave mod 4; cell-wise scalars (T, sums) are combined through LDS in wave order.  The LDS
-// footprint (80,000 B) lets two workgroups share a compute unit: two waves per SIMD.  Reaction tables
+// footprint (81,792 B) lets two workgroups share a compute unit: two waves per SIMD.  Reaction tables
 // are read with a wave-uniform row index (scalar loads).
 #include <algorithm>
 
@@ -39,8 +39,8 @@ struct LaneShared {
     double C[NSP * NC];        // concentrations, mol/cm^3
     double G[NSP * NC];        // S/R - H/RT
     double W[NSP * NC];        // production rates; row k is written by wave k mod NW only
-    double Q[CH * NC];         // rates of progress (forward minus reverse) of the current chunk
-    double S[NW * NC];         // per-wave values of a cross-wave sum
+    double Q[CH * NC];         // rates of progress (forward minus reverse) of the current chunk; outside
+                               // the reaction loop, the per-wave values of cross-wave sums
     double PF[7 * NC];         // pfac**m for m = -3 .. 3 (row m + 3), the equilibrium constants' factor
 };
 
@@ -104,7 +104,8 @@ __device__ __forceinline__ double lane_plain_q(const LanePlain *__restrict__ L, 
 template <int NSP>
 constexpr int slots() { return (NSP + NW - 1) / NW; }
 
-// Sum over the four waves of one value per cell, in wave order, through sh.S.
+// Sum over the four waves of one value per cell, in wave order, through a block of NW * NC doubles
+// (the start of the rate block Q, free outside the reaction loop).
 __device__ __forceinline__ double sum4(double *slot, int w, int cl, double v) {
     slot[w * NC + cl] = v;
     __syncthreads();
@@ -224,7 +225,7 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
             su_w = su_w + uk[j] * wd;
         }
     }
-    const double su = sum4(sh.S, w, cl, su_w);
+    const double su = sum4(sh.Q, w, cl, su_w);     // after the last chunk's closing barrier
     fT = -su / (rho * cvm);
 }
 
@@ -280,7 +281,7 @@ __global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
     const bool live = cell < a.ncells;
     const bool owner = live && int(threadIdx.x & 63) < NC;
     const int c = live ? cell : cell0;          // columns past the end shadow a live cell
-    double *slot = sh.S;
+    double *slot = sh.Q;
 
     long col = 0;
     double rho_si, dt, T0;
@@ -458,7 +459,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
     const bool live = item < n;
     const bool owner = live && int(threadIdx.x & 63) < NC;
     const int cell = a.list[live ? item : item0];   // columns past the end shadow a live cell
-    double *slot = sh.S;
+    double *slot = sh.Q;
 
     const int b = cell / a.ncell, i = cell % a.ncell;
     const long col = long(b) * a.S + a.lv + i;
