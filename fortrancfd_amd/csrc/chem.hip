@@ -1209,7 +1209,7 @@ hipError_t launch_rates(const DevThermo *th, const DevKin *K, int nsp, int n, co
 hipError_t launch_chem_verify(const ChemArgs &a, int nsp, const int *cand_list, const unsigned *ncand,
                               const double *cand, int rows_max, int *fallback, unsigned *nfallback, double tol,
                               hipStream_t st) {
-    // one-wave workgroups, at most four per CU (the register budget's residency)
+    // one-wave workgroups, at most 1024 (four per CU; two waves per SIMD and the LDS would admit five)
     const unsigned nb = unsigned(std::min(rows_max, 1024));
     if (nb == 0) return hipSuccess;
     VerifyArgs v{cand_list, ncand, cand, fallback, nfallback, tol};
@@ -1233,11 +1233,13 @@ hipError_t launch_chem_full(ChemArgs a, int nsp, int n_workgroups, int *list, un
     return launch_chem(a, nsp, n_workgroups, st);
 }
 
-// n_workgroups one-wave workgroups (four per CU) for long lists; a quarter as many four-wave
-// workgroups (one per CU) for lists of at most 8 * n_workgroups cells (32 per CU).  Up to there the
-// launch is set by its slowest cells, which run about twice as fast with four waves, more than the
-// four-wave mode's lower throughput costs.  Both kernels are launched; the one whose mode does not
-// match the list length on the device returns at once.
+// n_workgroups one-wave workgroups (five per CU by default) for long lists; a quarter as many
+// four-wave workgroups for lists of at most a.coop_max cells (the tuned factor, default 8, times
+// n_workgroups).  The four-wave workgroup's LDS admits one per CU: the first CU-count of them run,
+// and the others start only once the queue has drained and return.  Up to coop_max the launch is set
+// by its slowest cells, which run about twice as fast with four waves, more than the four-wave mode's
+// lower throughput costs.  Both kernels are launched; the one whose mode does not match the list
+// length on the device returns at once.
 hipError_t launch_chem(const ChemArgs &a0, int nsp, int n_workgroups, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, a0.counter);
     hipError_t e = hipGetLastError();
