@@ -5,9 +5,9 @@
 
 namespace fcfd {
 
-constexpr int LANE_NW = 4;       // waves per workgroup of the lane-per-cell kernels
-constexpr int LANE_CHUNK = 16;   // reaction rows per chunk of their scatter (four per wave)
-constexpr int LANE_NC = 56;      // cells per workgroup of the lane-per-cell kernels
+constexpr int LANE_NW = 8;       // waves per workgroup of the lane-per-cell kernels
+constexpr int LANE_CHUNK = 32;   // reaction rows per chunk of their scatter (four per wave)
+constexpr int LANE_NC = 64;      // cells per workgroup of the lane-per-cell kernels
 
 // A row of the branch-free range (nplain0 .. ii-1) as the lane-per-cell kernels read it, with one
 // row index for the whole wave: coefficients as doubles, and the species of each present slot as the

@@ -8,17 +8,18 @@
 // the order of the species sums differs.  k_chem_retry runs the integrator's next explicit attempt
 // (two half steps) the same way over a long work list, before the SDIRK kernel sees it.
 //
-// Mapping: a workgroup of four waves owns NC = 56 cells (lane l < NC of every wave is cell l; lanes
-// NC..63 repeat cell NC - 1 and store nothing).  Concentrations and S/R - H/RT of the cells live in
-// LDS as species-major [k][NC] blocks.  The reactions are taken in chunks of 16 rows: each wave
-// evaluates four rows of a chunk for all cells and puts their rates of progress in LDS; after a
-// barrier every wave adds the chunk's contributions to the species it owns (k mod 4 = wave) from a
-// precomputed list, in row order, into one shared production-rate block whose rows no two waves
-// write.  Each species' production rate is therefore one sequential sum over the rows, independent
-// of the wave count.  Species-wise work (thermo, state update, norms) is split over the waves by
-// k = wave mod 4; cell-wise scalars (T, sums) are combined through LDS in wave order.  The LDS
-// footprint (81,792 B) lets two workgroups share a compute unit: two waves per SIMD.  Reaction tables
-// are read with a wave-uniform row index (scalar loads).
+// Mapping: a workgroup of NW = 8 waves owns NC = 64 cells (lane l of every wave is cell l; with
+// NC < 64, lanes NC..63 would repeat cell NC - 1 and store nothing).  Concentrations and S/R - H/RT of
+// the cells live in LDS as species-major [k][NC] blocks.  The reactions are taken in chunks of
+// CH = 32 rows: each wave evaluates four rows of a chunk for all cells and puts their rates of
+// progress in LDS; after a barrier every wave adds the chunk's contributions to the species it owns
+// (k mod NW = wave) from a precomputed list, in row order, into one shared production-rate block
+// whose rows no two waves write.  Each species' production rate is therefore one sequential sum over
+// the rows, independent of the wave count.  Species-wise work (thermo, state update, norms) is split
+// over the waves by k mod NW; cell-wise scalars (T, sums) are combined through LDS in wave order.
+// The LDS footprint (~101 KB) holds one workgroup per compute unit: two waves per SIMD, and every
+// lane a cell, so 65,536 cells make exactly four rounds of 1024 workgroups on 256 CUs.  Reaction
+// tables are read with a wave-uniform row index (scalar loads).
 #include <algorithm>
 
 #include "chem.hpp"
@@ -104,25 +105,32 @@ __device__ __forceinline__ double lane_plain_q(const LanePlain *__restrict__ L, 
 template <int NSP>
 constexpr int slots() { return (NSP + NW - 1) / NW; }
 
-// Sum over the four waves of one value per cell, in wave order, through a block of NW * NC doubles
+// Sum over the NW waves of one value per cell, in wave order, through a block of NW * NC doubles
 // (the start of the rate block Q, free outside the reaction loop).
-__device__ __forceinline__ double sum4(double *slot, int w, int cl, double v) {
+__device__ __forceinline__ double sum_waves(double *slot, int w, int cl, double v) {
     slot[w * NC + cl] = v;
     __syncthreads();
-    const double s = ((slot[cl] + slot[NC + cl]) + slot[2 * NC + cl]) + slot[3 * NC + cl];
+    double s = slot[cl];
+#pragma unroll
+    for (int u = 1; u < NW; ++u) s = s + slot[u * NC + cl];
     __syncthreads();
     return s;
 }
 
 // The same for two values at once through a block of 2 * NW * NC doubles (the rate block Q, free
 // outside the reaction loop): one pair of barriers instead of two.
-__device__ __forceinline__ void sum4x2(double *pair, int w, int cl, double va, double vb, double &ra, double &rb) {
+__device__ __forceinline__ void sum_waves2(double *pair, int w, int cl, double va, double vb, double &ra, double &rb) {
     pair[w * NC + cl] = va;
     pair[(NW + w) * NC + cl] = vb;
     __syncthreads();
-    ra = ((pair[cl] + pair[NC + cl]) + pair[2 * NC + cl]) + pair[3 * NC + cl];
     const double *q = pair + NW * NC;
-    rb = ((q[cl] + q[NC + cl]) + q[2 * NC + cl]) + q[3 * NC + cl];
+    ra = pair[cl];
+    rb = q[cl];
+#pragma unroll
+    for (int u = 1; u < NW; ++u) {
+        ra = ra + pair[u * NC + cl];
+        rb = rb + q[u * NC + cl];
+    }
     __syncthreads();
 }
 
@@ -163,7 +171,7 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
     const double pfac = th->pa / (th->ru * T);
     {
         // pfac**m as reaction_plain forms it (x**3 = x * (x * x), x**-m = 1 / x**m); row m + 3 of PF,
-        // rows w and w + 4 written by wave w
+        // row r written by wave r mod NW
         const double p2 = pfac * pfac, p3 = pfac * p2;
         const double pm[7] = {1.0 / p3, 1.0 / p2, 1.0 / pfac, 1.0, pfac, p2, p3};
 #pragma unroll
@@ -172,7 +180,7 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
     }
     static_assert(CH >= 2 * NW, "the rate block holds a pair of cross-wave sums");
     double ctot, cvm;
-    sum4x2(sh.Q, w, cl, ctot_w, cvm_w, ctot, cvm);     // its barrier also publishes C, G and PF
+    sum_waves2(sh.Q, w, cl, ctot_w, cvm_w, ctot, cvm);     // its barrier also publishes C, G and PF
     const double *Cp = sh.C + cl, *Gp = sh.G + cl;
     const char *Cb = reinterpret_cast<const char *>(Cp), *Gb = reinterpret_cast<const char *>(Gp);
     const double *Pf = sh.PF + cl;
@@ -225,7 +233,7 @@ __device__ __forceinline__ void rhs_lane(const DevThermo *__restrict__ th, const
             su_w = su_w + uk[j] * wd;
         }
     }
-    const double su = sum4(sh.Q, w, cl, su_w);     // after the last chunk's closing barrier
+    const double su = sum_waves(sh.Q, w, cl, su_w);     // after the last chunk's closing barrier
     fT = -su / (rho * cvm);
 }
 
@@ -245,7 +253,7 @@ __device__ __forceinline__ void energy_lane(const DevThermo *__restrict__ th, do
             ec = ec + y[j] * (cp_mol(a, p) - th->ru) / th->wt[k];
         }
     }
-    sum4x2(pair, w, lane, eu, ec, e, cv);
+    sum_waves2(pair, w, lane, eu, ec, e, cv);
 }
 
 // adjust_sum (SRC/thermo.f90:136-149) of this wave's slots: clip at 0, divide by the cell's sum.
@@ -258,17 +266,17 @@ __device__ __forceinline__ void adjust_lane(double *slot, int w, int lane, doubl
         y[j] = (w + NW * j < NSP) ? fmax(y[j], 0.0) : 0.0;
         s = s + y[j];
     }
-    const double tot = sum4(slot, w, lane, s);
+    const double tot = sum_waves(slot, w, lane, s);
 #pragma unroll
     for (int j = 0; j < NS; ++j) y[j] = y[j] / tot;
 }
 
 template <int NSP>
-__global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
+__global__ void __launch_bounds__(64 * NW) k_chem_screen(ChemArgs a) {
     constexpr int NS = slots<NSP>();
     __shared__ LaneShared<NSP> sh;
     // the wave index is uniform across the wave; saying so turns every table read indexed by it
-    // (reactions i = w + 4 n, species k = w + 4 j) into a scalar load and every branch on the
+    // (reactions i = w + NW n, species k = w + NW j) into a scalar load and every branch on the
     // reaction kind into a scalar branch
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = min(int(threadIdx.x & 63), NC - 1);   // the LDS column (lanes >= NC repeat NC - 1)
@@ -337,7 +345,7 @@ __global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
         const double t = ((dt * 0.5) * (f1T - f0T)) * (1.0 / (a.rtol * fabs(T0)));
         err_w = err_w + t * t;
     }
-    const double err = sqrt(sum4(slot, w, lane, err_w) / double(NSP + 1));
+    const double err = sqrt(sum_waves(slot, w, lane, err_w) / double(NSP + 1));
     const bool ok = live && err <= 1.0;
 
     if (!ok && owner && a.f0) {   // f at the start state, for the retry of a long list
@@ -393,7 +401,7 @@ __global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
                 s2 = s2 + th->h298[k] * fn[j] * rho;     // W_k h_k wdot_k = h_k (W_k wdot_k / rho) rho
             }
         }
-        sum4x2(sh.Q, w, lane, s1, s2, hrr_s, hrr_d);
+        sum_waves2(sh.Q, w, lane, s1, s2, hrr_s, hrr_d);
     }
     if (!ok || !owner) return;
     if (a.layout == 0) {
@@ -434,7 +442,7 @@ __global__ void __launch_bounds__(256, 2) k_chem_screen(ChemArgs a) {
 // a.stiff unchanged with *a.retry_done = 0: the integrator then runs it four waves per cell and
 // retries each cell itself.  Domain layout only.
 template <int NSP>
-__global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
+__global__ void __launch_bounds__(64 * NW) k_chem_retry(ChemArgs a) {
     constexpr int NS = slots<NSP>();
     __shared__ LaneShared<NSP> sh;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -447,7 +455,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
             *a.retry_done = 0;
             *a.stiff_n = unsigned(n);
         }
-        for (int i = int(blockIdx.x) * 256 + int(threadIdx.x); i < n; i += int(gridDim.x) * 256) a.stiff[i] = a.list[i];
+        for (int i = int(blockIdx.x) * (64 * NW) + int(threadIdx.x); i < n; i += int(gridDim.x) * (64 * NW)) a.stiff[i] = a.list[i];
         return;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.retry_done = 1;
@@ -513,7 +521,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
             const double t = ((hs * 0.5) * (fbT - faT)) * (1.0 / (a.rtol * fabs(Th)));
             err_w = err_w + t * t;
         }
-        const double err = sqrt(sum4(slot, w, lane, err_w) / double(NSP + 1));
+        const double err = sqrt(sum_waves(slot, w, lane, err_w) / double(NSP + 1));
         ok = ok && err <= 1.0;
         if (ok) {
 #pragma unroll
@@ -521,7 +529,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
             Th = Th + (hs * 0.5) * (faT + fbT);
         }
         // no lane of the workgroup passed the first half step: the second would be wasted work.
-        // ok is the same in the four waves of a lane (err comes from sum4), and every wave reaches
+        // ok is the same in all waves of a lane (err comes from sum_waves), and every wave reaches
         // this barrier, so the exit is uniform
         if (sub == 0 && !__syncthreads_or(ok ? 1 : 0)) break;
     }
@@ -566,7 +574,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
                 s2 = s2 + th->h298[k] * fn[j] * rho;
             }
         }
-        sum4x2(sh.Q, w, lane, s1, s2, hrr_s, hrr_d);
+        sum_waves2(sh.Q, w, lane, s1, s2, hrr_s, hrr_d);
     }
     if (!ok || !owner) return;
 #pragma unroll
@@ -591,7 +599,7 @@ __global__ void __launch_bounds__(256) k_chem_retry(ChemArgs a) {
 hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st) {
     const unsigned nb = unsigned((a.ncells + NC - 1) / NC);
     if (nb == 0) return hipSuccess;
-#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_retry<N>), dim3(nb), dim3(256), 0, st, a); return hipGetLastError(); }
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_retry<N>), dim3(nb), dim3(64 * NW), 0, st, a); return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;
@@ -600,7 +608,7 @@ hipError_t launch_chem_retry(const ChemArgs &a, int nsp, hipStream_t st) {
 hipError_t launch_chem_screen(const ChemArgs &a, int nsp, hipStream_t st) {
     const unsigned nb = unsigned((a.ncells + NC - 1) / NC);
     if (nb == 0) return hipSuccess;
-#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_screen<N>), dim3(nb), dim3(256), 0, st, a); return hipGetLastError(); }
+#define FCFD_CASE(N) if (nsp == N) { hipLaunchKernelGGL((k_chem_screen<N>), dim3(nb), dim3(64 * NW), 0, st, a); return hipGetLastError(); }
     FCFD_FOR_EACH_NSP(FCFD_CASE)
 #undef FCFD_CASE
     return hipErrorInvalidValue;
