@@ -51,6 +51,11 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs a) {
     const int tile = blockIdx.x;
     const int m0 = (tile / tiles_n) * 64, n0 = (tile % tiles_n) * 64;
     if (a.m_dev && a.m_off + m0 >= int(*a.m_dev)) return;   // uniform per workgroup, before any barrier
+    if (a.m_dev && a.row_live) {
+        // every wave reads the same 64 flags, so all four leave together, before any barrier
+        const int r = a.m_off + m0 + lane;
+        if (!__any(r < int(*a.m_dev) && a.row_live[r] != 0)) return;
+    }
     const int split = blockIdx.y;
     const int kchunk = a.K / a.ksplit;
     const int kbeg = split * kchunk, kend = kbeg + kchunk;
@@ -306,6 +311,14 @@ __global__ void __launch_bounds__(256) k_apply(ApplyArgs a) {
     } else if (row >= a.n) {
         return;
     }
+    if (a.in_range[row] == 0) {     // outside the training range: the network skipped this row
+        if (lane == 0) {
+            if (a.route) a.route[t] = 0;
+            const unsigned q = atomicAdd(a.nfallback, 1u);
+            a.fallback[q] = t;
+        }
+        return;
+    }
     const DevThermo *__restrict__ th = a.thermo;
     const SurrogateNorm *__restrict__ nm = a.norm;
     const bool sp = lane < NSP;
@@ -316,7 +329,7 @@ __global__ void __launch_bounds__(256) k_apply(ApplyArgs a) {
     const float *out = a.Yh + long(row) * MLP_OUTP;
     double y1 = sp ? boxcox_inv(boxcox(y0) + (double(out[lane]) * nm->out_std[lane] + nm->out_mean[lane])) : 0.0;
     const double Tnet = T0 + (double(out[NSP]) * nm->out_std[NSP] + nm->out_mean[NSP]);
-    bool ok = a.in_range[row] != 0;
+    bool ok = true;
     const double iw = sp ? 1.0 / th->wt[lane] : 0.0;
     double ae[MM];
 #pragma unroll

@@ -40,6 +40,9 @@ struct GemmArgs {
     int epilogue;
     const unsigned *m_dev;   // optional device row count: 64-row tiles with m_off + m0 >= *m_dev are skipped
     int m_off;
+    // optional (with m_dev): per-row flags indexed like m_off + row; a 64-row tile none of whose rows
+    // below *m_dev is flagged is skipped, its output rows left as they were
+    const int *row_live;
 };
 
 hipError_t launch_gemm(const GemmArgs &a, hipStream_t st);

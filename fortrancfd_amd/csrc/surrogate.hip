@@ -75,10 +75,12 @@ hipError_t Surrogate::reserve(int rows, bool training) {
 
 hipError_t Surrogate::gemm(const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, int M,
                            int N, int K, int ksplit, int ep, float *C32, uint16_t *C16, float *P32,
-                           const float *bias, long ldc, hipStream_t st, const unsigned *m_dev, int m_off) const {
+                           const float *bias, long ldc, hipStream_t st, const unsigned *m_dev, int m_off,
+                           const int *row_live) const {
     GemmArgs g{};
     g.m_dev = m_dev;
     g.m_off = m_off;
+    g.row_live = row_live;
     g.A = A; g.B = B; g.sam = sam; g.sak = sak; g.sbk = sbk; g.sbn = sbn;
     g.M = M; g.N = N; g.K = K; g.ksplit = ksplit;
     g.C32 = C32; g.C16 = C16; g.P32 = P32; g.bias = bias; g.ldc = ldc; g.epilogue = ep;
@@ -88,7 +90,7 @@ hipError_t Surrogate::gemm(const uint16_t *A, long sam, long sak, const uint16_t
 // Forward of `rows` rows of X (rows multiple of 64).  Training keeps every layer's activations
 // and pre-activations; inference ping-pongs two activation buffers chunk by chunk.
 hipError_t Surrogate::forward(const uint16_t *x, int rows, float *out, bool training, hipStream_t st,
-                              const unsigned *m_dev) {
+                              const unsigned *m_dev, const int *row_live) {
     hipError_t e;
     if (training) {
         const uint16_t *in = x;
@@ -107,12 +109,12 @@ hipError_t Surrogate::forward(const uint16_t *x, int rows, float *out, bool trai
         for (int l = 0; l < 4; ++l) {
             uint16_t *o = Hping[l & 1];
             e = gemm(in, MLP_K[l], 1, W16[l], MLP_N[l], 1, cr, MLP_N[l], MLP_K[l], 1, EP_BIAS_GELU, nullptr, o,
-                     nullptr, b32[l], MLP_N[l], st, m_dev, r0);
+                     nullptr, b32[l], MLP_N[l], st, m_dev, r0, row_live);
             if (e != hipSuccess) return e;
             in = o;
         }
         e = gemm(in, MLP_HID, 1, W16[4], MLP_OUTP, 1, cr, MLP_OUTP, MLP_HID, 1, EP_BIAS, out + long(r0) * MLP_OUTP,
-                 nullptr, nullptr, b32[4], MLP_OUTP, st, m_dev, r0);
+                 nullptr, nullptr, b32[4], MLP_OUTP, st, m_dev, r0, row_live);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -172,7 +174,9 @@ hipError_t Surrogate::network(FeatArgs fa, const int *list, const unsigned *list
     const int rows = int((long(rows_max) + 63) / 64 * 64);
     hipError_t e = mlp_features(fa, rows, st);
     if (e != hipSuccess) return e;
-    return forward(X, rows, Yh, false, st, list_n);
+    // rows outside the training range go to the FP64 integrator whatever the network says: tiles of
+    // 64 such rows are not evaluated
+    return forward(X, rows, Yh, false, st, list_n, in_range);
 }
 
 hipError_t Surrogate::accept(ApplyArgs ap, const ChemArgs &ca, const int *list, const unsigned *list_n, int rows_max,

@@ -42,10 +42,12 @@ struct Surrogate {
     hipError_t reserve(int rows, bool training);
     hipError_t gemm(const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, int M, int N, int K,
                     int ksplit, int ep, float *C32, uint16_t *C16, float *P32, const float *bias, long ldc,
-                    hipStream_t st, const unsigned *m_dev = nullptr, int m_off = 0) const;
-    // rows: rows allocated in x / out (multiple of 64); m_dev: optional device count of live rows
+                    hipStream_t st, const unsigned *m_dev = nullptr, int m_off = 0,
+                    const int *row_live = nullptr) const;
+    // rows: rows allocated in x / out (multiple of 64); m_dev: optional device count of live rows;
+    // row_live (inference with m_dev): per-row flags, 64-row tiles with none set are not evaluated
     hipError_t forward(const uint16_t *x, int rows, float *out, bool training, hipStream_t st,
-                       const unsigned *m_dev = nullptr);
+                       const unsigned *m_dev = nullptr, const int *row_live = nullptr);
     hipError_t backward(int rows, hipStream_t st);
     hipError_t adam(float lr, hipStream_t st);
     // Surrogate branch of the chemistry source for the cells list[0 .. *list_n) that the explicit
