@@ -131,7 +131,7 @@ struct Shared {
     double rdiag[64];                // 1 / pivot of each elimination step
     int perm[64];
     // the four-wave layout's alternative factorizations; a placeholder element with one wave, so
-    // that the one-wave workgroup's LDS stays small enough for four workgroups per CU
+    // that five one-wave workgroups fit the LDS of a CU
     int rank[NW == 4 ? 64 : 1];                    // elimination step of each row (lu_blocked)
     double U12[NW == 4 ? 4 : 1][NW == 4 ? 1024 : 1];  // per-wave panel rows right of the panel (lu_blocked)
     double lmul[2][NW == 4 ? 64 : 1];              // multipliers of the last two elimination steps (lu_regs)
