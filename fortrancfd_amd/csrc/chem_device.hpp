@@ -136,15 +136,34 @@ __device__ __forceinline__ Rop reaction(const DevKin *__restrict__ K, int i, dou
 }
 
 // reaction() for kernels whose lanes evaluate different rows (the integrator, the rates kernel): the
-// same operations with the same operands, but every exponential a row can need is evaluated before
-// the branches on the row's kind -- the Arrhenius factor, the equilibrium constant, and for fall-off
-// rows the low-pressure factor and the Troe terms -- so their dependent chains run side by side
-// instead of one after another down the lanes' divergent paths.  Values a row does not use are
+// same operations with the same operands, but the third-body sum and every exponential a row can
+// need are evaluated before the branches on the row's kind -- the Arrhenius factor, the equilibrium
+// constant, and for fall-off rows the low-pressure factor and the Troe terms -- so their dependent
+// chains run side by side instead of one after another down the lanes' divergent paths.  Values a row does not use are
 // discarded; a row with B = E = 0 multiplies A by exp(0) = 1 (exact).  Results are bitwise those of
 // reaction().
 template <int STRIDE>
 __device__ __forceinline__ Rop reaction_ilp(const DevKin *__restrict__ K, int i, double T, double alogt, double pfac,
                                             double ctot, const double *__restrict__ c, const double *__restrict__ smh) {
+    // the third-body concentration first: its loads and chain of additions do not depend on the rate
+    // constants, so as straight-line code they overlap the exponentials below.  The first eight
+    // efficiencies are added under selects (past the row's count the previous sum is kept), any
+    // others in a loop: the additions of reaction(), in its order.
+    const int t = K->tb[i];
+    const int tt = t >= 0 ? t : 0;
+    const int nl = t >= 0 ? K->ntbs[tt] : 0;
+    double ctb = ctot;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int tk = K->tbk[tt][u];
+        const double v = ctb + (K->tba[tt][u] - 1.0) * c[(tk >= 0 ? tk : 0) * STRIDE];
+        ctb = u < nl ? v : ctb;
+    }
+    for (int l = 8; l < nl; ++l) {
+        const int tk = K->tbk[tt][l];
+        ctb = ctb + (K->tba[tt][l] - 1.0) * c[(tk >= 0 ? tk : 0) * STRIDE];
+    }
+    if (t < 0) ctb = 1.0;
     const double ekf = fm_exp(K->B[i] * alogt - K->E[i] / T);
     double s = 0.0;
     for (int n = 0; n < 6; ++n) {
@@ -171,26 +190,6 @@ __device__ __forceinline__ Rop reaction_ilp(const DevKin *__restrict__ K, int i,
         double eqk = s != 0.0 ? eks : 1.0;
         eqk = eqk * ipow(pfac, K->nus[i]);
         kr = kf / fmax(eqk, K->small);
-    }
-    double ctb = 1.0;
-    const int t = K->tb[i];
-    if (t >= 0) {
-        ctb = ctot;
-        const int nl = K->ntbs[t];
-        constexpr int TB = 8;
-        for (int l0 = 0; l0 < nl; l0 += TB) {
-            double ta[TB];
-            int tk[TB];
-#pragma unroll
-            for (int u = 0; u < TB; ++u) {
-                const int l = min(l0 + u, MAXTB - 1);
-                ta[u] = K->tba[t][l];
-                tk[u] = K->tbk[t][l];
-            }
-#pragma unroll
-            for (int u = 0; u < TB; ++u)
-                if (l0 + u < nl) ctb = ctb + (ta[u] - 1.0) * c[(tk[u] >= 0 ? tk[u] : 0) * STRIDE];
-        }
     }
     if (f >= 0) {
         const double rklow = fp[0] * erk;
