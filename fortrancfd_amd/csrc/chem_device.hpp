@@ -166,9 +166,11 @@ __device__ __forceinline__ Rop reaction_ilp(const DevKin *__restrict__ K, int i,
     if (t < 0) ctb = 1.0;
     const double ekf = fm_exp(K->B[i] * alogt - K->E[i] / T);
     double s = 0.0;
-    for (int n = 0; n < 6; ++n) {
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {       // empty slots keep the sum (selects, no branches)
         const int k = K->nk[i][n];
-        if (k >= 0) s = s + (double)K->nu[i][n] * smh[k * STRIDE];
+        const double v = s + (double)K->nu[i][n] * smh[(k >= 0 ? k : 0) * STRIDE];
+        s = k >= 0 ? v : s;
     }
     const double eks = fm_exp(fmin(s, K->exparg));
     const int f = K->fo[i];
