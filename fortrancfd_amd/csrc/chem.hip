@@ -333,15 +333,20 @@ __device__ __attribute__((noinline)) RhsOut jacobian(FCFD_GLB const DevThermo *t
                               ? reaction_plain<1>(K, i, T, alogt, pfac, sh.c, sh.smh)
                               : reaction_ilp<1>(K, i, T, alogt, pfac, ctot, sh.c, sh.smh);
             rr[u] = r;
+            // the slots' factors under selects instead of branches (a factor is applied only where
+            // the slot is filled, as before)
+            double pw[6];
+#pragma unroll
+            for (int s = 0; s < 6; ++s) pw[s] = ipow(sh.c[nk[s] >= 0 ? nk[s] : 0], abs(nu[s]));
+#pragma unroll
             for (int s = 0; s < 6; ++s) {
-                dq[u][s] = 0.0;
-                if (nk[s] < 0) continue;
                 const int lo = s < 3 ? 0 : 3;
                 const int a = abs(nu[s]);
-                double d = (double)a * ipow(sh.c[nk[s]], a - 1);
+                double d = (double)a * ipow(sh.c[nk[s] >= 0 ? nk[s] : 0], a - 1);
+#pragma unroll
                 for (int t = lo; t < lo + 3; ++t)
-                    if (t != s && nk[t] >= 0) d = d * ipow(sh.c[nk[t]], abs(nu[t]));
-                dq[u][s] = s < 3 ? r.kf * d : -(r.kr * d);
+                    if (t != s) d = nk[t] >= 0 ? d * pw[t] : d;
+                dq[u][s] = nk[s] < 0 ? 0.0 : (s < 3 ? r.kf * d : -(r.kr * d));
             }
         }
 #pragma unroll
