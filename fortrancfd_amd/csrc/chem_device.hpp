@@ -175,16 +175,12 @@ __device__ __forceinline__ Rop reaction_ilp(const DevKin *__restrict__ K, int i,
     const double eks = fm_exp(fmin(s, K->exparg));
     const int f = K->fo[i];
     const double *fp = K->fpar[f >= 0 ? f : 0];
-    const bool troe = f >= 0 && K->ifop[f] > 2;
-    double erk = 0.0, e4 = 0.0, e5 = 0.0, e6 = 0.0;
-    if (f >= 0) {
-        erk = fm_exp(fp[1] * alogt - fp[2] / T);
-        if (troe) {
-            e4 = fm_exp(-T / fp[4]);
-            e5 = fm_exp(-T / fp[5]);
-            e6 = fm_exp(-fp[6] / T);
-        }
-    }
+    // the fall-off terms in every lane (a row without them reads row 0's parameters and discards the
+    // values), outside any branch, so they interleave with the two exponentials above
+    const double erk = fm_exp(fp[1] * alogt - fp[2] / T);
+    const double e4 = fm_exp(-T / fp[4]);
+    const double e5 = fm_exp(-T / fp[5]);
+    const double e6 = fm_exp(-fp[6] / T);
     double kf = K->A[i];
     if (!K->arr0[i]) kf = kf * ekf;
     double kr = 0.0;
